@@ -1,0 +1,90 @@
+"""ctypes binding of libdwg_hip.so (declarations: include/dwg_hip.h).
+
+The library must have been built (``__graft_entry__.build()``); importing it
+on a machine without the built .so raises instead of falling back to any
+CPU path.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libdwg_hip.so")
+
+DWG_OK = 0
+DWG_ERR_ARG = 1
+DWG_ERR_HIP = 2
+DWG_ERR_CAPACITY = 3
+DWG_ERR_STATE = 4
+
+DWG_SCREENED = 1
+DWG_USE_RADII = 2
+DWG_KAHAN = 4
+DWG_FAR_BF16 = 8
+
+EXPORTS = [
+    "dwg_ctx_create", "dwg_ctx_destroy", "dwg_ctx_error", "dwg_ctx_sync", "dwg_ctx_profile",
+    "dwg_ctx_profile_get", "dwg_ctx_profile_reset", "dwg_gwn_direct", "dwg_octree_build",
+    "dwg_octree_destroy", "dwg_octree_info", "dwg_octree_update", "dwg_gwn_tree", "dwg_gwn_tree_grid",
+    "dwg_gwn_tree_terms", "dwg_gwn_tree_grid_terms", "dwg_knn_build", "dwg_knn_destroy", "dwg_knn_query",
+    "dwg_marching_cubes", "dwg_diffuse_normals", "dwg_radii", "dwg_gauss_map", "dwg_transfer_normals",
+    "dwg_outlier_filter", "dwg_reduce_sum", "dwg_bounds",
+]
+
+
+class DwgGrid(ctypes.Structure):
+    _fields_ = [("origin", ctypes.c_float * 3), ("h", ctypes.c_float), ("res", ctypes.c_int32 * 3)]
+
+
+_vp = ctypes.c_void_p
+_i64 = ctypes.c_int64
+_int = ctypes.c_int
+_u32 = ctypes.c_uint32
+_f = ctypes.c_float
+_lib = None
+
+
+def load():
+    """Load and prototype libdwg_hip.so; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"{LIB_PATH} is missing: build it with __graft_entry__.build() "
+                           "(there is no CPU fallback)")
+    L = ctypes.CDLL(LIB_PATH)
+    P = ctypes.POINTER
+    L.dwg_ctx_create.argtypes = [_int, _vp, P(_vp)]
+    L.dwg_ctx_destroy.argtypes = [_vp]
+    L.dwg_ctx_destroy.restype = None
+    L.dwg_ctx_error.argtypes = [_vp]
+    L.dwg_ctx_error.restype = ctypes.c_char_p
+    L.dwg_ctx_sync.argtypes = [_vp]
+    L.dwg_ctx_profile.argtypes = [_vp, _int]
+    L.dwg_ctx_profile_get.argtypes = [_vp, ctypes.c_char_p, P(ctypes.c_double), P(_i64)]
+    L.dwg_ctx_profile_reset.argtypes = [_vp]
+    L.dwg_gwn_direct.argtypes = [_vp, _vp, _vp, _vp, _vp, _i64, _vp, _i64, _f, _u32, _vp, _vp]
+    L.dwg_octree_build.argtypes = [_vp, _vp, _vp, _i64, _int, _int, P(_vp)]
+    L.dwg_octree_destroy.argtypes = [_vp]
+    L.dwg_octree_destroy.restype = None
+    L.dwg_octree_info.argtypes = [_vp, P(_i64), P(_i64), P(_int)]
+    L.dwg_octree_update.argtypes = [_vp, _vp, _vp, _vp, _vp]
+    L.dwg_gwn_tree.argtypes = [_vp, _vp, _vp, _i64, _f, _f, _u32, _vp]
+    L.dwg_gwn_tree_grid.argtypes = [_vp, _vp, P(DwgGrid), _f, _f, _u32, _vp]
+    L.dwg_gwn_tree_terms.argtypes = [_vp, _vp, _vp, _i64, _f, _vp]
+    L.dwg_gwn_tree_grid_terms.argtypes = [_vp, _vp, P(DwgGrid), _f, P(_i64)]
+    L.dwg_knn_build.argtypes = [_vp, _vp, _i64, _int, P(_vp)]
+    L.dwg_knn_destroy.argtypes = [_vp]
+    L.dwg_knn_destroy.restype = None
+    L.dwg_knn_query.argtypes = [_vp, _vp, _vp, _i64, _int, _vp, _vp, _vp, _vp]
+    L.dwg_marching_cubes.argtypes = [_vp, _vp, P(DwgGrid), _f, _i64, _vp, _vp, _vp, P(_i64)]
+    L.dwg_diffuse_normals.argtypes = [_vp, _vp, _vp, _i64, _vp, _vp, _i64, _int, _int, _int, _vp, P(_i64)]
+    L.dwg_radii.argtypes = [_vp, _vp, _vp, _i64, _int, _f, _f, _vp]
+    L.dwg_gauss_map.argtypes = [_vp, _vp, _i64, P(_f), _vp]
+    L.dwg_transfer_normals.argtypes = [_vp, _vp, _vp, _vp, _i64, _int, _vp]
+    L.dwg_outlier_filter.argtypes = [_vp, _vp, _i64, _f]
+    L.dwg_reduce_sum.argtypes = [_vp, _vp, _i64, P(ctypes.c_double)]
+    L.dwg_bounds.argtypes = [_vp, _vp, _i64, P(_f), P(_f)]
+    _lib = L
+    return L

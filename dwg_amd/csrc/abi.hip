@@ -1,0 +1,314 @@
+// C ABI of libdwg_hip.so: argument checks, exception → status translation,
+// context lifetime.  Declarations and reference citations: include/dwg_hip.h.
+#include <cstring>
+
+#include "common.h"
+#include "diffuse.h"
+#include "gwn.h"
+#include "knn.h"
+#include "mc.h"
+#include "scan.h"
+
+namespace {
+
+template <class F>
+int guarded(dwg_ctx* ctx, F&& fn) {
+    try {
+        fn();
+        if (ctx) ctx->err.clear();
+        return DWG_OK;
+    } catch (const dwg::Error& e) {
+        if (ctx) ctx->err = e.what();
+        return e.code;
+    } catch (const std::exception& e) {
+        if (ctx) ctx->err = e.what();
+        return DWG_ERR_HIP;
+    } catch (...) {
+        if (ctx) ctx->err = "unknown error";
+        return DWG_ERR_HIP;
+    }
+}
+
+void set_device(dwg_ctx* ctx) { DWG_HIP_CHECK(hipSetDevice(ctx->device)); }
+
+}  // namespace
+
+extern "C" {
+
+int dwg_ctx_create(int device, void* stream, dwg_ctx** out) {
+    if (!out) return DWG_ERR_ARG;
+    *out = nullptr;
+    auto* ctx = new dwg_ctx();
+    ctx->device = device;
+    ctx->stream = static_cast<hipStream_t>(stream);
+    int rc = guarded(ctx, [&] {
+        int count = 0;
+        DWG_HIP_CHECK(hipGetDeviceCount(&count));
+        DWG_REQUIRE(device >= 0 && device < count, "no such HIP device");
+        set_device(ctx);
+    });
+    if (rc != DWG_OK) {
+        delete ctx;
+        return rc;
+    }
+    *out = ctx;
+    return DWG_OK;
+}
+
+void dwg_ctx_destroy(dwg_ctx* ctx) {
+    if (!ctx) return;
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    for (auto& kv : ctx->times)
+        for (auto& ab : kv.second.pending) {
+            (void)hipEventDestroy(ab.first);
+            (void)hipEventDestroy(ab.second);
+        }
+    ctx->scratch.clear();
+    if (ctx->pinned) (void)hipHostFree(ctx->pinned);
+    delete ctx;
+}
+
+const char* dwg_ctx_error(const dwg_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+int dwg_ctx_sync(dwg_ctx* ctx) {
+    if (!ctx) return DWG_ERR_ARG;
+    return guarded(ctx, [&] { DWG_HIP_CHECK(hipStreamSynchronize(ctx->stream)); });
+}
+
+int dwg_ctx_profile(dwg_ctx* ctx, int enable) {
+    if (!ctx) return DWG_ERR_ARG;
+    ctx->profile = enable != 0;
+    return DWG_OK;
+}
+
+int dwg_ctx_profile_get(dwg_ctx* ctx, const char* name, double* total_ms, int64_t* launches) {
+    if (!ctx || !name) return DWG_ERR_ARG;
+    return guarded(ctx, [&] {
+        DWG_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+        auto it = ctx->times.find(name);
+        double ms = 0;
+        int64_t cnt = 0;
+        if (it != ctx->times.end()) {
+            auto& kt = it->second;
+            for (auto& ab : kt.pending) {
+                float t = 0;
+                DWG_HIP_CHECK(hipEventElapsedTime(&t, ab.first, ab.second));
+                kt.total_ms += t;
+                kt.launches += 1;
+                (void)hipEventDestroy(ab.first);
+                (void)hipEventDestroy(ab.second);
+            }
+            kt.pending.clear();
+            ms = kt.total_ms;
+            cnt = kt.launches;
+        }
+        if (total_ms) *total_ms = ms;
+        if (launches) *launches = cnt;
+    });
+}
+
+int dwg_ctx_profile_reset(dwg_ctx* ctx) {
+    if (!ctx) return DWG_ERR_ARG;
+    return guarded(ctx, [&] {
+        DWG_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+        for (auto& kv : ctx->times)
+            for (auto& ab : kv.second.pending) {
+                (void)hipEventDestroy(ab.first);
+                (void)hipEventDestroy(ab.second);
+            }
+        ctx->times.clear();
+    });
+}
+
+int dwg_gwn_direct(dwg_ctx* ctx, const float* d_pts, const float* d_nrm, const float* d_area, const float* d_radii,
+                   int64_t n, const float* d_q, int64_t nq, float sigma, uint32_t flags, float* d_out_w,
+                   float* d_out_grad) {
+    if (!ctx) return DWG_ERR_ARG;
+    return guarded(ctx, [&] {
+        DWG_REQUIRE(n == 0 || (d_pts && d_nrm), "null source arrays");
+        DWG_REQUIRE(nq == 0 || (d_q && d_out_w), "null query/output arrays");
+        set_device(ctx);
+        dwg::gwn_direct(ctx, d_pts, d_nrm, d_area, d_radii, n, d_q, nq, sigma, flags, d_out_w, d_out_grad);
+    });
+}
+
+int dwg_octree_build(dwg_ctx* ctx, const double* h_pts, const double* h_area, int64_t n, int max_depth, int leaf_size,
+                     dwg_octree** out) {
+    if (!ctx || !out) return DWG_ERR_ARG;
+    *out = nullptr;
+    return guarded(ctx, [&] {
+        DWG_REQUIRE(h_pts, "null points");
+        set_device(ctx);
+        dwg::HostOctree H = dwg::build_octree_host(h_pts, h_area, n, max_depth, leaf_size);
+        H.P = h_pts;
+        *out = dwg::octree_upload(ctx, H);
+    });
+}
+
+void dwg_octree_destroy(dwg_octree* tree) { delete tree; }
+
+int dwg_octree_info(const dwg_octree* tree, int64_t* n_nodes, int64_t* n_leaves, int* max_level) {
+    if (!tree) return DWG_ERR_ARG;
+    if (n_nodes) *n_nodes = tree->n_nodes;
+    if (n_leaves) *n_leaves = tree->n_leaves;
+    if (max_level) *max_level = tree->max_level;
+    return DWG_OK;
+}
+
+int dwg_octree_update(dwg_ctx* ctx, dwg_octree* tree, const float* d_nrm, const float* d_area, const float* d_radii) {
+    if (!ctx || !tree) return DWG_ERR_ARG;
+    return guarded(ctx, [&] {
+        DWG_REQUIRE(d_nrm, "null normals");
+        set_device(ctx);
+        dwg::octree_update(ctx, tree, d_nrm, d_area, d_radii);
+    });
+}
+
+int dwg_gwn_tree(dwg_ctx* ctx, const dwg_octree* tree, const float* d_q, int64_t nq, float beta, float sigma,
+                 uint32_t flags, float* d_out) {
+    if (!ctx || !tree) return DWG_ERR_ARG;
+    return guarded(ctx, [&] {
+        DWG_REQUIRE(nq == 0 || d_out, "null output");
+        set_device(ctx);
+        dwg::gwn_tree(ctx, tree, d_q, nq, beta, sigma, flags, d_out);
+    });
+}
+
+int dwg_gwn_tree_grid(dwg_ctx* ctx, const dwg_octree* tree, const dwg_grid* grid, float beta, float sigma,
+                      uint32_t flags, float* d_out) {
+    if (!ctx || !tree) return DWG_ERR_ARG;
+    return guarded(ctx, [&] {
+        DWG_REQUIRE(d_out, "null output");
+        set_device(ctx);
+        dwg::gwn_tree_grid(ctx, tree, grid, beta, sigma, flags, d_out);
+    });
+}
+
+int dwg_gwn_tree_terms(dwg_ctx* ctx, const dwg_octree* tree, const float* d_q, int64_t nq, float beta,
+                       int64_t* d_terms) {
+    if (!ctx || !tree) return DWG_ERR_ARG;
+    return guarded(ctx, [&] {
+        DWG_REQUIRE(nq == 0 || d_terms, "null output");
+        set_device(ctx);
+        dwg::gwn_tree_terms(ctx, tree, d_q, nq, beta, d_terms);
+    });
+}
+
+int dwg_gwn_tree_grid_terms(dwg_ctx* ctx, const dwg_octree* tree, const dwg_grid* grid, float beta,
+                            int64_t* h_total_terms) {
+    if (!ctx || !tree || !h_total_terms) return DWG_ERR_ARG;
+    return guarded(ctx, [&] {
+        set_device(ctx);
+        *h_total_terms = dwg::gwn_tree_grid_terms(ctx, tree, grid, beta);
+    });
+}
+
+int dwg_knn_build(dwg_ctx* ctx, const float* d_pts, int64_t n, int k_hint, dwg_knn_index** out) {
+    if (!ctx || !out) return DWG_ERR_ARG;
+    *out = nullptr;
+    return guarded(ctx, [&] {
+        DWG_REQUIRE(d_pts, "null points");
+        set_device(ctx);
+        *out = dwg::knn_build(ctx, d_pts, n, k_hint);
+    });
+}
+
+void dwg_knn_destroy(dwg_knn_index* index) { delete index; }
+
+int dwg_knn_query(dwg_ctx* ctx, const dwg_knn_index* index, const float* d_q, int64_t nq, int k, const uint8_t* d_skip,
+                  const uint8_t* d_qmask, int32_t* d_idx, float* d_d2) {
+    if (!ctx || !index) return DWG_ERR_ARG;
+    return guarded(ctx, [&] {
+        DWG_REQUIRE(nq == 0 || (d_q && d_idx), "null query/output arrays");
+        set_device(ctx);
+        dwg::knn_query(ctx, index, d_q, nq, k, d_skip, d_qmask, d_idx, d_d2);
+    });
+}
+
+int dwg_marching_cubes(dwg_ctx* ctx, const float* d_field, const dwg_grid* grid, float iso, int64_t capacity,
+                       float* d_tris, float* d_cent, float* d_nrm, int64_t* h_count) {
+    if (!ctx) return DWG_ERR_ARG;
+    try {
+        DWG_REQUIRE(d_field && grid, "null field/grid");
+        set_device(ctx);
+        int64_t t = dwg::marching_cubes(ctx, d_field, grid, iso, capacity, d_tris, d_cent, d_nrm);
+        if (h_count) *h_count = t;
+        ctx->err.clear();
+        return DWG_OK;
+    } catch (const dwg::Error& e) {
+        ctx->err = e.what();
+        if (e.code == DWG_ERR_CAPACITY && h_count) *h_count = e.value;
+        return e.code;
+    } catch (const std::exception& e) {
+        ctx->err = e.what();
+        return DWG_ERR_HIP;
+    }
+}
+
+int dwg_diffuse_normals(dwg_ctx* ctx, const dwg_knn_index* src_index, const float* d_src, int64_t n_src,
+                        const float* d_cent, const float* d_tri_nrm, int64_t ntri, int k, int zero_k,
+                        int zero_threshold, float* d_nrm, int64_t* h_nzero) {
+    if (!ctx || !src_index) return DWG_ERR_ARG;
+    return guarded(ctx, [&] {
+        DWG_REQUIRE(d_src && d_nrm && (ntri == 0 || (d_cent && d_tri_nrm)), "null arrays");
+        set_device(ctx);
+        dwg::diffuse_normals(ctx, src_index, d_src, n_src, d_cent, d_tri_nrm, ntri, k, zero_k, zero_threshold, d_nrm,
+                             h_nzero);
+    });
+}
+
+int dwg_radii(dwg_ctx* ctx, const dwg_knn_index* index, const float* d_pts, int64_t n, int k, float rmin, float rmax,
+              float* d_out) {
+    if (!ctx || !index) return DWG_ERR_ARG;
+    return guarded(ctx, [&] {
+        DWG_REQUIRE(k >= 1 && k <= 32, "k must be in [1, 32]");
+        set_device(ctx);
+        dwg::radii(ctx, index, d_pts, n, k, rmin, rmax, d_out);
+    });
+}
+
+int dwg_gauss_map(dwg_ctx* ctx, const float* d_pts, int64_t n, const float* h_center, float* d_nrm) {
+    if (!ctx) return DWG_ERR_ARG;
+    return guarded(ctx, [&] {
+        DWG_REQUIRE(h_center && (n == 0 || (d_pts && d_nrm)), "null arrays");
+        set_device(ctx);
+        dwg::gauss_map(ctx, d_pts, n, h_center, d_nrm);
+    });
+}
+
+int dwg_transfer_normals(dwg_ctx* ctx, const dwg_knn_index* src_index, const float* d_src_nrm, const float* d_q,
+                         int64_t nq, int k, float* d_out_nrm) {
+    if (!ctx || !src_index) return DWG_ERR_ARG;
+    return guarded(ctx, [&] {
+        DWG_REQUIRE(k >= 1 && k <= 32, "k must be in [1, 32]");
+        set_device(ctx);
+        dwg::transfer_normals(ctx, src_index, d_src_nrm, d_q, nq, k, d_out_nrm);
+    });
+}
+
+int dwg_outlier_filter(dwg_ctx* ctx, float* d_area, int64_t n, float threshold) {
+    if (!ctx) return DWG_ERR_ARG;
+    return guarded(ctx, [&] {
+        set_device(ctx);
+        dwg::outlier_filter(ctx, d_area, n, threshold);
+    });
+}
+
+int dwg_reduce_sum(dwg_ctx* ctx, const float* d_vals, int64_t n, double* h_sum) {
+    if (!ctx || !h_sum) return DWG_ERR_ARG;
+    return guarded(ctx, [&] {
+        set_device(ctx);
+        *h_sum = n > 0 ? dwg::reduce_sum_f32(ctx, d_vals, n) : 0.0;
+    });
+}
+
+int dwg_bounds(dwg_ctx* ctx, const float* d_pts, int64_t n, float* h_lo, float* h_hi) {
+    if (!ctx || !h_lo || !h_hi) return DWG_ERR_ARG;
+    return guarded(ctx, [&] {
+        set_device(ctx);
+        dwg::bounds(ctx, d_pts, n, h_lo, h_hi);
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,26 @@
+// Exact kNN on a uniform cell grid.
+#pragma once
+#include "common.h"
+
+namespace dwg {
+struct KnnGridArgs {
+    float lo[3];
+    float cell, inv_cell;
+    int dim[3];
+};
+}  // namespace dwg
+
+struct dwg_knn_index {
+    int64_t n = 0, ncells = 0;
+    dwg::KnnGridArgs G{};
+    int64_t* cell_start = nullptr;  // ncells + 1
+    float4* pts = nullptr;          // (x, y, z, index bits) bucketed by cell
+    ~dwg_knn_index();
+};
+
+namespace dwg {
+void bounds(dwg_ctx* ctx, const float* p, int64_t n, float* lo, float* hi);
+dwg_knn_index* knn_build(dwg_ctx* ctx, const float* p, int64_t n, int k_hint);
+void knn_query(dwg_ctx* ctx, const dwg_knn_index* I, const float* q, int64_t nq, int k, const uint8_t* skip,
+               const uint8_t* qmask, int32_t* out_idx, float* out_d2);
+}  // namespace dwg

@@ -1,0 +1,10 @@
+// Marching cubes.
+#pragma once
+#include "common.h"
+
+namespace dwg {
+// Returns the triangle count; throws Error(DWG_ERR_CAPACITY, value = count)
+// when it exceeds capacity (nothing written then).
+int64_t marching_cubes(dwg_ctx* ctx, const float* field, const dwg_grid* g, float iso, int64_t capacity, float* tris,
+                       float* cent, float* nrm);
+}  // namespace dwg

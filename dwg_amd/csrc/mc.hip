@@ -1,0 +1,176 @@
+// Marching cubes with wavefront prefix-sum compaction.
+//
+// The reference launches one 12-thread block per cube and appends triangles
+// through a global atomic counter into a buffer of max(R³/2, 1e6) triangles
+// (TurboMC.cu:383-478, 572-638), so only 12 of 64 lanes work and the output
+// order is unordered.  Here one lane owns one cube (x fastest, coalesced
+// field reads) and the launch runs twice: a count pass writes one triangle
+// count per 256-cube tile, a scan turns tile counts into offsets, and an
+// emit pass recomputes the cube case, takes its in-tile offset from a block
+// prefix sum and writes triangles in cube order.  The result is exact-size
+// and deterministic.  Corner convention, edge interpolation, table and the
+// negated face normal follow TurboMC.cu:399-470.
+#include "mc.h"
+#include "mc_table.h"
+#include "scan.h"
+
+namespace dwg {
+
+__constant__ uint64_t c_mc_rows[256];
+static bool g_mc_table_ready[64] = {false};
+
+static void ensure_table(dwg_ctx* ctx) {
+    if (ctx->device >= 0 && ctx->device < 64 && g_mc_table_ready[ctx->device]) return;
+    static const dwg_mc::PackedTable tab = dwg_mc::make_packed_table();
+    DWG_HIP_CHECK(hipMemcpyToSymbol(HIP_SYMBOL(c_mc_rows), tab.row, sizeof(tab.row)));
+    if (ctx->device >= 0 && ctx->device < 64) g_mc_table_ready[ctx->device] = true;
+}
+
+constexpr int kMcThreads = 256;
+
+struct McArgs {
+    const float* f;
+    int rx, ry, rz;
+    int64_t ncubes;
+    float ox, oy, oz, h, iso;
+};
+
+__device__ __forceinline__ int cube_case(const McArgs& A, int x, int y, int z, float (&v)[8]) {
+    int c = 0;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+        const int mx = x + dwg_mc::corner_dx(i), my = y + dwg_mc::corner_dy(i), mz = z + dwg_mc::corner_dz(i);
+        v[i] = A.f[int64_t(mx) + int64_t(A.rx) * (int64_t(my) + int64_t(A.ry) * mz)];
+        c |= (v[i] > A.iso) ? (1 << i) : 0;
+    }
+    return c;
+}
+
+__device__ __forceinline__ void cube_of(const McArgs& A, int64_t id, int& x, int& y, int& z) {
+    const int64_t nx = A.rx - 1, ny = A.ry - 1;
+    x = int(id % nx);
+    const int64_t t = id / nx;
+    y = int(t % ny);
+    z = int(t / ny);
+}
+
+__device__ int block_exclusive_sum(int v, int& total) {
+    __shared__ int lds[kMcThreads];
+    __shared__ int wsum[kMcThreads / 64];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    int x = v;
+    lds[threadIdx.x] = x;
+    __syncthreads();
+    for (int off = 1; off < 64; off <<= 1) {
+        const int y = lane >= off ? lds[threadIdx.x - off] : 0;
+        __syncthreads();
+        x += y;
+        lds[threadIdx.x] = x;
+        __syncthreads();
+    }
+    if (lane == 63) wsum[wave] = x;
+    __syncthreads();
+    int before = 0;
+    total = 0;
+    for (int w = 0; w < kMcThreads / 64; ++w) {
+        if (w < wave) before += wsum[w];
+        total += wsum[w];
+    }
+    return before + x - v;
+}
+
+__global__ __launch_bounds__(kMcThreads) void mc_count_kernel(McArgs A, int64_t* __restrict__ tile_counts) {
+    const int64_t id = int64_t(blockIdx.x) * kMcThreads + threadIdx.x;
+    int cnt = 0;
+    if (id < A.ncubes) {
+        int x, y, z;
+        cube_of(A, id, x, y, z);
+        float v[8];
+        cnt = int(c_mc_rows[cube_case(A, x, y, z, v)] >> 60);
+    }
+    int total;
+    (void)block_exclusive_sum(cnt, total);
+    if (threadIdx.x == 0) tile_counts[blockIdx.x] = total;
+}
+
+__global__ __launch_bounds__(kMcThreads) void mc_emit_kernel(McArgs A, const int64_t* __restrict__ tile_offsets,
+                                                             float* __restrict__ tris, float* __restrict__ cent,
+                                                             float* __restrict__ nrm) {
+    const int64_t id = int64_t(blockIdx.x) * kMcThreads + threadIdx.x;
+    int cnt = 0, x = 0, y = 0, z = 0, cs = 0;
+    float v[8];
+    if (id < A.ncubes) {
+        cube_of(A, id, x, y, z);
+        cs = cube_case(A, x, y, z, v);
+        cnt = int(c_mc_rows[cs] >> 60);
+    }
+    int total;
+    const int64_t base = tile_offsets[blockIdx.x] + block_exclusive_sum(cnt, total);
+    if (cnt == 0) return;
+    const uint64_t row = c_mc_rows[cs];
+    for (int t = 0; t < cnt; ++t) {
+        float P[3][3];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            const int e = int((row >> (4 * (3 * t + k))) & 0xF);
+            const int a = dwg_mc::edge_a(e), b = dwg_mc::edge_b(e);
+            const float pa[3] = {__builtin_fmaf(float(x + dwg_mc::corner_dx(a)), A.h, A.ox),
+                                 __builtin_fmaf(float(y + dwg_mc::corner_dy(a)), A.h, A.oy),
+                                 __builtin_fmaf(float(z + dwg_mc::corner_dz(a)), A.h, A.oz)};
+            const float pb[3] = {__builtin_fmaf(float(x + dwg_mc::corner_dx(b)), A.h, A.ox),
+                                 __builtin_fmaf(float(y + dwg_mc::corner_dy(b)), A.h, A.oy),
+                                 __builtin_fmaf(float(z + dwg_mc::corner_dz(b)), A.h, A.oz)};
+            // P = P1 − ell (P2 − P1), ell = (a1 − iso)/(a2 − a1 + 1e-10)
+            const float ell = (v[a] - A.iso) / (v[b] - v[a] + 1e-10f);
+#pragma unroll
+            for (int d = 0; d < 3; ++d) P[k][d] = pa[d] - ell * (pb[d] - pa[d]);
+        }
+        const int64_t o = base + t;
+        if (tris) {
+#pragma unroll
+            for (int k = 0; k < 3; ++k)
+#pragma unroll
+                for (int d = 0; d < 3; ++d) tris[9 * o + 3 * k + d] = P[k][d];
+        }
+        if (cent) {
+#pragma unroll
+            for (int d = 0; d < 3; ++d) cent[3 * o + d] = (P[0][d] + P[1][d] + P[2][d]) / 3.0f;
+        }
+        if (nrm) {
+            const float u[3] = {P[1][0] - P[0][0], P[1][1] - P[0][1], P[1][2] - P[0][2]};
+            const float w[3] = {P[2][0] - P[0][0], P[2][1] - P[0][1], P[2][2] - P[0][2]};
+            const float c[3] = {u[1] * w[2] - u[2] * w[1], u[2] * w[0] - u[0] * w[2], u[0] * w[1] - u[1] * w[0]};
+            const float s = -1.0f / (sqrtf(c[0] * c[0] + c[1] * c[1] + c[2] * c[2]) + 1e-8f);
+#pragma unroll
+            for (int d = 0; d < 3; ++d) nrm[3 * o + d] = c[d] * s;
+        }
+    }
+}
+
+int64_t marching_cubes(dwg_ctx* ctx, const float* field, const dwg_grid* g, float iso, int64_t capacity, float* tris,
+                       float* cent, float* nrm) {
+    DWG_REQUIRE(g && g->res[0] >= 2 && g->res[1] >= 2 && g->res[2] >= 2, "grid needs at least 2 samples per axis");
+    ensure_table(ctx);
+    McArgs A{field, g->res[0], g->res[1], g->res[2], 0, g->origin[0], g->origin[1], g->origin[2], g->h, iso};
+    A.ncubes = int64_t(A.rx - 1) * (A.ry - 1) * (A.rz - 1);
+    const int64_t tiles = (A.ncubes + kMcThreads - 1) / kMcThreads;
+    int64_t* offs = static_cast<int64_t*>(ctx->buf("mc_tiles", sizeof(int64_t) * (tiles + 1)));
+    {
+        ProfileScope ps(ctx, "mc_count");
+        hipLaunchKernelGGL(mc_count_kernel, dim3(grid_for(A.ncubes, kMcThreads)), dim3(kMcThreads), 0, ctx->stream, A,
+                           offs);
+        check_launch();
+        exclusive_scan<int64_t>(ctx, offs, tiles);
+    }
+    const int64_t total = read_scalar<int64_t>(ctx, offs + tiles);
+    if (total > capacity) throw Error(DWG_ERR_CAPACITY, "marching cubes output exceeds capacity", total);
+    if (total > 0 && (tris || cent || nrm)) {
+        ProfileScope ps(ctx, "mc_emit");
+        hipLaunchKernelGGL(mc_emit_kernel, dim3(grid_for(A.ncubes, kMcThreads)), dim3(kMcThreads), 0, ctx->stream, A,
+                           offs, tris, cent, nrm);
+        check_launch();
+    }
+    return total;
+}
+
+}  // namespace dwg

@@ -1,0 +1,22 @@
+// Device-wide scans and reductions.
+#pragma once
+#include "common.h"
+
+namespace dwg {
+
+struct Double3 {
+    double x, y, z;  // trivial default ctor (LDS arrays); Double3() value-initialises to 0
+    Double3() = default;
+    __host__ __device__ Double3(double a, double b, double c) : x(a), y(b), z(c) {}
+    __host__ __device__ Double3 operator+(const Double3& o) const { return Double3(x + o.x, y + o.y, z + o.z); }
+    __host__ __device__ Double3 operator-(const Double3& o) const { return Double3(x - o.x, y - o.y, z - o.z); }
+};
+
+// In-place exclusive scan of d[0..n); d must hold n+1 elements, d[n] = total.
+template <class T>
+void exclusive_scan(dwg_ctx* ctx, T* d, int64_t n, int level = 0);
+
+// FP64 sum of a float array (synchronises, returns the host value).
+double reduce_sum_f32(dwg_ctx* ctx, const float* d, int64_t n);
+
+}  // namespace dwg

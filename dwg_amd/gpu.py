@@ -1,0 +1,263 @@
+"""Thin Python mirror of the C ABI over torch device tensors.
+
+torch only provides device memory and the stream; every computation is a
+call into libdwg_hip.so.  Names follow the reference's operators:
+``Octree.update`` ↔ TurboLibiGWN::UpdateMassNormals, ``Octree.eval`` ↔
+TurboLibiScreenedGWNLaunch, ``KnnIndex`` ↔ GenerateKDTreeForDWG +
+CudaKNNWithKDTreeforDWG, ``marching_cubes`` ↔ TurboMC::TurboMarchingCubes.
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+import torch
+
+from . import _lib
+from ._lib import DWG_FAR_BF16, DWG_KAHAN, DWG_SCREENED, DWG_USE_RADII, DwgGrid
+
+
+class DwgError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"dwg error {code}: {msg}")
+        self.code = code
+
+
+def _ptr(t):
+    return None if t is None else ctypes.c_void_p(t.data_ptr())
+
+
+def _f32(t, device):
+    if isinstance(t, np.ndarray):
+        t = torch.from_numpy(np.ascontiguousarray(t, dtype=np.float32))
+    return t.to(device=device, dtype=torch.float32).contiguous()
+
+
+class Context:
+    """dwg_ctx on one device, issuing work on torch's current stream."""
+
+    def __init__(self, device: int = 0):
+        if not torch.cuda.is_available():
+            raise RuntimeError("libdwg_hip needs a HIP device")
+        self.L = _lib.load()
+        self.device = torch.device("cuda", device)
+        stream = torch.cuda.current_stream(self.device).cuda_stream
+        h = ctypes.c_void_p()
+        rc = self.L.dwg_ctx_create(int(device), ctypes.c_void_p(stream), ctypes.byref(h))
+        if rc != 0:
+            raise DwgError(rc, "dwg_ctx_create failed")
+        self.h = h
+
+    def check(self, rc):
+        if rc != 0:
+            raise DwgError(rc, self.L.dwg_ctx_error(self.h).decode())
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.dwg_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def sync(self):
+        self.check(self.L.dwg_ctx_sync(self.h))
+
+    def profile(self, enable=True):
+        self.check(self.L.dwg_ctx_profile(self.h, 1 if enable else 0))
+
+    def profile_get(self, name):
+        ms = ctypes.c_double()
+        n = ctypes.c_int64()
+        self.check(self.L.dwg_ctx_profile_get(self.h, name.encode(), ctypes.byref(ms), ctypes.byref(n)))
+        return ms.value, n.value
+
+    def profile_reset(self):
+        self.check(self.L.dwg_ctx_profile_reset(self.h))
+
+    def f32(self, x):
+        return _f32(x, self.device)
+
+    # ---- sums ---------------------------------------------------------------
+    def gwn_direct(self, pts, nrm, area, q, sigma=10.0, screened=False, radii=None, kahan=True, grad=False):
+        pts, nrm, area, q = self.f32(pts), self.f32(nrm), self.f32(area), self.f32(q)
+        rad = None if radii is None else self.f32(radii)
+        flags = (DWG_SCREENED if screened else 0) | (DWG_USE_RADII if rad is not None else 0) | \
+                (DWG_KAHAN if kahan else 0)
+        nq = q.numel() // 3
+        w = torch.empty(nq, device=self.device, dtype=torch.float32)
+        g = torch.empty((nq, 3), device=self.device, dtype=torch.float32) if grad else None
+        self.check(self.L.dwg_gwn_direct(self.h, _ptr(pts), _ptr(nrm), _ptr(area), _ptr(rad), pts.numel() // 3,
+                                         _ptr(q), nq, float(sigma), flags, _ptr(w), _ptr(g)))
+        return (w, g) if grad else w
+
+    def reduce_sum(self, v):
+        v = self.f32(v)
+        out = ctypes.c_double()
+        self.check(self.L.dwg_reduce_sum(self.h, _ptr(v), v.numel(), ctypes.byref(out)))
+        return out.value
+
+    def bounds(self, pts):
+        pts = self.f32(pts)
+        lo = (ctypes.c_float * 3)()
+        hi = (ctypes.c_float * 3)()
+        self.check(self.L.dwg_bounds(self.h, _ptr(pts), pts.numel() // 3, lo, hi))
+        return np.array(lo[:], np.float32), np.array(hi[:], np.float32)
+
+
+def make_grid(origin, h, res) -> DwgGrid:
+    g = DwgGrid()
+    for d in range(3):
+        g.origin[d] = float(origin[d])
+        g.res[d] = int(res[d])
+    g.h = float(h)
+    return g
+
+
+class Octree:
+    """dwg_octree: octreeDWG split rule + order-0 moments, resident on the GPU."""
+
+    def __init__(self, ctx: Context, pts, area=None, max_depth=10, leaf_size=1):
+        self.ctx = ctx
+        P = np.ascontiguousarray(np.asarray(pts, dtype=np.float64).reshape(-1, 3))
+        A = None if area is None else np.ascontiguousarray(np.asarray(area, dtype=np.float64).reshape(-1))
+        self.n = len(P)
+        h = ctypes.c_void_p()
+        ctx.check(ctx.L.dwg_octree_build(ctx.h, P.ctypes.data, None if A is None else A.ctypes.data, self.n,
+                                         int(max_depth), int(leaf_size), ctypes.byref(h)))
+        self.h = h
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            self.ctx.L.dwg_octree_destroy(self.h)
+            self.h = None
+
+    def info(self):
+        m, l, d = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int()
+        self.ctx.L.dwg_octree_info(self.h, ctypes.byref(m), ctypes.byref(l), ctypes.byref(d))
+        return {"nodes": m.value, "leaves": l.value, "depth": d.value}
+
+    def update(self, nrm, area, radii=None):
+        c = self.ctx
+        self._nrm, self._area = c.f32(nrm), c.f32(area)
+        self._radii = None if radii is None else c.f32(radii)
+        c.check(c.L.dwg_octree_update(c.h, self.h, _ptr(self._nrm), _ptr(self._area), _ptr(self._radii)))
+
+    def _flags(self, screened, use_radii, kahan, bf16):
+        return (DWG_SCREENED if screened else 0) | (DWG_USE_RADII if use_radii else 0) | \
+               (DWG_KAHAN if kahan else 0) | (DWG_FAR_BF16 if bf16 else 0)
+
+    def eval(self, q, beta=2.3, sigma=10.0, screened=True, kahan=False, bf16=False, out=None):
+        c = self.ctx
+        q = c.f32(q)
+        nq = q.numel() // 3
+        out = torch.empty(nq, device=c.device, dtype=torch.float32) if out is None else out
+        c.check(c.L.dwg_gwn_tree(c.h, self.h, _ptr(q), nq, float(beta), float(sigma),
+                                 self._flags(screened, False, kahan, bf16), _ptr(out)))
+        return out
+
+    def eval_grid(self, grid: DwgGrid, beta=2.3, sigma=10.0, screened=True, kahan=False, bf16=False, out=None):
+        c = self.ctx
+        G = grid.res[0] * grid.res[1] * grid.res[2]
+        out = torch.empty(G, device=c.device, dtype=torch.float32) if out is None else out
+        c.check(c.L.dwg_gwn_tree_grid(c.h, self.h, ctypes.byref(grid), float(beta), float(sigma),
+                                      self._flags(screened, False, kahan, bf16), _ptr(out)))
+        return out
+
+    def terms(self, q, beta=2.3):
+        c = self.ctx
+        q = c.f32(q)
+        nq = q.numel() // 3
+        out = torch.empty(nq, device=c.device, dtype=torch.int64)
+        c.check(c.L.dwg_gwn_tree_terms(c.h, self.h, _ptr(q), nq, float(beta), _ptr(out)))
+        return out
+
+    def grid_terms(self, grid: DwgGrid, beta=2.3):
+        c = self.ctx
+        tot = ctypes.c_int64()
+        c.check(c.L.dwg_gwn_tree_grid_terms(c.h, self.h, ctypes.byref(grid), float(beta), ctypes.byref(tot)))
+        return tot.value
+
+
+class KnnIndex:
+    """dwg_knn_index: exact kNN, (d², index) order."""
+
+    def __init__(self, ctx: Context, pts, k_hint=15):
+        self.ctx = ctx
+        self.pts = ctx.f32(pts)
+        self.n = self.pts.numel() // 3
+        h = ctypes.c_void_p()
+        ctx.check(ctx.L.dwg_knn_build(ctx.h, _ptr(self.pts), self.n, int(k_hint), ctypes.byref(h)))
+        self.h = h
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            self.ctx.L.dwg_knn_destroy(self.h)
+            self.h = None
+
+    def query(self, q, k, skip=None, qmask=None, d2=False):
+        c = self.ctx
+        q = c.f32(q)
+        nq = q.numel() // 3
+        idx = torch.full((nq, k), -1, device=c.device, dtype=torch.int32)
+        dd = torch.empty((nq, k), device=c.device, dtype=torch.float32) if d2 else None
+        sk = None if skip is None else skip.to(device=c.device, dtype=torch.uint8).contiguous()
+        qm = None if qmask is None else qmask.to(device=c.device, dtype=torch.uint8).contiguous()
+        c.check(c.L.dwg_knn_query(c.h, self.h, _ptr(q), nq, int(k), _ptr(sk), _ptr(qm), _ptr(idx), _ptr(dd)))
+        return (idx, dd) if d2 else idx
+
+
+def marching_cubes(ctx: Context, field, grid: DwgGrid, iso, tris=False):
+    """→ (tris or None, centroids, normals) as device tensors, cube order."""
+    field = ctx.f32(field)
+    cnt = ctypes.c_int64()
+    rc = ctx.L.dwg_marching_cubes(ctx.h, _ptr(field), ctypes.byref(grid), float(iso), 0, None, None, None,
+                                  ctypes.byref(cnt))
+    if rc not in (0, _lib.DWG_ERR_CAPACITY):
+        ctx.check(rc)
+    T = cnt.value
+    t = torch.empty((T, 3, 3), device=ctx.device, dtype=torch.float32) if tris else None
+    c = torch.empty((T, 3), device=ctx.device, dtype=torch.float32)
+    n = torch.empty((T, 3), device=ctx.device, dtype=torch.float32)
+    ctx.check(ctx.L.dwg_marching_cubes(ctx.h, _ptr(field), ctypes.byref(grid), float(iso), T, _ptr(t), _ptr(c),
+                                       _ptr(n), ctypes.byref(cnt)))
+    return t, c, n
+
+
+def diffuse_normals(ctx: Context, index: KnnIndex, cent, tri_nrm, nrm, k=15, zero_k=10, zero_threshold=10):
+    """In-place update of nrm (n×3 float32 device tensor); returns the zero count."""
+    cent, tri_nrm = ctx.f32(cent), ctx.f32(tri_nrm)
+    nz = ctypes.c_int64()
+    ctx.check(ctx.L.dwg_diffuse_normals(ctx.h, index.h, _ptr(index.pts), index.n, _ptr(cent), _ptr(tri_nrm),
+                                        cent.numel() // 3, int(k), int(zero_k), int(zero_threshold), _ptr(nrm),
+                                        ctypes.byref(nz)))
+    return nz.value
+
+
+def radii(ctx: Context, index: KnnIndex, k=10, rmin=0.0015, rmax=0.015):
+    out = torch.empty(index.n, device=ctx.device, dtype=torch.float32)
+    ctx.check(ctx.L.dwg_radii(ctx.h, index.h, _ptr(index.pts), index.n, int(k), float(rmin), float(rmax), _ptr(out)))
+    return out
+
+
+def gauss_map(ctx: Context, pts, center):
+    pts = ctx.f32(pts)
+    out = torch.empty_like(pts)
+    c = (ctypes.c_float * 3)(*[float(v) for v in center])
+    ctx.check(ctx.L.dwg_gauss_map(ctx.h, _ptr(pts), pts.numel() // 3, c, _ptr(out)))
+    return out
+
+
+def transfer_normals(ctx: Context, index: KnnIndex, src_nrm, q, k=10):
+    src_nrm, q = ctx.f32(src_nrm), ctx.f32(q)
+    out = torch.empty_like(q)
+    ctx.check(ctx.L.dwg_transfer_normals(ctx.h, index.h, _ptr(src_nrm), _ptr(q), q.numel() // 3, int(k), _ptr(out)))
+    return out
+
+
+def outlier_filter(ctx: Context, area, threshold):
+    ctx.check(ctx.L.dwg_outlier_filter(ctx.h, _ptr(area), area.numel(), float(threshold)))
+    return area

@@ -1,0 +1,160 @@
+/*
+ * dwg_hip.h — C ABI of the MI355X-native DWG (diffusing winding gradients)
+ * reconstruction library (libdwg_hip.so).
+ *
+ * Plain C: opaque handles, plain pointers, int64 sizes, int status codes.
+ * Device pointers ("d_") are HIP device memory owned by the caller; host
+ * pointers ("h_") are ordinary host memory.  Point/normal arrays are packed
+ * float triples (x,y,z per element, 12-byte stride), the layout of the
+ * reference's float3 buffers.  Every call is asynchronous on the context's
+ * stream unless it returns a host value.
+ *
+ * Status: 0 = DWG_OK; otherwise an error code, and dwg_ctx_error() returns
+ * the message.  No call exits or throws across the ABI (the reference
+ * aborts via gpuErrchk → exit(), src/TurboDWG.cu:3-12).
+ *
+ * Each entry point names the reference interface it replaces.
+ */
+#ifndef DWG_HIP_H
+#define DWG_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define DWG_OK 0
+#define DWG_ERR_ARG 1       /* invalid argument or size */
+#define DWG_ERR_HIP 2       /* HIP runtime error */
+#define DWG_ERR_CAPACITY 3  /* output buffer too small; required size returned */
+#define DWG_ERR_STATE 4     /* object used in the wrong state */
+
+/* flags for the winding-number sums */
+#define DWG_SCREENED 1u   /* screened kernel (TurboLibiGWN.cu:552-585); else plain libigl kernel */
+#define DWG_USE_RADII 2u  /* screening radius per source; else 0.0015 for every source */
+#define DWG_KAHAN 4u      /* compensated (Kahan) accumulation of per-tile partial sums */
+#define DWG_FAR_BF16 8u   /* far-field dipole moments stored and read as bf16 (tree only) */
+
+typedef struct dwg_ctx dwg_ctx;
+typedef struct dwg_octree dwg_octree;
+typedef struct dwg_knn_index dwg_knn_index;
+typedef struct dwg_mlp dwg_mlp;
+
+/* Implicit regular grid: point (i,j,k) = origin + (i,j,k)·h, linear index
+ * i + res[0]·(j + res[1]·k) (igl::grid order, grid.cpp:28-42). */
+typedef struct dwg_grid {
+    float origin[3];
+    float h;
+    int32_t res[3];
+} dwg_grid;
+
+/* ---- context ------------------------------------------------------------ */
+/* stream: a hipStream_t, or NULL for the default stream. */
+int dwg_ctx_create(int device, void* stream, dwg_ctx** out);
+void dwg_ctx_destroy(dwg_ctx* ctx);
+const char* dwg_ctx_error(const dwg_ctx* ctx);
+int dwg_ctx_sync(dwg_ctx* ctx);
+/* Per-kernel timing with HIP events on the context stream (off by default). */
+int dwg_ctx_profile(dwg_ctx* ctx, int enable);
+/* Sum of recorded durations (ms) and launch count of kernel `name`; syncs. */
+int dwg_ctx_profile_get(dwg_ctx* ctx, const char* name, double* total_ms, int64_t* launches);
+int dwg_ctx_profile_reset(dwg_ctx* ctx);
+
+/* ---- winding-number sums ------------------------------------------------ */
+/* Direct all-pairs sums w(q) = Σ_j term(p_j − q) over n sources at nq
+ * queries; replaces the β ≤ 0 branch of igl::fast_winding_number
+ * (fast_winding_number.cpp:313-323) and, with DWG_SCREENED, the per-point
+ * term of ScreenedGWNLibiPStoQ (TurboLibiGWN.cu:544-585).
+ * d_radii may be NULL unless DWG_USE_RADII.  d_out_grad (nullable, nq×3)
+ * receives ∇_q w of the plain kernel (new; plain kernel only). */
+int dwg_gwn_direct(dwg_ctx* ctx, const float* d_pts, const float* d_nrm, const float* d_area,
+                   const float* d_radii, int64_t n, const float* d_q, int64_t nq, float sigma,
+                   uint32_t flags, float* d_out_w, float* d_out_grad);
+
+/* Octree over sources: octreeDWG split rule (a node with more than
+ * leaf_size points and depth < max_depth is split into 8 children,
+ * octree.cpp:171-335; the reference uses leaf_size 1) plus the order-0
+ * moments CM, R (fast_winding_number.cpp:54-119) and the GPU upload
+ * (TurboLibiGWN::PreLibiGWN, TurboLibiGWN.cu:132-231).  h_pts: host
+ * doubles (n×3); h_area: host doubles (n) or NULL for the constant 2e-6
+ * (DWGProcess.h:619).  Point order of every other call stays the caller's. */
+int dwg_octree_build(dwg_ctx* ctx, const double* h_pts, const double* h_area, int64_t n,
+                     int max_depth, int leaf_size, dwg_octree** out);
+void dwg_octree_destroy(dwg_octree* tree);
+int dwg_octree_info(const dwg_octree* tree, int64_t* n_nodes, int64_t* n_leaves, int* max_level);
+/* Per-iteration refresh of the far-field moment EC = Σ A·n per node and of
+ * the packed sources (TurboLibiGWN::UpdateMassNormals, TurboLibiGWN.cu:1051-1059).
+ * d_radii may be NULL. */
+int dwg_octree_update(dwg_ctx* ctx, dwg_octree* tree, const float* d_nrm, const float* d_area,
+                      const float* d_radii);
+/* Barnes-Hut sums at nq query points (TurboLibiGWN::TurboLibiScreenedGWNLaunch,
+ * TurboLibiGWN.cu:912-922; plain: TurboLibiGWNLaunch :900-910). */
+int dwg_gwn_tree(dwg_ctx* ctx, const dwg_octree* tree, const float* d_q, int64_t nq, float beta,
+                 float sigma, uint32_t flags, float* d_out);
+/* Barnes-Hut sums at every point of an implicit grid (the reference
+ * materialises the grid and calls the launch above, TurboDWG.cu:537). */
+int dwg_gwn_tree_grid(dwg_ctx* ctx, const dwg_octree* tree, const dwg_grid* grid, float beta,
+                      float sigma, uint32_t flags, float* d_out);
+/* Exact number of terms (point terms + dipoles) each query takes (new). */
+int dwg_gwn_tree_terms(dwg_ctx* ctx, const dwg_octree* tree, const float* d_q, int64_t nq,
+                       float beta, int64_t* d_terms);
+int dwg_gwn_tree_grid_terms(dwg_ctx* ctx, const dwg_octree* tree, const dwg_grid* grid,
+                            float beta, int64_t* h_total_terms);
+
+/* ---- k nearest neighbours ------------------------------------------------ */
+/* Exact kNN under float d² = fma(dz,dz,fma(dy,dy,dx·dx)), results ascending
+ * by (d², index) — the order of cukd's FixedCandidateList (knn.h:403-414);
+ * the index is the position in the caller's d_pts.  Replaces
+ * GenerateKDTreeForDWG + CudaKNNWithKDTreeforDWG (KNN_GWN.cu:183-212, 490-715). */
+int dwg_knn_build(dwg_ctx* ctx, const float* d_pts, int64_t n, int k_hint, dwg_knn_index** out);
+void dwg_knn_destroy(dwg_knn_index* index);
+/* k ≤ 32.  d_skip (nullable): points with d_skip[j] != 0 are not candidates.
+ * d_qmask (nullable): queries with d_qmask[i] == 0 are not computed.
+ * d_d2 (nullable) receives the distances.  Missing neighbours are -1. */
+int dwg_knn_query(dwg_ctx* ctx, const dwg_knn_index* index, const float* d_q, int64_t nq, int k,
+                  const uint8_t* d_skip, const uint8_t* d_qmask, int32_t* d_idx, float* d_d2);
+
+/* ---- marching cubes ------------------------------------------------------ */
+/* Iso-surface of a grid field (TurboMC::TurboMarchingCubes, TurboMC.cu:572-638).
+ * Deterministic order: cubes x-fastest, triangles in table order.  Any of
+ * d_tris (T×9), d_cent (T×3), d_nrm (T×3) may be NULL.  *h_count receives
+ * T; if T > capacity nothing is written and DWG_ERR_CAPACITY is returned. */
+int dwg_marching_cubes(dwg_ctx* ctx, const float* d_field, const dwg_grid* grid, float iso,
+                       int64_t capacity, float* d_tris, float* d_cent, float* d_nrm,
+                       int64_t* h_count);
+
+/* ---- normal diffusion ---------------------------------------------------- */
+/* One diffusion update of source normals from triangle face normals
+ * (TurboDWG.cu:560-644): 15-NN of every centroid among the sources, sum of
+ * face normals per source (exact fixed-point, deterministic), normalise
+ * (|acc| < 1e-10 keeps the old normal and counts as zero), and when more
+ * than zero_threshold sources are zero, refill them with the normalised sum
+ * of their zero_k nearest non-zero sources.  *h_nzero receives the count. */
+int dwg_diffuse_normals(dwg_ctx* ctx, const dwg_knn_index* src_index, const float* d_src,
+                        int64_t n_src, const float* d_cent, const float* d_tri_nrm, int64_t ntri,
+                        int k, int zero_k, int zero_threshold, float* d_nrm, int64_t* h_nzero);
+/* r_i = clamp(sqrt(mean_{j∈kNN(i)} |p_i − p_j|²), rmin, rmax), self included
+ * (DWGProcess.h:325-343; TurboDWG::UpdateRadii, TurboDWG.cu:674-691). */
+int dwg_radii(dwg_ctx* ctx, const dwg_knn_index* index, const float* d_pts, int64_t n, int k,
+              float rmin, float rmax, float* d_out);
+/* n_i = (p_i − c)/(|p_i − c| + 1e-8) (TurboNormalInit.cu:3-27). */
+int dwg_gauss_map(dwg_ctx* ctx, const float* d_pts, int64_t n, const float* h_center,
+                  float* d_nrm);
+/* n_q = normalise(mean of the k nearest source normals) (LoadNormals,
+ * TurboDWG.cu:206-236). */
+int dwg_transfer_normals(dwg_ctx* ctx, const dwg_knn_index* src_index, const float* d_src_nrm,
+                         const float* d_q, int64_t nq, int k, float* d_out_nrm);
+/* Divide the largest ceil(n·threshold) areas by 1000 (OutlierFilter,
+ * TurboDWG.cu:991-1009). */
+int dwg_outlier_filter(dwg_ctx* ctx, float* d_area, int64_t n, float threshold);
+/* FP64 sum of a float array (the reference's thrust::reduce, TurboDWG.cu:541). */
+int dwg_reduce_sum(dwg_ctx* ctx, const float* d_vals, int64_t n, double* h_sum);
+/* Axis-aligned bounds of a point array (cukd computeBounds, builder_common.h:76-150). */
+int dwg_bounds(dwg_ctx* ctx, const float* d_pts, int64_t n, float* h_lo, float* h_hi);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* DWG_HIP_H */

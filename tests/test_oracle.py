@@ -1,0 +1,343 @@
+"""Pins the CPU oracle against facts it does not encode itself.
+
+Each check is chosen so that a plausible restatement mistake (dropped term,
+wrong sign, wrong 4π, transposed index, flipped octant bit, mirrored MC
+corner) fails it:
+  * Gauss's law: area-weighted dipoles on a sphere give w = 1 inside, 0 outside.
+  * Disk solid angle: a dipole sheet on a disk seen from its axis gives
+    Ω/4π = (1 − z/√(z²+R²))/2.
+  * Screened kernel = −A n·∇G_σ of the Yukawa Green's function
+    G_σ(r) = e^{−σr}/(4πr), checked by numerical differentiation.
+  * Gradient = central finite differences of the FP64 direct sum.
+  * Barnes-Hut with β → ∞ equals the direct sum.
+  * Octree: partition, octant rule, depth rule, CM/R brute force.
+  * Voxel grid: closed form of igl::voxel_grid on cube and non-cube boxes.
+  * Marching cubes: exact planes for linear fields, orientation toward
+    increasing values, watertight consistently-wound meshes, sphere area.
+  * kNN: brute force vs an independent numpy ranking; grid vs brute exact.
+  * Radii: closed form on a square lattice.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+from dwg_amd import shapes
+
+
+def _sphere(n, seed=1):
+    P, N = shapes.sphere(n, seed)
+    return P.astype(np.float64), N.astype(np.float64), np.full(n, 4 * np.pi / n)
+
+
+def test_gauss_law_sphere():
+    P, N, A = _sphere(20000)
+    inside = np.array([[0, 0, 0], [0.3, -0.2, 0.1], [0, 0.6, 0]], float)
+    outside = np.array([[2, 0, 0], [0, -1.7, 0.3], [1, 1, 1]], float)
+    w_in = O.direct(P, N, A, inside)
+    w_out = O.direct(P, N, A, outside)
+    assert np.all(np.abs(w_in - 1) < 0.01), w_in
+    assert np.all(np.abs(w_out) < 0.01), w_out
+    # flipping the normals flips the sign
+    assert np.allclose(O.direct(P, -N, A, inside), -w_in)
+
+
+def test_disk_solid_angle():
+    R, m = 1.0, 400
+    g = (np.arange(m) + 0.5) / m * 2 * R - R
+    X, Y = np.meshgrid(g, g)
+    keep = X ** 2 + Y ** 2 < R ** 2
+    P = np.stack([X[keep], Y[keep], np.zeros(keep.sum())], -1)
+    N = np.tile([0.0, 0.0, 1.0], (len(P), 1))
+    A = np.full(len(P), (2 * R / m) ** 2)
+    z = np.array([0.5, 1.0, 2.0])
+    Q = np.stack([np.zeros(3), np.zeros(3), -z], -1)  # below the sheet: d·n > 0
+    w = O.direct(P, N, A, Q)
+    exact = 0.5 * (1 - z / np.sqrt(z ** 2 + R ** 2))
+    assert np.allclose(w, exact, rtol=3e-3), (w, exact)
+
+
+def test_screened_is_yukawa_dipole():
+    sigma, A = 7.0, 0.3
+    n = np.array([0.3, -0.5, 0.81])
+    n /= np.linalg.norm(n)
+    P = np.zeros((1, 3))
+    for q in [np.array([0.2, 0.1, -0.3]), np.array([-0.05, 0.4, 0.2]), np.array([0.01, 0.0, 0.02])]:
+        # −A n·∇_p G(|p − q|) at p = 0, numerically
+        def G(p):
+            r = np.linalg.norm(p - q)
+            return np.exp(-sigma * r) / (4 * np.pi * r)
+        h = 1e-6
+        grad = np.array([(G(h * e) - G(-h * e)) / (2 * h) for e in np.eye(3)])
+        expected = -A * n @ grad
+        # the kernel regularises its denominator D = e^{σr}4πr³ by +1e-7
+        r = np.linalg.norm(q)
+        D = np.exp(sigma * r) * 4 * np.pi * r ** 3
+        expected *= D / (D + 1e-7)
+        w = O.direct(P, n[None], np.array([A]), q[None], kind="screened", sigma=sigma)[0]
+        assert w == pytest.approx(expected, rel=1e-5)
+    # inside the screening radius the term freezes ρ at s
+    q = np.array([[0.0005, 0.0, 0.0]])
+    s = 0.0015
+    d = -q[0]
+    expect = (sigma * s + 1) * A * (d @ n) / (np.exp(sigma * s) * 4 * np.pi * s ** 3 + 1e-7)
+    assert O.direct(P, n[None], np.array([A]), q, kind="screened", sigma=sigma)[0] == pytest.approx(expect, rel=1e-12)
+    # the coincident point contributes nothing
+    assert O.direct(P, n[None], np.array([A]), P, kind="screened", sigma=sigma)[0] == 0.0
+
+
+def test_plain_regularisation_matters():
+    # ε = 1e-8 is part of the kernel: at r = 1e-3, 4πr³ ≈ 1.26e-8
+    P = np.zeros((1, 3))
+    n = np.array([[0.0, 0.0, 1.0]])
+    q = np.array([[0.0, 0.0, -1e-3]])
+    w = O.direct(P, n, np.array([1.0]), q)[0]
+    assert w == pytest.approx(1e-3 / (4 * np.pi * 1e-9 + 1e-8), rel=1e-12)
+
+
+def test_gradient_matches_finite_differences():
+    P, N, A = _sphere(500, 3)
+    Q = np.array([[0.1, 0.2, -0.3], [1.4, 0.2, 0.1], [0.0, 0.97, 0.0]])
+    w, g = O.direct_grad(P, N, A, Q)
+    assert np.allclose(w, O.direct(P, N, A, Q))
+    h = 1e-6
+    fd = np.stack([(O.direct(P, N, A, Q + h * e) - O.direct(P, N, A, Q - h * e)) / (2 * h) for e in np.eye(3)], -1)
+    assert np.allclose(g, fd, rtol=1e-5, atol=1e-6)
+
+
+def test_tree_infinite_beta_equals_direct():
+    P, N, A = _sphere(3000, 4)
+    Q = np.random.default_rng(0).uniform(-1.3, 1.3, (50, 3))
+    t = O.Octree(P, 10, 1)
+    t.precompute(N, A)
+    for kind in ["plain", "screened"]:
+        wt, terms = t.eval(Q, 1e30, kind=kind, count_terms=True)
+        wd = O.direct(P, N, A, Q, kind=kind)
+        assert np.allclose(wt, wd, rtol=1e-12, atol=1e-13)
+        assert np.all(terms == len(P))
+
+
+def test_tree_error_shrinks_with_beta():
+    P, N, A = _sphere(4000, 5)
+    Q = np.random.default_rng(1).uniform(-1.5, 1.5, (200, 3))
+    t = O.Octree(P, 10, 1)
+    t.precompute(N, A)
+    wd = O.direct(P, N, A, Q)
+    errs = [np.abs(t.eval(Q, b, kind="plain") - wd).mean() for b in [1.0, 2.3, 5.0]]
+    assert errs[0] > errs[1] > errs[2]
+    assert errs[1] < 0.02
+
+
+def test_octree_invariants():
+    rng = np.random.default_rng(7)
+    P = rng.uniform(-0.5, 0.5, (1500, 3))
+    P[:20] = P[20]  # duplicates force splitting down to max_depth
+    t = O.Octree(P, 6, 1)
+    ch, cnt, dep = t.topology()
+    A = rng.uniform(0.5, 1.5, len(P))
+    t.precompute(np.tile([0, 0, 1.0], (len(P), 1)), A)
+    cm, r, ec = t.moments()
+    lo, hi = P.min(0), P.max(0)
+    centers = {0: ((lo + hi) / 2, (hi - lo).max())}
+    seen = np.zeros(len(P), int)
+    for i in range(len(ch)):
+        pts = t.node_points(i)
+        c, w = centers[i]
+        if ch[i, 0] == -1:
+            assert cnt[i] <= 1 or dep[i] == 6
+            seen[pts] += 1
+        else:
+            assert cnt[i] > 1 and dep[i] < 6
+            kids = ch[i]
+            assert np.all(kids == kids[0] + np.arange(8))
+            assert sorted(np.concatenate([t.node_points(k) for k in kids]).tolist()) == sorted(pts.tolist())
+            for k in range(8):
+                sign = np.array([1 if k & 1 else -1, 1 if k & 2 else -1, 1 if k & 4 else -1])
+                centers[int(kids[k])] = (c + sign * w / 4, w / 2)
+                kp = P[t.node_points(kids[k])]
+                if len(kp):
+                    octant = (kp[:, 0] >= c[0]) * 1 + (kp[:, 1] >= c[1]) * 2 + (kp[:, 2] >= c[2]) * 4
+                    assert np.all(octant == k)
+        if cnt[i]:
+            want = (A[pts, None] * P[pts]).sum(0) / A[pts].sum()
+            assert np.allclose(cm[i], want)
+            assert r[i] == pytest.approx(np.linalg.norm(P[pts] - want, axis=1).max())
+            assert ec[i] == pytest.approx([0, 0, A[pts].sum()])
+    assert np.all(seen == 1)
+
+
+def test_voxel_grid_closed_form():
+    V = np.array([[0, 0, 0], [1, 1, 1.0]])
+    for s in [16, 33, 128]:
+        o, h, res = O.voxel_grid(V, 0.0, s)
+        assert list(res) == [s, s, s]
+        assert h == pytest.approx(1 / (s - 3))
+        assert o == pytest.approx(np.full(3, 0.5 - (s - 1) / (2 * (s - 3))))
+    V = np.array([[0, 0, 0], [2, 1, 0.5]])
+    o, h, res = O.voxel_grid(V, 0.0, 40)
+    assert list(res) == [40, int(np.ceil(38 * 0.5)) + 2, int(np.ceil(38 * 0.25)) + 2]
+    assert h == pytest.approx(2 / 37)  # a = 2/(1 − 2/39) along x, h = a/39
+    # the grid is centred on the box; the offset pads it
+    o2, h2, res2 = O.voxel_grid(V, 0.1, 40)
+    assert h2 == pytest.approx(2.2 / 37)
+    G = O.grid_points(o2, h2, res2)
+    assert np.allclose((G.min(0) + G.max(0)) / 2, [1.0, 0.5, 0.25])
+    assert np.allclose(G[1] - G[0], [h2, 0, 0]) and np.allclose(G[res2[0]] - G[0], [0, h2, 0])
+
+
+def test_mc_linear_field_planes_and_orientation():
+    res = np.array([9, 8, 7], np.int32)
+    origin = np.array([-0.3, 0.2, 0.1])
+    h = 0.1
+    G = O.grid_points(origin, h, res)
+    a = np.array([0.3, -0.7, 0.5])
+    f = G @ a + 0.05
+    iso = float(np.median(f))
+    tris, c, n = O.marching_cubes(f, res, origin, h, iso)
+    assert len(tris) > 0
+    assert np.allclose(tris.reshape(-1, 3) @ a + 0.05, iso, atol=1e-12)
+    # normals point towards increasing field values
+    ln = np.linalg.norm(n, axis=1)
+    assert np.allclose(n / ln[:, None], a / np.linalg.norm(a), atol=1e-6)
+    assert np.allclose(ln, 1.0, atol=1e-4)  # |n| = |c|/(|c| + 1e-8)
+    assert np.allclose(c, tris.mean(1))
+
+
+def _watertight_and_oriented(tris):
+    # the 1e-10 in the lerp denominator makes the two cubes sharing an edge
+    # place its vertex up to ~1e-10 apart, so vertices are keyed at 1e-6
+    key = lambda v: tuple(np.round(v, 6))
+    directed = {}
+    for t in tris:
+        vs = [key(v) for v in t]
+        if len(set(vs)) < 3:
+            continue  # degenerate triangle (vertex on a corner)
+        for a, b in [(0, 1), (1, 2), (2, 0)]:
+            e = (vs[a], vs[b])
+            directed[e] = directed.get(e, 0) + 1
+    # (directed edges used twice = inconsistent winding, edges without a twin)
+    dup = sum(1 for c in directed.values() if c != 1)
+    unpaired = sum(1 for (u, v) in directed if directed.get((v, u), 0) == 0)
+    return dup, unpaired, len(directed)
+
+
+def test_mc_table_watertight_random_fields():
+    rng = np.random.default_rng(11)
+    res = np.array([14, 14, 14], np.int32)
+    origin = np.zeros(3)
+    h = 1.0 / 13
+    G = O.grid_points(origin, h, res)
+    cases = set()
+    rows = O.mc_table()
+    for trial in range(4):
+        cs = rng.uniform(0.2, 0.8, (5, 3))
+        f = sum(np.exp(-np.sum((G - c) ** 2, 1) / 0.01) for c in cs)
+        # keep the field away from the edges of the box so the surface closes
+        f *= np.all((G > 0.05) & (G < 0.95), 1)
+        iso = 0.3 + 0.1 * trial
+        tris, _, _ = O.marching_cubes(f, res, origin, h, iso)
+        # the classic table has no asymptotic decider, so ambiguous faces can
+        # leave a few cracks; winding must still be consistent everywhere
+        dup, unpaired, total = _watertight_and_oriented(tris)
+        assert dup == 0
+        assert unpaired <= 0.05 * total
+        vals = f.reshape(res[2], res[1], res[0])
+        for z in range(res[2] - 1):
+            for y in range(res[1] - 1):
+                for x in range(res[0] - 1):
+                    c = 0
+                    for i in range(8):
+                        dx = int(bool(i & 1) != bool(i & 2))
+                        dy = int(bool(i & 2))
+                        dz = 0 if i & 4 else 1
+                        c |= int(vals[z + dz, y + dy, x + dx] > iso) << i
+                    cases.add(c)
+    assert len(cases) > 20
+    # every row uses exactly the edges its corners cut
+    ea = [e & 7 for e in range(12)]
+    eb = [((e + 1) & 3) if e < 4 else (4 + ((e - 3) & 3) if e < 8 else e - 4) for e in range(12)]
+    for c in range(256):
+        row = int(rows[c])
+        nt = row >> 60
+        used = {(row >> (4 * v)) & 0xF for v in range(3 * nt)}
+        cut = {e for e in range(12) if ((c >> ea[e]) & 1) != ((c >> eb[e]) & 1)}
+        assert used == cut, c
+
+
+def test_mc_sphere_area_converges():
+    res = np.array([60, 60, 60], np.int32)
+    origin = np.full(3, -1.2)
+    h = 2.4 / 59
+    G = O.grid_points(origin, h, res)
+    f = np.linalg.norm(G, axis=1)
+    tris, _, n = O.marching_cubes(f, res, origin, h, 1.0)
+    area = 0.5 * np.linalg.norm(np.cross(tris[:, 1] - tris[:, 0], tris[:, 2] - tris[:, 0]), axis=1).sum()
+    assert area == pytest.approx(4 * np.pi, rel=0.01)
+    assert _watertight_and_oriented(tris)[:2] == (0, 0)
+    # normals point towards increasing |x|, i.e. outwards
+    c = tris.mean(1)
+    assert np.all(np.einsum("ij,ij->i", n, c) > 0)
+
+
+def test_knn_brute_matches_numpy_and_grid():
+    rng = np.random.default_rng(3)
+    P = rng.uniform(0, 1, (3000, 3)).astype(np.float32)
+    Q = rng.uniform(0, 1, (300, 3)).astype(np.float32)
+    idx, d2 = O.knn_brute(P, Q, 12, return_d2=True)
+    D = ((Q[:, None, :].astype(np.float64) - P[None].astype(np.float64)) ** 2).sum(-1)
+    ref = np.argsort(D, axis=1, kind="stable")[:, :12]
+    assert np.array_equal(idx, ref)
+    assert np.allclose(d2, np.take_along_axis(D, ref, 1), rtol=1e-6)
+    assert np.array_equal(O.knn_grid(P, Q, 12), idx)
+    # ties resolve to the lower index; skip removes candidates
+    L = np.stack(np.meshgrid(np.arange(6), np.arange(6), np.arange(6), indexing="ij"), -1).reshape(-1, 3)
+    L = (L * 0.01).astype(np.float32)
+    Lq = L[[0, 43, 100]] + np.float32(0.0)
+    a = O.knn_brute(L, Lq, 7)
+    assert np.array_equal(a, O.knn_grid(L, Lq, 7))
+    for row, qi in zip(a, [0, 43, 100]):
+        assert row[0] == qi
+        d = ((L[row[1:7]] - L[qi]) ** 2).sum(1)
+        assert np.all(np.diff(d) >= -1e-12)
+    skip = np.zeros(len(L), np.uint8)
+    skip[::2] = 1
+    b = O.knn_brute(L, Lq, 5, skip=skip)
+    assert np.all(skip[b] == 0)
+    assert np.array_equal(b, O.knn_grid(L, Lq, 5, skip=skip))
+
+
+def test_radii_square_lattice():
+    hh = 0.005
+    g = np.arange(20) * hh
+    X, Y = np.meshgrid(g, g)
+    P = np.stack([X.ravel(), Y.ravel(), np.zeros(X.size)], -1).astype(np.float32)
+    r = O.radii(P, k=5)
+    inner = (X.ravel() > 0) & (X.ravel() < g[-1]) & (Y.ravel() > 0) & (Y.ravel() < g[-1])
+    assert np.allclose(r[inner], hh * np.sqrt(4 / 5), rtol=1e-6)
+    assert np.all(r >= 0.0015) and np.all(r <= 0.015)
+
+
+def test_gauss_map_and_sigma_schedule():
+    P, N, _ = _sphere(100)
+    # the six axis points make the AABB exactly [-1, 1]³, centred at 0
+    P = np.concatenate([P, np.eye(3), -np.eye(3)])
+    N = np.concatenate([N, np.eye(3), -np.eye(3)])
+    assert np.allclose(O.gauss_map(P), N, atol=1e-6)
+    assert np.allclose(O.gauss_map(P + [3.0, -1.0, 2.0]), N, atol=1e-6)
+    sched = [O.sigma_schedule(i, 3.0) for i in range(16)]
+    assert sched == [20, 20, 20, 20, 3, 3, 3, 10, 3, 3, 3, 3, 3, 3, 10, 3]
+
+
+def test_dwg_loop_sign_alternates_and_converges():
+    P, Nt, _ = _sphere(3000, 9)
+    P = P * 0.5
+    A = np.full(len(P), 2e-6)
+    o, h, res = O.voxel_grid(P, 0.05, 36)
+    N0 = O.gauss_map(P)
+    n1, wm1, nt1, nz1 = O.dwg_iterate(P, N0, A, res, o, h, 1, sigma=10.0)
+    n2, wm2, nt2, nz2 = O.dwg_iterate(P, N0, A, res, o, h, 2, sigma=10.0)
+    cos1 = np.einsum("ij,ij->i", n1, Nt)
+    cos2 = np.einsum("ij,ij->i", n2, Nt)
+    assert np.median(cos1) < -0.9  # one iteration: face normals point inwards
+    assert np.median(cos2) > 0.9   # two: outwards again
+    assert wm1[0] > 0 and wm2[1] < 0
+    assert nt1[0] > 0 and nt2[1] > 0
