@@ -1,0 +1,200 @@
+"""DWG reconstruction benchmark on one MI355X.
+
+Workload (the headline metric's configuration): 1,000,000 points sampled
+area-uniformly from a seeded union of 32 random spheres / ellipsoids / tori
+in the unit cube (seed 3), Gaussian noise 0.001·bbox-diagonal; 512³
+iteration and final grids; 40 iterations; octree-leaf sources, β = 2.3,
+σ = 10 (the reference's defaults).  A step is one whole reconstruction
+(setup, iterations, final mesh) through dwg_launch.
+
+Prints one JSON line: metric/value as the headline metric (wall seconds of
+the reconstruction, lower is better), the roofline of the dominant kernel
+(algorithmic FLOPs from exact Barnes-Hut term counts × FLOPs per term over
+the kernel's HIP-event time), quality (normal angle vs the analytic
+normals, Chamfer to a dense clean sample) and a CPU baseline measured on a
+bounded sample with the FP64 CPU restatement.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "recon wall time @1M pts: learned+mixed vs plain FP32, equal normal error"
+PEAK_FP32 = 157.3  # TFLOP/s, MI355X FP32 vector (packed) peak
+FLOPS_PER_TERM = {"screened": 27, "plain": 20}
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def workload(n, seed):
+    from dwg_amd import shapes
+    P, N = shapes.union(n, 32, seed)
+    return P, N
+
+
+def surface_sample(n, seed):
+    from dwg_amd import shapes
+    rng = np.random.default_rng(seed)
+    prims = shapes.random_primitives(rng, 32)
+    S, _ = shapes.union_of(prims, n, np.random.default_rng(seed + 10_000), 0.0)
+    return S
+
+
+def plain_options(args):
+    return dict(grid_reso=args.grid, final_reso=args.grid, total_iter=args.iters, input_normal=3, beta=2.3,
+                leaf_size=1, source_mode=0)
+
+
+def run_once(ctx, P, opts, profile=False, count_terms=False):
+    from dwg_amd import pipeline
+    if profile:
+        ctx.profile_reset()
+        ctx.profile(True)
+    r = pipeline.reconstruct(ctx, P, count_terms=1 if count_terms else 0, **opts)
+    if profile:
+        ctx.profile(False)
+    return r
+
+
+KERNELS = ["gwn_tree_grid_screened", "gwn_tree_screened", "octree_pack", "octree_ec_scan", "octree_ec", "mc_count",
+           "mc_emit", "knn_query_centroids", "knn_query_zero_fill", "knn_query_radii", "knn_query_transfer",
+           "knn_build", "diffuse_scatter", "diffuse_normalize", "diffuse_zero_fill", "radii", "gauss_map"]
+
+
+def cpu_baseline(P, grid_reso, iters, budget_s=20.0):
+    """FP64 CPU restatement (the oracle) on a bounded sample of the same work:
+    octree + moments over all sources, then screened Barnes-Hut sums at a
+    random subset of the source and grid queries; the per-query rates are
+    scaled to the full reconstruction (iters × (sources + grid) queries)."""
+    import oracle as O
+    from dwg_amd.pipeline import options  # noqa: F401  (same defaults)
+    cores = O.threads()
+    c = P.astype(np.float64).mean(0)
+    Pn = P.astype(np.float64) - c
+    Pn /= 2 * np.abs(Pn).max()
+    t0 = time.perf_counter()
+    tree = O.Octree(Pn, 10, 1)
+    N0 = O.gauss_map(Pn)
+    A = np.full(len(Pn), 2e-6)
+    tree.precompute(N0, A)
+    t_build = time.perf_counter() - t0
+    origin, h, res = O.voxel_grid(Pn, 0.05, grid_reso)
+    G = int(res[0]) * int(res[1]) * int(res[2])
+    rng = np.random.default_rng(0)
+    # grow the sample until it takes a measurable fraction of the budget
+    m = 2000
+    while True:
+        qs = Pn[rng.integers(0, len(Pn), m)]
+        gi = rng.integers(0, G, m)
+        gq = np.stack([origin[0] + (gi % res[0]) * h, origin[1] + ((gi // res[0]) % res[1]) * h,
+                       origin[2] + (gi // (res[0] * res[1])) * h], -1)
+        t1 = time.perf_counter()
+        tree.eval(qs, 2.3, kind="screened", sigma=10.0)
+        t_src = time.perf_counter() - t1
+        t1 = time.perf_counter()
+        tree.eval(gq, 2.3, kind="screened", sigma=10.0)
+        t_grid = time.perf_counter() - t1
+        if t_src + t_grid > budget_s / 4 or m >= 2_000_000:
+            break
+        m *= 4
+    per_iter = t_src / m * len(Pn) + t_grid / m * G
+    total = t_build + (iters + 1) * per_iter
+    return {"value": round(total, 1), "unit": "s", "cores": cores, "kind": "port",
+            "sample": f"FP64 oracle: octree+moments over {len(Pn)} points ({t_build:.1f} s) and screened "
+                      f"Barnes-Hut sums at {m} source + {m} grid queries, scaled to {iters + 1} evaluations of "
+                      f"{len(Pn)} sources + {G} grid points (MC/kNN/diffusion not counted)"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--n", type=int, default=1_000_000)
+    ap.add_argument("--grid", type=int, default=512)
+    ap.add_argument("--iters", type=int, default=40)
+    ap.add_argument("--seed", type=int, default=3)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-quality", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only: --gpus {args.gpus} is not supported in this build "
+              "(every box has one GPU; the sharding design is described in DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+    import torch
+    from dwg_amd import gpu, metrics
+
+    log(f"generating workload: {args.n} points, seed {args.seed}")
+    P, Ntrue = workload(args.n, args.seed)
+    ctx = gpu.Context(0)
+    opts = plain_options(args)
+    for w in range(args.warmup):
+        t0 = time.perf_counter()
+        run_once(ctx, P, opts)
+        log(f"warmup {w}: {time.perf_counter() - t0:.2f} s")
+    torch.cuda.synchronize()
+    times, last = [], None
+    for s in range(args.steps):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        last = run_once(ctx, P, opts)
+        torch.cuda.synchronize()
+        times.append(time.perf_counter() - t0)
+        log(f"step {s}: {times[-1]:.3f} s (setup {last.time_setup_s:.2f}, iterations {last.time_iter_s:.2f}, "
+            f"final {last.time_final_s:.2f})")
+    plain_s = max(times)
+    # profiled run: per-kernel HIP-event times and exact term counts
+    prof = run_once(ctx, P, opts, profile=True, count_terms=True)
+    ktimes = {}
+    for k in KERNELS + [k + ":final" for k in KERNELS]:
+        ms, cnt = ctx.profile_get(k)
+        if cnt:
+            ktimes[k] = (ms, cnt)
+    dom = max(ktimes, key=lambda k: ktimes[k][0])
+    log("kernel times (ms total, launches): " + ", ".join(f"{k}={v[0]:.1f}/{v[1]}" for k, v in ktimes.items()))
+    g_ms, g_cnt = ktimes["gwn_tree_grid_screened"]
+    # the final pass's launches are recorded as "<kernel>:final" (another
+    # source set), so this average covers the iteration grid only
+    it_launches = args.iters
+    avg_ms = g_ms / g_cnt
+    achieved = prof.terms_grid * FLOPS_PER_TERM["screened"] / (avg_ms * 1e-3) / 1e12
+    roofline = {"kernel": "gwn_tree_grid_screened", "bound": "mfma", "pipe": "fp32 VALU",
+                "achieved": round(achieved, 2), "peak": PEAK_FP32, "unit": "TFLOP/s",
+                "frac": round(achieved / PEAK_FP32, 4), "traffic": None,
+                "terms_per_launch": int(prof.terms_grid), "flops_per_term": FLOPS_PER_TERM["screened"],
+                "avg_launch_ms": round(avg_ms, 3), "launches": int(g_cnt), "iteration_launches": it_launches,
+                "dominant_by_time": dom}
+    quality = None
+    if not args.no_quality:
+        ang = metrics.normal_angle_deg(last.normals, Ntrue)
+        S = surface_sample(2_000_000, args.seed)
+        cd = metrics.chamfer(ctx, last.tris_input_frame(), S)
+        quality = {"mean_angle_deg": round(float(ang.mean()), 3), "median_angle_deg": round(float(np.median(ang)), 3),
+                   "flipped_frac": round(float((ang > 90).mean()), 5), "chamfer": round(cd, 7),
+                   "triangles": int(last.tris.shape[0]) if last.tris is not None else 0}
+    cpu = None if args.no_cpu else cpu_baseline(P, args.grid, args.iters)
+    out = {"metric": METRIC, "value": round(plain_s, 3), "unit": "s", "n_gpus": 1, "steps": args.steps,
+           "warmup": args.warmup, "ms_per_step": round(plain_s * 1000, 1), "higher_is_better": False,
+           "scaling": "strong", "vs_baseline": None, "dtype": "fp32", "data": "synthetic",
+           "config": {"workload": f"union of 32 primitives, {args.n} points (seed {args.seed}), noise 0.001*diag, "
+                                  f"{args.grid}^3 grids, {args.iters} iterations, octree-leaf sources, beta 2.3",
+                      "method": "plain FP32 (Gauss-map init)", "n_sources": int(last.n_sources),
+                      "phases_s": {"setup": round(last.time_setup_s, 3), "iterations": round(last.time_iter_s, 3),
+                                   "final": round(last.time_final_s, 3)},
+                      "quality": quality},
+           "roofline": roofline, "cpu_baseline": cpu}
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

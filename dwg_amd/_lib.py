@@ -29,12 +29,34 @@ EXPORTS = [
     "dwg_octree_destroy", "dwg_octree_info", "dwg_octree_update", "dwg_gwn_tree", "dwg_gwn_tree_grid",
     "dwg_gwn_tree_terms", "dwg_gwn_tree_grid_terms", "dwg_knn_build", "dwg_knn_destroy", "dwg_knn_query",
     "dwg_marching_cubes", "dwg_diffuse_normals", "dwg_radii", "dwg_gauss_map", "dwg_transfer_normals",
-    "dwg_outlier_filter", "dwg_reduce_sum", "dwg_bounds",
+    "dwg_outlier_filter", "dwg_reduce_sum", "dwg_bounds", "dwg_options_default", "dwg_launch",
+    "dwg_free",
 ]
 
 
 class DwgGrid(ctypes.Structure):
     _fields_ = [("origin", ctypes.c_float * 3), ("h", ctypes.c_float), ("res", ctypes.c_int32 * 3)]
+
+
+class DwgOptions(ctypes.Structure):
+    _fields_ = [("input_normal", ctypes.c_int32), ("grid_reso", ctypes.c_int32), ("final_reso", ctypes.c_int32),
+                ("kw", ctypes.c_int32), ("final_kw", ctypes.c_int32), ("normal_kw", ctypes.c_int32),
+                ("max_depth", ctypes.c_int32), ("sigma", ctypes.c_float), ("min_num", ctypes.c_int32),
+                ("use_radii", ctypes.c_int32), ("com_As", ctypes.c_int32), ("As_threshold", ctypes.c_float),
+                ("offset_1", ctypes.c_float), ("offset_2", ctypes.c_float), ("use_leaves_generate", ctypes.c_int32),
+                ("total_iter", ctypes.c_int32), ("test_iter", ctypes.c_int32), ("output_mode", ctypes.c_int32),
+                ("source_mode", ctypes.c_int32), ("leaf_size", ctypes.c_int32), ("beta", ctypes.c_float),
+                ("precision", ctypes.c_uint32), ("seed", ctypes.c_uint64), ("normalize_input", ctypes.c_int32),
+                ("verbose", ctypes.c_int32), ("count_terms", ctypes.c_int32), ("mlp", ctypes.c_void_p)]
+
+
+class DwgResult(ctypes.Structure):
+    _fields_ = [("normals", ctypes.c_void_p), ("want_mesh", ctypes.c_int32), ("mesh", ctypes.c_void_p),
+                ("tri_count", ctypes.c_int64), ("n_sources", ctypes.c_int64), ("center", ctypes.c_double * 3),
+                ("scale", ctypes.c_double), ("time_setup_s", ctypes.c_double), ("time_iter_s", ctypes.c_double),
+                ("time_final_s", ctypes.c_double), ("time_total_s", ctypes.c_double), ("w_mean", ctypes.c_void_p),
+                ("iter_tris", ctypes.c_void_p), ("iter_zeros", ctypes.c_void_p), ("terms_src", ctypes.c_int64),
+                ("terms_grid", ctypes.c_int64), ("grid_points", ctypes.c_int64)]
 
 
 _vp = ctypes.c_void_p
@@ -86,5 +108,10 @@ def load():
     L.dwg_outlier_filter.argtypes = [_vp, _vp, _i64, _f]
     L.dwg_reduce_sum.argtypes = [_vp, _vp, _i64, P(ctypes.c_double)]
     L.dwg_bounds.argtypes = [_vp, _vp, _i64, P(_f), P(_f)]
+    L.dwg_options_default.argtypes = [P(DwgOptions)]
+    L.dwg_options_default.restype = None
+    L.dwg_launch.argtypes = [_vp, _vp, _vp, _i64, P(DwgOptions), ctypes.c_char_p, P(DwgResult)]
+    L.dwg_free.argtypes = [_vp]
+    L.dwg_free.restype = None
     _lib = L
     return L

@@ -69,6 +69,7 @@ struct dwg_ctx {
     hipStream_t stream = nullptr;
     std::string err;
     bool profile = false;
+    std::string profile_suffix;  // appended to recorded kernel names (phase tag)
     std::map<std::string, dwg::KernelTimes> times;
     // named scratch buffers owned by the context
     std::map<std::string, dwg::DevBuf> scratch;
@@ -95,7 +96,7 @@ struct ProfileScope {
     ~ProfileScope() {
         if (ctx->profile && a) {
             (void)hipEventRecord(b, ctx->stream);
-            ctx->times[name].pending.emplace_back(a, b);
+            ctx->times[std::string(name) + ctx->profile_suffix].pending.emplace_back(a, b);
         }
     }
 };

@@ -95,7 +95,7 @@ void diffuse_normals(dwg_ctx* ctx, const dwg_knn_index* I, const float* src, int
     fill_u32(ctx, acc, 0, 2 * (3 * n + 1));
     if (ntri > 0) {
         int32_t* knn = static_cast<int32_t*>(ctx->buf("diff_knn", sizeof(int32_t) * ntri * k));
-        knn_query(ctx, I, cent, ntri, k, nullptr, nullptr, knn, nullptr);
+        knn_query(ctx, I, cent, ntri, k, nullptr, nullptr, knn, nullptr, "knn_query_centroids");
         ProfileScope ps(ctx, "diffuse_scatter");
         hipLaunchKernelGGL(scatter_fixed_kernel, dim3(grid_for(ntri * k, 256)), dim3(256), 0, ctx->stream, knn, k, tri_nrm,
                            ntri, acc);
@@ -110,7 +110,7 @@ void diffuse_normals(dwg_ctx* ctx, const dwg_knn_index* I, const float* src, int
     if (h_nzero) *h_nzero = nzero;
     if (nzero > zero_threshold && nzero < n) {
         int32_t* zknn = static_cast<int32_t*>(ctx->buf("diff_zknn", sizeof(int32_t) * n * zero_k));
-        knn_query(ctx, I, src, n, zero_k, zero, zero, zknn, nullptr);
+        knn_query(ctx, I, src, n, zero_k, zero, zero, zknn, nullptr, "knn_query_zero_fill");
         ProfileScope ps(ctx, "diffuse_zero_fill");
         hipLaunchKernelGGL(zero_fill_kernel, dim3(grid_for(n, 256)), dim3(256), 0, ctx->stream, zknn, zero_k, zero, n, nrm);
         check_launch();
@@ -141,7 +141,7 @@ __global__ void radii_kernel(const int32_t* __restrict__ knn, int k, const float
 void radii(dwg_ctx* ctx, const dwg_knn_index* I, const float* p, int64_t n, int k, float rmin, float rmax, float* out) {
     DWG_REQUIRE(I && I->n == n, "kNN index does not match the point count");
     int32_t* knn = static_cast<int32_t*>(ctx->buf("radii_knn", sizeof(int32_t) * n * k));
-    knn_query(ctx, I, p, n, k, nullptr, nullptr, knn, nullptr);
+    knn_query(ctx, I, p, n, k, nullptr, nullptr, knn, nullptr, "knn_query_radii");
     ProfileScope ps(ctx, "radii");
     hipLaunchKernelGGL(radii_kernel, dim3(grid_for(n, 256)), dim3(256), 0, ctx->stream, knn, k, p, n, rmin, rmax, out);
     check_launch();
@@ -193,7 +193,7 @@ void transfer_normals(dwg_ctx* ctx, const dwg_knn_index* I, const float* src_nrm
                       float* out) {
     if (nq == 0) return;
     int32_t* knn = static_cast<int32_t*>(ctx->buf("transfer_knn", sizeof(int32_t) * nq * k));
-    knn_query(ctx, I, q, nq, k, nullptr, nullptr, knn, nullptr);
+    knn_query(ctx, I, q, nq, k, nullptr, nullptr, knn, nullptr, "knn_query_transfer");
     ProfileScope ps(ctx, "transfer_normals");
     hipLaunchKernelGGL(transfer_kernel, dim3(grid_for(nq, 256)), dim3(256), 0, ctx->stream, knn, k, src_nrm, nq, out);
     check_launch();

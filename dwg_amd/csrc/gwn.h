@@ -10,6 +10,7 @@ namespace dwg {
 struct HostOctree {
     std::vector<int> first_child, begin, end, depth;
     std::vector<double> cm, r;  // 3 per node, 1 per node
+    std::vector<double> cn, w;  // cell centre (3 per node) and cell width
     std::vector<int> perm;      // packed slot → caller index
     const double* P = nullptr;
     int64_t n_points = 0, n_leaves = 0;

@@ -250,6 +250,8 @@ HostOctree build_octree_host(const double* P, const double* A, int64_t n, int ma
     T.begin.push_back(0);
     T.end.push_back(int(n));
     T.depth.push_back(0);
+    T.cn.insert(T.cn.end(), {c0[0], c0[1], c0[2]});
+    T.w.push_back(w0);
     std::function<void(int, int)> split = [&](int idx, int depth) {
         const int b = T.begin[idx], e = T.end[idx];
         if (!(e - b > leaf_size && depth < max_depth)) return;
@@ -257,6 +259,8 @@ HostOctree build_octree_host(const double* P, const double* A, int64_t n, int ma
         T.first_child[idx] = m;
         const int shift = 3 * (max_depth - 1 - depth);
         int cur = b;
+        const double pc[3] = {T.cn[3 * idx], T.cn[3 * idx + 1], T.cn[3 * idx + 2]};
+        const double q = T.w[idx] / 4;
         for (int k = 0; k < 8; ++k) {
             int stop = cur;
             while (stop < e && int((sc[stop] >> shift) & 7) == k) ++stop;
@@ -264,6 +268,8 @@ HostOctree build_octree_host(const double* P, const double* A, int64_t n, int ma
             T.begin.push_back(cur);
             T.end.push_back(stop);
             T.depth.push_back(depth + 1);
+            T.cn.insert(T.cn.end(), {pc[0] + ((k & 1) ? q : -q), pc[1] + ((k & 2) ? q : -q), pc[2] + ((k & 4) ? q : -q)});
+            T.w.push_back(T.w[idx] / 2);
             cur = stop;
         }
         for (int k = 0; k < 8; ++k) split(m + k, depth + 1);

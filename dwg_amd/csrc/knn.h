@@ -10,11 +10,17 @@ struct KnnGridArgs {
 };
 }  // namespace dwg
 
+// Two cell grids over the same points: a fine one sized from the measured
+// occupancy and a coarse one (8× larger cells) for queries whose fine search
+// does not close within a few shells.
 struct dwg_knn_index {
-    int64_t n = 0, ncells = 0;
-    dwg::KnnGridArgs G{};
-    int64_t* cell_start = nullptr;  // ncells + 1
-    float4* pts = nullptr;          // (x, y, z, index bits) bucketed by cell
+    int64_t n = 0;
+    struct Level {
+        dwg::KnnGridArgs G{};
+        int64_t ncells = 0;
+        int64_t* cell_start = nullptr;  // ncells + 1
+        float4* pts = nullptr;          // (x, y, z, index bits) bucketed by cell
+    } lvl[2];
     ~dwg_knn_index();
 };
 
@@ -22,5 +28,5 @@ namespace dwg {
 void bounds(dwg_ctx* ctx, const float* p, int64_t n, float* lo, float* hi);
 dwg_knn_index* knn_build(dwg_ctx* ctx, const float* p, int64_t n, int k_hint);
 void knn_query(dwg_ctx* ctx, const dwg_knn_index* I, const float* q, int64_t nq, int k, const uint8_t* skip,
-               const uint8_t* qmask, int32_t* out_idx, float* out_d2);
+               const uint8_t* qmask, int32_t* out_idx, float* out_d2, const char* tag = nullptr);
 }  // namespace dwg

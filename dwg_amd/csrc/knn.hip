@@ -16,8 +16,10 @@
 #include "scan.h"
 
 dwg_knn_index::~dwg_knn_index() {
-    if (cell_start) (void)hipFree(cell_start);
-    if (pts) (void)hipFree(pts);
+    for (auto& L : lvl) {
+        if (L.cell_start) (void)hipFree(L.cell_start);
+        if (L.pts) (void)hipFree(L.pts);
+    }
 }
 
 namespace dwg {
@@ -96,6 +98,60 @@ void bounds(dwg_ctx* ctx, const float* p, int64_t n, float* lo, float* hi) {
         }
 }
 
+__global__ void count_occupied_kernel(const int64_t* __restrict__ counts, int64_t nc,
+                                      unsigned long long* __restrict__ out) {
+    __shared__ int lds[256];
+    int c = 0;
+    for (int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x; i < nc; i += int64_t(gridDim.x) * 256) c += counts[i] != 0;
+    lds[threadIdx.x] = c;
+    __syncthreads();
+    for (int off = 128; off > 0; off >>= 1) {
+        if (threadIdx.x < off) lds[threadIdx.x] += lds[threadIdx.x + off];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) atomicAdd(out, (unsigned long long)lds[0]);
+}
+
+static KnnGridArgs grid_args(const float* lo, const float* hi, double cell) {
+    KnnGridArgs G{};
+    G.cell = float(cell);
+    G.inv_cell = float(1.0 / cell);
+    for (int d = 0; d < 3; ++d) {
+        G.lo[d] = lo[d];
+        G.dim[d] = std::max(1, int(std::floor((hi[d] - lo[d]) / cell)) + 1);
+    }
+    return G;
+}
+
+static double cells_of(const float* lo, const float* hi, double cell) {
+    double nc = 1;
+    for (int d = 0; d < 3; ++d) nc *= std::floor((hi[d] - lo[d]) / cell) + 1;
+    return nc;
+}
+
+// histogram + scan (+ scatter when out != nullptr) of the points into level L
+static void bucket(dwg_ctx* ctx, const float* p, int64_t n, dwg_knn_index::Level& L, bool scatter) {
+    L.ncells = int64_t(L.G.dim[0]) * L.G.dim[1] * L.G.dim[2];
+    if (L.cell_start) (void)hipFree(L.cell_start);
+    L.cell_start = nullptr;
+    DWG_HIP_CHECK(hipMalloc(&L.cell_start, sizeof(int64_t) * (L.ncells + 1)));
+    int* cell_of = static_cast<int*>(ctx->buf("knn_cell_of", sizeof(int) * n));
+    fill_u32(ctx, L.cell_start, 0, 2 * (L.ncells + 1));
+    hipLaunchKernelGGL(knn_count_kernel, dim3(grid_for(n, 256)), dim3(256), 0, ctx->stream, p, n, L.G,
+                       reinterpret_cast<unsigned long long*>(L.cell_start), cell_of);
+    check_launch();
+    if (!scatter) return;
+    unsigned long long* cursor = static_cast<unsigned long long*>(ctx->buf("knn_cursor", sizeof(uint64_t) * L.ncells));
+    exclusive_scan<int64_t>(ctx, L.cell_start, L.ncells);
+    hipLaunchKernelGGL(copy_i64_kernel, dim3(grid_for(L.ncells, 256)), dim3(256), 0, ctx->stream, L.cell_start,
+                       cursor, L.ncells);
+    check_launch();
+    DWG_HIP_CHECK(hipMalloc(&L.pts, sizeof(float4) * n));
+    hipLaunchKernelGGL(knn_scatter_kernel, dim3(grid_for(n, 256)), dim3(256), 0, ctx->stream, p, n, cell_of, cursor,
+                       L.pts);
+    check_launch();
+}
+
 dwg_knn_index* knn_build(dwg_ctx* ctx, const float* p, int64_t n, int k_hint) {
     DWG_REQUIRE(n > 0 && n < (int64_t(1) << 31) - 1, "kNN index needs 1 ≤ n < 2^31 points");
     float lo[3], hi[3];
@@ -103,41 +159,36 @@ dwg_knn_index* knn_build(dwg_ctx* ctx, const float* p, int64_t n, int k_hint) {
     auto* I = new dwg_knn_index();
     try {
         I->n = n;
-        const double ext = std::max({double(hi[0] - lo[0]), double(hi[1] - lo[1]), double(hi[2] - lo[2]), 1e-6});
-        // points on surfaces: occupancy of a cell grows with cell², so the
-        // cell edge is set from N for about target points per occupied cell
-        const double target = std::max(2.0, double(std::max(k_hint, 1)) / 2.0);
-        double cell = ext / std::max(1.0, std::sqrt(double(n) / target));
-        const double max_cells = double(1 << 28);
-        for (;;) {
-            double nc = 1;
-            for (int d = 0; d < 3; ++d) nc *= std::floor((hi[d] - lo[d]) / cell) + 1;
-            if (nc <= max_cells) break;
-            cell *= 1.25;
-        }
-        I->G.cell = float(cell);
-        I->G.inv_cell = float(1.0 / cell);
-        for (int d = 0; d < 3; ++d) {
-            I->G.lo[d] = lo[d];
-            I->G.dim[d] = std::max(1, int(std::floor((hi[d] - lo[d]) / cell)) + 1);
-        }
-        I->ncells = int64_t(I->G.dim[0]) * I->G.dim[1] * I->G.dim[2];
-        DWG_HIP_CHECK(hipMalloc(&I->cell_start, sizeof(int64_t) * (I->ncells + 1)));
-        DWG_HIP_CHECK(hipMalloc(&I->pts, sizeof(float4) * n));
-        int* cell_of = static_cast<int*>(ctx->buf("knn_cell_of", sizeof(int) * n));
-        unsigned long long* cursor = static_cast<unsigned long long*>(ctx->buf("knn_cursor", sizeof(uint64_t) * I->ncells));
-        fill_u32(ctx, I->cell_start, 0, 2 * (I->ncells + 1));
         ProfileScope ps(ctx, "knn_build");
-        hipLaunchKernelGGL(knn_count_kernel, dim3(grid_for(n, 256)), dim3(256), 0, ctx->stream, p, n, I->G,
-                           reinterpret_cast<unsigned long long*>(I->cell_start), cell_of);
+        const double ext = std::max({double(hi[0] - lo[0]), double(hi[1] - lo[1]), double(hi[2] - lo[2]), 1e-6});
+        const double max_cells = double(1 << 27);
+        // about `target` points per occupied cell puts ~k points in the 3×3×3
+        // block around a query on a surface-like cloud
+        const double target = std::max(2.0, 0.3 * double(std::max(k_hint, 1)));
+        double cell = ext / std::max(1.0, std::sqrt(double(n) / target));
+        while (cells_of(lo, hi, cell) > max_cells) cell *= 1.25;
+        // measure the occupancy at this cell size and rescale (occupancy of a
+        // surface-like cloud grows with cell²)
+        dwg_knn_index::Level probe;
+        probe.G = grid_args(lo, hi, cell);
+        bucket(ctx, p, n, probe, false);
+        unsigned long long* occ = static_cast<unsigned long long*>(ctx->buf("knn_occ", sizeof(uint64_t)));
+        fill_u32(ctx, occ, 0, 2);
+        hipLaunchKernelGGL(count_occupied_kernel, dim3(512), dim3(256), 0, ctx->stream, probe.cell_start, probe.ncells,
+                           occ);
         check_launch();
-        exclusive_scan<int64_t>(ctx, I->cell_start, I->ncells);
-        hipLaunchKernelGGL(copy_i64_kernel, dim3(grid_for(I->ncells, 256)), dim3(256), 0, ctx->stream, I->cell_start,
-                           cursor, I->ncells);
-        check_launch();
-        hipLaunchKernelGGL(knn_scatter_kernel, dim3(grid_for(n, 256)), dim3(256), 0, ctx->stream, p, n, cell_of, cursor,
-                           I->pts);
-        check_launch();
+        const double occupied = double(std::max<unsigned long long>(1, read_scalar<unsigned long long>(ctx, occ)));
+        (void)hipFree(probe.cell_start);
+        probe.cell_start = nullptr;
+        const double o = double(n) / occupied;
+        double scale = std::sqrt(target / o);
+        scale = std::min(16.0, std::max(0.25, scale));
+        cell *= scale;
+        while (cells_of(lo, hi, cell) > max_cells) cell *= 1.25;
+        I->lvl[0].G = grid_args(lo, hi, cell);
+        bucket(ctx, p, n, I->lvl[0], true);
+        I->lvl[1].G = grid_args(lo, hi, cell * 8.0);
+        bucket(ctx, p, n, I->lvl[1], true);
     } catch (...) {
         delete I;
         throw;
@@ -168,35 +219,31 @@ __device__ __forceinline__ uint64_t kth_of(const uint64_t (&best)[K], int k) {
     return v;
 }
 
+// Visits shells r = 0..rmax of cells around q on one grid level, keeping
+// the k smallest (d², index) keys.  Returns true once no unvisited cell can
+// hold a closer point (the search is then exact on this level).
 template <int K>
-__global__ __launch_bounds__(256) void knn_query_kernel(KnnGridArgs G, const int64_t* __restrict__ cell_start,
-                                                        const float4* __restrict__ pts, const float* __restrict__ q,
-                                                        int64_t nq, int k, const uint8_t* __restrict__ skip,
-                                                        const uint8_t* __restrict__ qmask, int32_t* __restrict__ out_idx,
-                                                        float* __restrict__ out_d2) {
-    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (i >= nq) return;
-    if (qmask && !qmask[i]) return;
-    const float qx = q[3 * i], qy = q[3 * i + 1], qz = q[3 * i + 2];
-    uint64_t best[K];
+__device__ bool search_level(const KnnGridArgs& G, const int64_t* __restrict__ cell_start,
+                             const float4* __restrict__ pts, float qx, float qy, float qz, int k,
+                             const uint8_t* __restrict__ skip, uint64_t (&best)[K], int rmax) {
+    uint64_t kth = ~0ull;
 #pragma unroll
     for (int j = 0; j < K; ++j) best[j] = ~0ull;
     const int c0x = cell_coord(qx, G.lo[0], G.inv_cell, G.dim[0]);
     const int c0y = cell_coord(qy, G.lo[1], G.inv_cell, G.dim[1]);
     const int c0z = cell_coord(qz, G.lo[2], G.inv_cell, G.dim[2]);
-    const int maxr = max(G.dim[0], max(G.dim[1], G.dim[2]));
-    uint64_t kth = ~0ull;
-    for (int r = 0; r <= maxr; ++r) {
+    for (int r = 0; r <= rmax; ++r) {
         const int zlo = max(c0z - r, 0), zhi = min(c0z + r, G.dim[2] - 1);
         const int ylo = max(c0y - r, 0), yhi = min(c0y + r, G.dim[1] - 1);
         for (int z = zlo; z <= zhi; ++z) {
             const bool zedge = abs(z - c0z) == r;
             for (int y = ylo; y <= yhi; ++y) {
                 const bool yedge = zedge || abs(y - c0y) == r;
-                // on a shell row either all x in range (edge rows) or just the two ends
+                // rows on the shell's faces are scanned fully, other rows only at both ends
                 const int xs = yedge ? max(c0x - r, 0) : c0x - r;
                 const int step = yedge ? 1 : 2 * r;
-                for (int x = xs; x <= min(c0x + r, G.dim[0] - 1); x += step) {
+                const int xe = min(c0x + r, G.dim[0] - 1);
+                for (int x = xs; x <= xe; x += step) {
                     if (x < 0) continue;
                     const int64_t c = x + int64_t(G.dim[0]) * (y + int64_t(G.dim[1]) * z);
                     const int64_t sb = cell_start[c], se = cell_start[c + 1];
@@ -223,12 +270,32 @@ __global__ __launch_bounds__(256) void knn_query_kernel(KnnGridArgs G, const int
         if (c0y + r + 1 < G.dim[1]) dmin = fminf(dmin, (G.lo[1] + float(c0y + r + 1) * G.cell) - qy);
         if (c0z - r > 0) dmin = fminf(dmin, qz - (G.lo[2] + float(c0z - r) * G.cell));
         if (c0z + r + 1 < G.dim[2]) dmin = fminf(dmin, (G.lo[2] + float(c0z + r + 1) * G.cell) - qz);
-        if (dmin == INFINITY) break;  // the block covers the whole grid
+        if (dmin == INFINITY) return true;  // the block covers the whole grid
         if (kth != ~0ull) {
             const float kd2 = __uint_as_float(uint32_t(kth >> 32));
-            const float safe = dmin * (1.0f - 1e-5f) - 1e-7f * G.cell;
-            if (safe > 0.f && kd2 < safe * safe) break;
+            const float safe = dmin * (1.0f - 1e-5f) - 1e-6f * G.cell;
+            if (safe > 0.f && kd2 < safe * safe) return true;
         }
+    }
+    return false;
+}
+
+template <int K>
+__global__ __launch_bounds__(256) void knn_query_kernel(KnnGridArgs G0, const int64_t* __restrict__ cs0,
+                                                        const float4* __restrict__ p0, KnnGridArgs G1,
+                                                        const int64_t* __restrict__ cs1,
+                                                        const float4* __restrict__ p1, const float* __restrict__ q,
+                                                        int64_t nq, int k, const uint8_t* __restrict__ skip,
+                                                        const uint8_t* __restrict__ qmask, int32_t* __restrict__ out_idx,
+                                                        float* __restrict__ out_d2) {
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= nq) return;
+    if (qmask && !qmask[i]) return;
+    const float qx = q[3 * i], qy = q[3 * i + 1], qz = q[3 * i + 2];
+    uint64_t best[K];
+    if (!search_level<K>(G0, cs0, p0, qx, qy, qz, k, skip, best, 2)) {
+        const int maxr = max(G1.dim[0], max(G1.dim[1], G1.dim[2]));
+        (void)search_level<K>(G1, cs1, p1, qx, qy, qz, k, skip, best, maxr);
     }
     for (int j = 0; j < K; ++j) {
         if (j >= k) break;
@@ -239,15 +306,17 @@ __global__ __launch_bounds__(256) void knn_query_kernel(KnnGridArgs G, const int
 }
 
 void knn_query(dwg_ctx* ctx, const dwg_knn_index* I, const float* q, int64_t nq, int k, const uint8_t* skip,
-               const uint8_t* qmask, int32_t* out_idx, float* out_d2) {
+               const uint8_t* qmask, int32_t* out_idx, float* out_d2, const char* tag) {
     DWG_REQUIRE(I, "null kNN index");
     DWG_REQUIRE(k >= 1 && k <= 32, "k must be in [1, 32]");
     DWG_REQUIRE(nq >= 0, "negative query count");
     if (nq == 0) return;
     dim3 grid(grid_for(nq, 256)), block(256);
-    ProfileScope ps(ctx, "knn_query");
+    ProfileScope ps(ctx, tag ? tag : "knn_query");
 #define DWG_KNN(KK) \
-    hipLaunchKernelGGL(knn_query_kernel<KK>, grid, block, 0, ctx->stream, I->G, I->cell_start, I->pts, q, nq, k, skip, qmask, out_idx, out_d2)
+    hipLaunchKernelGGL(knn_query_kernel<KK>, grid, block, 0, ctx->stream, I->lvl[0].G, I->lvl[0].cell_start,   \
+                       I->lvl[0].pts, I->lvl[1].G, I->lvl[1].cell_start, I->lvl[1].pts, q, nq, k, skip, qmask, \
+                       out_idx, out_d2)
     if (k <= 8) DWG_KNN(8);
     else if (k <= 10) DWG_KNN(10);
     else if (k <= 16) DWG_KNN(16);

@@ -1,0 +1,603 @@
+// Host-side DWG driver over the C ABI: TurboDWG (src/TurboDWG.cu) and the
+// test3D orchestration (include/DWGProcess.h), without Eigen/libigl/PCL.
+#include "turbo_dwg.h"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <deque>
+#include <fstream>
+#include <random>
+#include <sstream>
+#include <stdexcept>
+
+#include "common.h"
+#include "gwn.h"
+
+namespace dwg_host {
+
+using Clock = std::chrono::steady_clock;
+
+static double seconds_since(Clock::time_point t) {
+    return std::chrono::duration<double>(Clock::now() - t).count();
+}
+
+template <class T>
+DevArray<T>::~DevArray() {
+    if (ptr) (void)hipFree(ptr);
+}
+
+template <class T>
+void DevArray<T>::resize(int64_t n) {
+    if (n <= count && ptr) return;
+    if (ptr) (void)hipFree(ptr);
+    ptr = nullptr;
+    count = 0;
+    if (hipMalloc(&ptr, sizeof(T) * std::max<int64_t>(n, 1)) != hipSuccess)
+        throw dwg::Error(DWG_ERR_HIP, "device allocation failed");
+    count = n;
+}
+
+template <class T>
+void DevArray<T>::upload(const T* h, int64_t n, void* stream) {
+    resize(n);
+    if (n > 0 && hipMemcpyAsync(ptr, h, sizeof(T) * n, hipMemcpyHostToDevice, static_cast<hipStream_t>(stream)) != hipSuccess)
+        throw dwg::Error(DWG_ERR_HIP, "upload failed");
+}
+
+template <class T>
+void DevArray<T>::download(T* h, int64_t n, void* stream) const {
+    if (n > 0 && hipMemcpyAsync(h, ptr, sizeof(T) * n, hipMemcpyDeviceToHost, static_cast<hipStream_t>(stream)) != hipSuccess)
+        throw dwg::Error(DWG_ERR_HIP, "download failed");
+    if (hipStreamSynchronize(static_cast<hipStream_t>(stream)) != hipSuccess) throw dwg::Error(DWG_ERR_HIP, "sync failed");
+}
+
+template struct DevArray<float>;
+template struct DevArray<int32_t>;
+template struct DevArray<int64_t>;
+
+dwg_grid Grid::abi() const {
+    dwg_grid g;
+    for (int d = 0; d < 3; ++d) {
+        g.origin[d] = float(origin[d]);
+        g.res[d] = res[d];
+    }
+    g.h = float(h);
+    return g;
+}
+
+Grid voxel_grid(const double* V, int64_t n, double offset, int s) {
+    if (n <= 0) throw dwg::Error(DWG_ERR_ARG, "voxel grid of no points");
+    if (s < 4) throw dwg::Error(DWG_ERR_ARG, "grid resolution must be ≥ 4");
+    double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for (int64_t i = 0; i < n; ++i)
+        for (int d = 0; d < 3; ++d) {
+            lo[d] = std::min(lo[d], V[3 * i + d]);
+            hi[d] = std::max(hi[d], V[3 * i + d]);
+        }
+    double ext[3], ctr[3];
+    for (int d = 0; d < 3; ++d) {
+        lo[d] -= offset;
+        hi[d] += offset;
+        ext[d] = hi[d] - lo[d];
+        ctr[d] = (lo[d] + hi[d]) / 2;
+    }
+    int si = 0;
+    for (int d = 1; d < 3; ++d)
+        if (ext[d] > ext[si]) si = d;
+    // the longest side gets s samples, the others ceil((s−2)·ext/ext_max)+2,
+    // one padding sample each side; spacing from the largest per-axis scale
+    double side[3];
+    for (int d = 0; d < 3; ++d) side[d] = (d == si ? double(s - 2) : std::ceil(double(s - 2) * ext[d] / ext[si])) + 2;
+    double amax = -1;
+    int ai = 0;
+    for (int d = 0; d < 3; ++d) {
+        const double a = ext[d] / (1.0 - 2.0 / (side[d] - 1.0));
+        if (a > amax) {
+            amax = a;
+            ai = d;
+        }
+    }
+    Grid g;
+    g.h = amax / (side[ai] - 1.0);
+    for (int d = 0; d < 3; ++d) {
+        g.res[d] = int(side[d]);
+        g.origin[d] = ctr[d] - (side[d] - 1.0) / 2.0 * g.h;
+    }
+    return g;
+}
+
+void save_binary_stl(const std::string& path, const float* tris, int64_t ntri) {
+    std::ofstream f(path, std::ios::binary | std::ios::trunc);
+    if (!f) throw dwg::Error(DWG_ERR_ARG, "cannot create " + path);
+    char header[80] = {0};
+    f.write(header, 80);
+    const uint32_t n = uint32_t(ntri);
+    f.write(reinterpret_cast<const char*>(&n), 4);
+    const char pad[2] = {0, 0};
+    for (int64_t t = 0; t < ntri; ++t) {
+        const float* v = tris + 9 * t;
+        const float e1[3] = {v[3] - v[0], v[4] - v[1], v[5] - v[2]};
+        const float e2[3] = {v[6] - v[0], v[7] - v[1], v[8] - v[2]};
+        float c[3] = {e1[1] * e2[2] - e1[2] * e2[1], e1[2] * e2[0] - e1[0] * e2[2], e1[0] * e2[1] - e1[1] * e2[0]};
+        const float l = std::sqrt(c[0] * c[0] + c[1] * c[1] + c[2] * c[2]);
+        for (float& x : c) x = l > 0 ? x / l : 0.f;
+        f.write(reinterpret_cast<const char*>(c), 12);
+        f.write(reinterpret_cast<const char*>(v), 36);
+        f.write(pad, 2);
+    }
+    if (!f.good()) throw dwg::Error(DWG_ERR_ARG, "cannot write " + path);
+}
+
+void save_xyz(const std::string& path, const float* pts, const float* nrm, int64_t n) {
+    std::ofstream f(path);
+    if (!f) throw dwg::Error(DWG_ERR_ARG, "cannot create " + path);
+    for (int64_t i = 0; i < n; ++i)
+        f << pts[3 * i] << " " << pts[3 * i + 1] << " " << pts[3 * i + 2] << " " << nrm[3 * i] << " " << nrm[3 * i + 1]
+          << " " << nrm[3 * i + 2] << "\n";
+}
+
+bool read_xyz(const std::string& path, std::vector<float>& pts, std::vector<float>& nrm) {
+    std::ifstream in(path);
+    if (!in) return false;
+    pts.clear();
+    nrm.clear();
+    for (std::string line; std::getline(in, line);) {
+        std::istringstream ls(line);
+        double v[6] = {0, 0, 0, 0, 0, 0};
+        if (!(ls >> v[0] >> v[1] >> v[2])) continue;
+        ls >> v[3] >> v[4] >> v[5];
+        for (int d = 0; d < 3; ++d) {
+            pts.push_back(float(v[d]));
+            nrm.push_back(float(v[3 + d]));
+        }
+    }
+    return true;
+}
+
+// ---------------------------------------------------------------------------
+TurboDWG::TurboDWG(dwg_ctx* ctx, const dwg_options& opt) : ctx_(ctx), opt_(opt) {}
+
+TurboDWG::~TurboDWG() {
+    if (tree_) dwg_octree_destroy(tree_);
+    if (knn_) dwg_knn_destroy(knn_);
+}
+
+void TurboDWG::check(int rc, const char* what) const {
+    if (rc != DWG_OK) throw dwg::Error(rc, std::string(what) + ": " + dwg_ctx_error(ctx_));
+}
+
+void TurboDWG::SetSources(const std::vector<double>& pts, const std::vector<float>& nrm,
+                          const std::vector<float>& areas, const Grid& grid) {
+    n_ = int64_t(pts.size() / 3);
+    pts_host_ = pts;
+    std::vector<float> pf(pts.begin(), pts.end());
+    void* s = ctx_->stream;
+    pts_.upload(pf.data(), 3 * n_, s);
+    nrm_.upload(nrm.data(), 3 * n_, s);
+    area_.upload(areas.data(), n_, s);
+    std::vector<float> r(n_, 0.0015f);
+    radii_.upload(r.data(), n_, s);
+    w_src_.resize(n_);
+    grid_ = grid;
+    w_grid_.resize(grid_.size());
+    std::vector<double> ad(areas.begin(), areas.end());
+    if (tree_) dwg_octree_destroy(tree_);
+    tree_ = nullptr;
+    if (knn_) dwg_knn_destroy(knn_);
+    knn_ = nullptr;
+    check(dwg_octree_build(ctx_, pts.data(), ad.data(), n_, opt_.max_depth, opt_.leaf_size, &tree_), "octree");
+    check(dwg_knn_build(ctx_, pts_.ptr, n_, 15, &knn_), "knn index");
+}
+
+void TurboDWG::UpdateRadii(int k) {
+    check(dwg_radii(ctx_, knn_, pts_.ptr, n_, k, 0.0015f, 0.015f, radii_.ptr), "radii");
+}
+
+void TurboDWG::GaussMapInit() {
+    float lo[3], hi[3];
+    check(dwg_bounds(ctx_, pts_.ptr, n_, lo, hi), "bounds");
+    const float c[3] = {(lo[0] + hi[0]) / 2.0f, (lo[1] + hi[1]) / 2.0f, (lo[2] + hi[2]) / 2.0f};
+    check(dwg_gauss_map(ctx_, pts_.ptr, n_, c, nrm_.ptr), "gauss map");
+}
+
+void TurboDWG::download_normals(std::vector<float>& out) const {
+    out.resize(3 * n_);
+    nrm_.download(out.data(), 3 * n_, ctx_->stream);
+}
+
+void TurboDWG::set_normals(const std::vector<float>& n) { nrm_.upload(n.data(), 3 * n_, ctx_->stream); }
+
+static float sigma_schedule(int it, float sigma_in) {
+    // TurboDWG.cu:526-533
+    float s = sigma_in;
+    if (it < 4) s = 20.0f;
+    if (it > 0 && it % 7 == 0) s = 10.0f;
+    return s;
+}
+
+double TurboDWG::TurboScreenedDWGLaunch(int total_iters, double* w_mean, int64_t* iter_tris, int64_t* iter_zeros) {
+    const auto t0 = Clock::now();
+    if (opt_.As_threshold > 0) check(dwg_outlier_filter(ctx_, area_.ptr, n_, opt_.As_threshold), "outlier filter");
+    const uint32_t flags = DWG_SCREENED | (opt_.use_radii ? DWG_USE_RADII : 0u) | opt_.precision;
+    const dwg_grid g = grid_.abi();
+    DevArray<float> grid_pts;
+    if (opt_.beta <= 0) {  // direct sums need explicit grid points
+        std::vector<float> gp(3 * grid_.size());
+        for (int64_t k = 0; k < grid_.res[2]; ++k)
+            for (int64_t j = 0; j < grid_.res[1]; ++j)
+                for (int64_t i = 0; i < grid_.res[0]; ++i) {
+                    const int64_t id = i + grid_.res[0] * (j + grid_.res[1] * k);
+                    gp[3 * id] = std::fma(float(i), g.h, g.origin[0]);
+                    gp[3 * id + 1] = std::fma(float(j), g.h, g.origin[1]);
+                    gp[3 * id + 2] = std::fma(float(k), g.h, g.origin[2]);
+                }
+        grid_pts.upload(gp.data(), int64_t(gp.size()), ctx_->stream);
+    }
+    int64_t capacity = std::max<int64_t>(cent_.count / 3, 1 << 20);
+    for (int it = 0; it < total_iters; ++it) {
+        const float sigma = sigma_schedule(it, opt_.sigma);
+        const float* rad = opt_.use_radii ? radii_.ptr : nullptr;
+        check(dwg_octree_update(ctx_, tree_, nrm_.ptr, area_.ptr, rad), "moments");
+        if (opt_.beta > 0) {
+            check(dwg_gwn_tree(ctx_, tree_, pts_.ptr, n_, opt_.beta, sigma, flags, w_src_.ptr), "gwn sources");
+            check(dwg_gwn_tree_grid(ctx_, tree_, &g, opt_.beta, sigma, flags, w_grid_.ptr), "gwn grid");
+        } else {
+            check(dwg_gwn_direct(ctx_, pts_.ptr, nrm_.ptr, area_.ptr, rad, n_, pts_.ptr, n_, sigma, flags, w_src_.ptr,
+                                 nullptr), "gwn sources");
+            check(dwg_gwn_direct(ctx_, pts_.ptr, nrm_.ptr, area_.ptr, rad, n_, grid_pts.ptr, grid_.size(), sigma, flags,
+                                 w_grid_.ptr, nullptr), "gwn grid");
+        }
+        double sum = 0;
+        check(dwg_reduce_sum(ctx_, w_src_.ptr, n_, &sum), "reduce");
+        float iso = float(sum / double(n_));
+        if (w_mean) w_mean[it] = double(iso);
+        if (it % 5 == 0 && it != 0) iso *= 0.5f;  // TurboDWG.cu:547-550
+        int64_t ntri = 0;
+        cent_.resize(3 * capacity);
+        tri_nrm_.resize(3 * capacity);
+        int rc = dwg_marching_cubes(ctx_, w_grid_.ptr, &g, iso, capacity, nullptr, cent_.ptr, tri_nrm_.ptr, &ntri);
+        if (rc == DWG_ERR_CAPACITY) {
+            capacity = ntri + ntri / 4 + 1024;
+            cent_.resize(3 * capacity);
+            tri_nrm_.resize(3 * capacity);
+            rc = dwg_marching_cubes(ctx_, w_grid_.ptr, &g, iso, capacity, nullptr, cent_.ptr, tri_nrm_.ptr, &ntri);
+        }
+        check(rc, "marching cubes");
+        if (iter_tris) iter_tris[it] = ntri;
+        if (opt_.test_iter != 999999 && opt_.test_iter > 0 && it % opt_.test_iter == 0 && opt_.verbose)
+            std::printf("iter %d: periodic mesh dumps are written by the command-line tool only\n", it);
+        int64_t nz = 0;
+        check(dwg_diffuse_normals(ctx_, knn_, pts_.ptr, n_, cent_.ptr, tri_nrm_.ptr, ntri, 15, 10, 10, nrm_.ptr, &nz),
+              "diffuse");
+        if (iter_zeros) iter_zeros[it] = nz;
+        if (opt_.verbose) std::printf("iter = %d  W_mean = %.6g  triangles = %lld  zero normals = %lld\n", it,
+                                      double(w_mean ? w_mean[it] : iso), (long long)ntri, (long long)nz);
+    }
+    check(dwg_ctx_sync(ctx_), "sync");
+    return seconds_since(t0);
+}
+
+void TurboDWG::CountTerms(int64_t* src_terms, int64_t* grid_terms) {
+    *src_terms = 0;
+    *grid_terms = 0;
+    if (opt_.beta <= 0) {  // direct sums: every pair
+        *src_terms = n_ * n_;
+        *grid_terms = n_ * grid_.size();
+        return;
+    }
+    check(dwg_octree_update(ctx_, tree_, nrm_.ptr, area_.ptr, nullptr), "moments");
+    DevArray<int64_t> t(n_);
+    check(dwg_gwn_tree_terms(ctx_, tree_, pts_.ptr, n_, opt_.beta, t.ptr), "terms");
+    std::vector<int64_t> h(n_);
+    t.download(h.data(), n_, ctx_->stream);
+    for (int64_t v : h) *src_terms += v;
+    const dwg_grid g = grid_.abi();
+    check(dwg_gwn_tree_grid_terms(ctx_, tree_, &g, opt_.beta, grid_terms), "grid terms");
+}
+
+void TurboDWG::mesh_pass(const Grid& grid, float iso_scale, int64_t* tri_count, std::vector<float>* tris) {
+    // one screened evaluation at the sources and on the grid with the
+    // unscheduled σ, iso = mean source value, marching cubes
+    struct Tag {  // kernels of the final pass are profiled as "<kernel>:final"
+        dwg_ctx* c;
+        explicit Tag(dwg_ctx* x) : c(x) { c->profile_suffix = ":final"; }
+        ~Tag() { c->profile_suffix.clear(); }
+    } tag(ctx_);
+    const uint32_t flags = DWG_SCREENED | (opt_.use_radii ? DWG_USE_RADII : 0u) | opt_.precision;
+    const dwg_grid g = grid.abi();
+    DevArray<float> wg(grid.size());
+    const float* rad = opt_.use_radii ? radii_.ptr : nullptr;
+    check(dwg_octree_update(ctx_, tree_, nrm_.ptr, area_.ptr, rad), "moments");
+    if (opt_.As_threshold > 0) check(dwg_outlier_filter(ctx_, area_.ptr, n_, opt_.As_threshold), "outlier filter");
+    const float beta = opt_.beta > 0 ? opt_.beta : 1e30f;  // β → ∞ takes every point term
+    check(dwg_gwn_tree(ctx_, tree_, pts_.ptr, n_, beta, opt_.sigma, flags, w_src_.ptr), "gwn points");
+    check(dwg_gwn_tree_grid(ctx_, tree_, &g, beta, opt_.sigma, flags, wg.ptr), "gwn final grid");
+    double sum = 0;
+    check(dwg_reduce_sum(ctx_, w_src_.ptr, n_, &sum), "reduce");
+    const float iso = float(sum / double(n_)) * iso_scale;
+    int64_t ntri = 0;
+    int rc = dwg_marching_cubes(ctx_, wg.ptr, &g, iso, 0, nullptr, nullptr, nullptr, &ntri);
+    if (rc != DWG_OK && rc != DWG_ERR_CAPACITY) check(rc, "marching cubes");
+    if (tris) {
+        DevArray<float> t(9 * std::max<int64_t>(ntri, 1));
+        check(dwg_marching_cubes(ctx_, wg.ptr, &g, iso, ntri, t.ptr, nullptr, nullptr, &ntri), "marching cubes");
+        tris->resize(9 * ntri);
+        t.download(tris->data(), 9 * ntri, ctx_->stream);
+    }
+    *tri_count = ntri;
+}
+
+void TurboDWG::GenerateMesh(const std::vector<double>& points, const Grid& final_grid, int64_t* tri_count,
+                            std::vector<float>* tris, std::vector<float>* point_normals) {
+    // normals of the points from their normal_kw nearest sources (LoadNormals)
+    const int64_t np = int64_t(points.size() / 3);
+    std::vector<float> pf(points.begin(), points.end());
+    DevArray<float> dp, dn(3 * np);
+    dp.upload(pf.data(), 3 * np, ctx_->stream);
+    check(dwg_transfer_normals(ctx_, knn_, nrm_.ptr, dp.ptr, np, opt_.normal_kw, dn.ptr), "transfer normals");
+    // the points become the sources (TurboDWG.cu:865-876)
+    std::vector<float> pn(3 * np);
+    dn.download(pn.data(), 3 * np, ctx_->stream);
+    std::vector<float> areas(np, 2e-6f);
+    SetSources(points, pn, areas, grid_);
+    UpdateRadii(opt_.final_kw);
+    mesh_pass(final_grid, 1.0f, tri_count, tris);
+    if (point_normals) *point_normals = pn;
+}
+
+void TurboDWG::GenerateMeshFromSources(const Grid& final_grid, int64_t* tri_count, std::vector<float>* tris) {
+    UpdateRadii(opt_.final_kw);
+    mesh_pass(final_grid, 1.0f, tri_count, tris);
+}
+
+// ---------------------------------------------------------------------------
+namespace test3D {
+
+void NormalizePoints(const float* pts, int64_t n, std::vector<double>& out, double center[3], double* scale) {
+    out.resize(3 * n);
+    double c[3] = {0, 0, 0};
+    for (int64_t i = 0; i < n; ++i)
+        for (int d = 0; d < 3; ++d) c[d] += double(pts[3 * i + d]);
+    for (int d = 0; d < 3; ++d) c[d] /= double(n);
+    double s = 0;
+    for (int64_t i = 0; i < n; ++i)
+        for (int d = 0; d < 3; ++d) s = std::max(s, 2.0 * std::fabs(double(pts[3 * i + d]) - c[d]));
+    if (s <= 0) s = 1;
+    for (int64_t i = 0; i < n; ++i)
+        for (int d = 0; d < 3; ++d) out[3 * i + d] = (double(pts[3 * i + d]) - c[d]) / s;
+    for (int d = 0; d < 3; ++d) center[d] = c[d];
+    *scale = s;
+}
+
+void GetOctreeLeaves(const std::vector<double>& P, int max_depth, int min_num, std::vector<double>& leaves,
+                     std::vector<int>& leaf_node, std::vector<double>& leaf_width,
+                     std::vector<std::vector<int>>& leaf_points) {
+    const int64_t n = int64_t(P.size() / 3);
+    dwg::HostOctree T = dwg::build_octree_host(P.data(), nullptr, n, max_depth, 1);
+    leaves.clear();
+    leaf_node.clear();
+    leaf_width.clear();
+    leaf_points.clear();
+    std::deque<int> q{0};
+    while (!q.empty()) {
+        const int idx = q.front();
+        q.pop_front();
+        if (T.first_child[idx] < 0) {
+            if (T.end[idx] - T.begin[idx] > min_num) {
+                leaves.insert(leaves.end(), {T.cn[3 * idx], T.cn[3 * idx + 1], T.cn[3 * idx + 2]});
+                leaf_node.push_back(idx);
+                leaf_width.push_back(T.w[idx]);
+                std::vector<int> pts(T.perm.begin() + T.begin[idx], T.perm.begin() + T.end[idx]);
+                std::sort(pts.begin(), pts.end());
+                leaf_points.push_back(std::move(pts));
+            }
+            continue;
+        }
+        for (int k = 0; k < 8; ++k) {
+            const int c = T.first_child[idx] + k;
+            if (T.end[c] > T.begin[c]) q.push_back(c);
+        }
+    }
+}
+
+void SmoothNormalstoOctreeLeaves(const std::vector<double>& leaves, const std::vector<double>& leaf_width,
+                                 const std::vector<std::vector<int>>& leaf_points, const std::vector<double>& P,
+                                 const std::vector<double>& N, std::vector<float>& leaf_normals) {
+    const size_t m = leaf_width.size();
+    std::vector<double> out(3 * m, 0.0);
+    // per component: Σ_j exp(−((P_jk − C_k)/W)²/4)·N_jk
+    for (size_t i = 0; i < m; ++i) {
+        const double W = leaf_width[i];
+        for (int p : leaf_points[i])
+            for (int k = 0; k < 3; ++k) {
+                const double x = (P[3 * p + k] - leaves[3 * i + k]) / W;
+                out[3 * i + k] += std::exp(-x * x / 4.0) * N[3 * p + k];
+            }
+    }
+    // whole-matrix Frobenius normalisation (DWGProcess.h:537)
+    double f = 0;
+    for (double v : out) f += v * v;
+    f = std::sqrt(f);
+    leaf_normals.resize(3 * m);
+    for (size_t i = 0; i < 3 * m; ++i) leaf_normals[i] = float(f > 0 ? out[i] / f : 0.0);
+}
+
+void GetRandomNs(int64_t n, uint64_t seed, std::vector<double>& N) {
+    std::mt19937_64 gen(seed);
+    std::uniform_real_distribution<double> dis(0.0, 2.0 * M_PI);
+    N.resize(3 * n);
+    for (int64_t i = 0; i < n; ++i) {
+        const double u = dis(gen), v = dis(gen);
+        N[3 * i] = std::sin(u) * std::cos(v);
+        N[3 * i + 1] = std::sin(u) * std::sin(v);
+        N[3 * i + 2] = std::cos(u);
+    }
+}
+
+void DWGLaunch(dwg_ctx* ctx, const float* pts, const float* nrm, int64_t n, const dwg_options& opt,
+               const char* out_base, dwg_result* res) {
+    if (n <= 0 || !pts) throw dwg::Error(DWG_ERR_ARG, "no input points");
+    if (opt.com_As) throw dwg::Error(DWG_ERR_ARG, "--com_As 1 (Voronoi point areas) is not available in this build");
+    if (opt.input_normal == 1 && !nrm) throw dwg::Error(DWG_ERR_ARG, "--in_normal 1 needs input normals");
+    if (opt.input_normal == 4 && !opt.mlp) throw dwg::Error(DWG_ERR_ARG, "--in_normal 4 needs MLP weights");
+    const auto t0 = Clock::now();
+    std::vector<double> P;
+    double center[3] = {0, 0, 0}, scale = 1;
+    if (opt.normalize_input) {
+        NormalizePoints(pts, n, P, center, &scale);
+    } else {
+        P.assign(pts, pts + 3 * n);
+    }
+    // initial point normals (DWGProcess.h:488-510)
+    std::vector<double> Ns;
+    if (opt.input_normal == 1) {
+        Ns.assign(nrm, nrm + 3 * n);
+    } else {
+        GetRandomNs(n, opt.seed, Ns);
+    }
+    // sources
+    std::vector<double> S;
+    std::vector<float> Sn;
+    if (opt.source_mode == 0) {
+        std::vector<int> leaf_node;
+        std::vector<double> leaf_w;
+        std::vector<std::vector<int>> leaf_pts;
+        GetOctreeLeaves(P, opt.max_depth, opt.min_num, S, leaf_node, leaf_w, leaf_pts);
+        SmoothNormalstoOctreeLeaves(S, leaf_w, leaf_pts, P, Ns, Sn);
+    } else {
+        S = P;
+        Sn.assign(Ns.begin(), Ns.end());
+    }
+    const int64_t ns = int64_t(S.size() / 3);
+    std::vector<float> areas(ns, 2e-6f);  // DWGProcess.h:619
+    // the iteration grid is built over the float input points (DWGProcess.h:662-672)
+    std::vector<double> Pf(P.size());
+    for (size_t i = 0; i < P.size(); ++i) Pf[i] = double(float(P[i]));
+    const Grid grid = voxel_grid(Pf.data(), n, opt.offset_1, opt.grid_reso);
+    TurboDWG dwg(ctx, opt);
+    dwg.SetSources(S, Sn, areas, grid);
+    dwg.UpdateRadii(opt.kw);
+    if (opt.input_normal == 3) dwg.GaussMapInit();
+    if (opt.verbose)
+        std::printf("sources %lld, grid %d x %d x %d\n", (long long)ns, grid.res[0], grid.res[1], grid.res[2]);
+    if (dwg_ctx_sync(ctx) != DWG_OK) throw dwg::Error(DWG_ERR_HIP, dwg_ctx_error(ctx));
+    const double t_setup = seconds_since(t0);
+    if (res) res->grid_points = grid.size();
+    if (opt.count_terms && res) {
+        // outside every timed phase: the term set depends only on the tree and β
+        std::vector<float> keep;
+        dwg.download_normals(keep);
+        dwg.CountTerms(&res->terms_src, &res->terms_grid);
+        dwg.set_normals(keep);
+    }
+    const auto t1 = Clock::now();
+    const double t_iter = dwg.TurboScreenedDWGLaunch(opt.total_iter, res ? res->w_mean : nullptr,
+                                                     res ? res->iter_tris : nullptr, res ? res->iter_zeros : nullptr);
+    const auto t2 = Clock::now();
+    int64_t ntri = 0;
+    std::vector<float> tris, point_nrm;
+    const bool want_tris = (res && res->want_mesh) || (out_base && (opt.output_mode == 1 || opt.output_mode == 3));
+    if (opt.use_leaves_generate == 1) {
+        const Grid fg = voxel_grid(S.data(), ns, 0.01, opt.final_reso);
+        dwg.GenerateMeshFromSources(fg, &ntri, want_tris ? &tris : nullptr);
+        dwg.download_normals(point_nrm);
+    } else {
+        const Grid fg = voxel_grid(Pf.data(), n, opt.offset_2, opt.final_reso);
+        if (opt.source_mode == 0) {
+            dwg.GenerateMesh(P, fg, &ntri, want_tris ? &tris : nullptr, &point_nrm);
+        } else {
+            dwg.GenerateMeshFromSources(fg, &ntri, want_tris ? &tris : nullptr);
+            dwg.download_normals(point_nrm);
+        }
+    }
+    const double t_final = seconds_since(t2);
+    if (out_base) {
+        if (opt.output_mode == 1 || opt.output_mode == 3) save_binary_stl(std::string(out_base) + ".stl", tris.data(), ntri);
+        if (opt.output_mode == 2 || opt.output_mode == 3) {
+            const int64_t np = int64_t(point_nrm.size() / 3);
+            const std::vector<double>& src = (opt.use_leaves_generate == 1) ? S : P;
+            std::vector<float> spf(src.begin(), src.end());
+            save_xyz(std::string(out_base) + ".xyz", spf.data(), point_nrm.data(), np);
+        }
+    }
+    if (res) {
+        if (res->normals && opt.use_leaves_generate != 1)
+            std::copy(point_nrm.begin(), point_nrm.begin() + std::min<size_t>(point_nrm.size(), size_t(3 * n)), res->normals);
+        res->tri_count = ntri;
+        res->mesh = nullptr;
+        if (res->want_mesh && ntri > 0) {
+            res->mesh = static_cast<float*>(std::malloc(sizeof(float) * 9 * size_t(ntri)));
+            if (!res->mesh) throw dwg::Error(DWG_ERR_ARG, "host allocation of the mesh failed");
+            std::copy(tris.begin(), tris.begin() + 9 * ntri, res->mesh);
+        }
+        res->n_sources = ns;
+        for (int d = 0; d < 3; ++d) res->center[d] = center[d];
+        res->scale = scale;
+        res->time_setup_s = t_setup;
+        res->time_iter_s = t_iter;
+        res->time_final_s = t_final;
+        res->time_total_s = seconds_since(t0) - std::chrono::duration<double>(t1 - t0).count() + t_setup;
+    }
+}
+
+}  // namespace test3D
+}  // namespace dwg_host
+
+extern "C" {
+
+void dwg_options_default(dwg_options* o) {
+    if (!o) return;
+    // src/main.cpp:19-39
+    o->input_normal = 3;
+    o->grid_reso = 400;
+    o->final_reso = 500;
+    o->kw = 10;
+    o->final_kw = 10;
+    o->normal_kw = 10;
+    o->max_depth = 10;
+    o->sigma = 10.0f;
+    o->min_num = 0;
+    o->use_radii = 0;
+    o->com_As = 0;
+    o->As_threshold = 0.0f;
+    o->offset_1 = 0.05f;
+    o->offset_2 = 0.01f;
+    o->use_leaves_generate = 0;
+    o->total_iter = 40;
+    o->test_iter = 999999;
+    o->output_mode = 1;
+    o->source_mode = 0;
+    o->leaf_size = 1;
+    o->beta = 2.3f;
+    o->precision = 0;
+    o->seed = 1;
+    o->normalize_input = 1;
+    o->verbose = 0;
+    o->count_terms = 0;
+    o->mlp = nullptr;
+}
+
+void dwg_free(void* p) { std::free(p); }
+
+int dwg_launch(dwg_ctx* ctx, const float* h_pts, const float* h_nrm, int64_t n, const dwg_options* opt,
+               const char* out_base, dwg_result* res) {
+    if (!ctx || !opt) return DWG_ERR_ARG;
+    try {
+        if (hipSetDevice(ctx->device) != hipSuccess) throw dwg::Error(DWG_ERR_HIP, "hipSetDevice failed");
+        dwg_host::test3D::DWGLaunch(ctx, h_pts, h_nrm, n, *opt, out_base, res);
+        ctx->err.clear();
+        return DWG_OK;
+    } catch (const dwg::Error& e) {
+        ctx->err = e.what();
+        return e.code;
+    } catch (const std::exception& e) {
+        ctx->err = e.what();
+        return DWG_ERR_HIP;
+    }
+}
+
+}  // extern "C"

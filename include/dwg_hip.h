@@ -153,6 +153,72 @@ int dwg_reduce_sum(dwg_ctx* ctx, const float* d_vals, int64_t n, double* h_sum);
 /* Axis-aligned bounds of a point array (cukd computeBounds, builder_common.h:76-150). */
 int dwg_bounds(dwg_ctx* ctx, const float* d_pts, int64_t n, float* h_lo, float* h_hi);
 
+/* ---- whole reconstruction ------------------------------------------------ */
+/* Options of DWGLaunch (src/main.cpp:5-15); defaults of src/main.cpp:19-39
+ * via dwg_options_default().  Fields after output_mode have no reference
+ * flag. */
+typedef struct dwg_options {
+    int32_t input_normal;        /* 0 random, 1 input normals, 3 Gauss map, 4 learned MLP */
+    int32_t grid_reso;           /* --grid_reso, 400 */
+    int32_t final_reso;          /* --final_reso, 500 */
+    int32_t kw;                  /* --kw, 10 */
+    int32_t final_kw;            /* --final_kw, 10 */
+    int32_t normal_kw;           /* --normal_kw, 10 */
+    int32_t max_depth;           /* --max_depth, 10 */
+    float sigma;                 /* --lambda, 10 */
+    int32_t min_num;             /* --min_num, 0 */
+    int32_t use_radii;           /* --use_radii, 0 */
+    int32_t com_As;              /* --com_As, 0 */
+    float As_threshold;          /* --As_thres, 0 */
+    float offset_1;              /* --offset_1, 0.05 */
+    float offset_2;              /* --offset_2, 0.01 */
+    int32_t use_leaves_generate; /* --use_leaves_generate, 0 */
+    int32_t total_iter;          /* --total_iter, 40 */
+    int32_t test_iter;           /* --test_iter, 999999 */
+    int32_t output_mode;         /* --output_mode, 1 */
+    int32_t source_mode;         /* 0: octree-leaf centres as the reference, 1: the input points */
+    int32_t leaf_size;           /* octree leaf capacity for the sums (1 = octreeDWG) */
+    float beta;                  /* Barnes-Hut accuracy, 2.3 (DWGProcess.h:689); ≤ 0: direct sums */
+    uint32_t precision;          /* DWG_KAHAN | DWG_FAR_BF16 */
+    uint64_t seed;               /* random initial normals */
+    int32_t normalize_input;     /* 1: centre and scale as ReadPointandNormals (DWGProcess.h:53-97) */
+    int32_t verbose;             /* progress lines on stdout */
+    int32_t count_terms;         /* 1: count the Barnes-Hut terms of one iteration (untimed) */
+    const dwg_mlp* mlp;          /* weights for input_normal 4 */
+} dwg_options;
+
+void dwg_options_default(dwg_options* opt);
+
+typedef struct dwg_result {
+    float* normals;        /* in: n×3 buffer or NULL; out: final point normals (normalised frame) */
+    int32_t want_mesh;     /* in: 1 to receive the final mesh in `mesh` */
+    float* mesh;           /* out: tri_count×9 floats (normalised frame), allocated by the
+                              library; release with dwg_free() */
+    int64_t tri_count;     /* out: triangles of the final mesh */
+    int64_t n_sources;     /* out: sources of the iterations */
+    double center[3];      /* out: input normalisation, p_norm = (p − center)/scale */
+    double scale;
+    double time_setup_s;   /* out: wall time of setup (trees, radii, grid, init) */
+    double time_iter_s;    /* out: wall time of the iterations */
+    double time_final_s;   /* out: wall time of the final mesh pass */
+    double time_total_s;
+    double* w_mean;        /* in: total_iter buffer or NULL; out: mean source WN per iteration */
+    int64_t* iter_tris;    /* in: total_iter buffer or NULL; out: triangles per iteration */
+    int64_t* iter_zeros;   /* in: total_iter buffer or NULL; out: zero normals per iteration */
+    int64_t terms_src;     /* out (count_terms): terms of one source evaluation */
+    int64_t terms_grid;    /* out (count_terms): terms of one grid evaluation */
+    int64_t grid_points;   /* out: points of the iteration grid */
+} dwg_result;
+
+/* Full reconstruction of n host points (DWGLaunch → DataprePointCloud →
+ * MainIterative, src/main.cpp:5-15, DWGProcess.h:99-808).  h_nrm: input
+ * normals for input_normal 1 (else NULL).  out_base (nullable): path prefix
+ * for the STL / .xyz outputs as the reference names them. */
+int dwg_launch(dwg_ctx* ctx, const float* h_pts, const float* h_nrm, int64_t n, const dwg_options* opt,
+               const char* out_base, dwg_result* res);
+/* Releases memory the library allocated for a caller (dwg_result.mesh). */
+void dwg_free(void* p);
+
 #ifdef __cplusplus
 }
 #endif

@@ -267,3 +267,32 @@ def dwg_iterate(P, N0, A, res, origin, h, iters, sigma=10.0, beta=2.3, max_depth
                           int(max_depth), int(leaf_size), float(beta), float(sigma), int(iters),
                           1 if rad is not None else 0, wm, nt, nz)
     return N, wm, nt, nz
+
+
+def octree_leaves(P, max_depth=10, min_num=0):
+    """GetOctreeLeaves (DWGProcess.h:383-428): breadth-first over octreeDWG,
+    the centres of leaf cells holding more than min_num points.
+    Returns (centres [m,3], point index lists)."""
+    from collections import deque
+    P = _c(P, np.float64).reshape(-1, 3)
+    t = Octree(P, max_depth, 1)
+    ch, cnt, _ = t.topology()
+    lo, hi = P.min(0), P.max(0)
+    centres = {0: ((lo + hi) / 2, (hi - lo).max())}
+    q = deque([0])
+    out, pts = [], []
+    while q:
+        i = q.popleft()
+        c, w = centres[i]
+        if ch[i, 0] == -1:
+            if cnt[i] > min_num:
+                out.append(c)
+                pts.append(t.node_points(i))
+            continue
+        for k in range(8):
+            kid = int(ch[i, k])
+            sign = np.array([1.0 if k & 1 else -1.0, 1.0 if k & 2 else -1.0, 1.0 if k & 4 else -1.0])
+            centres[kid] = (c + sign * w / 4, w / 2)
+            if cnt[kid] > 0:
+                q.append(kid)
+    return np.array(out), pts
