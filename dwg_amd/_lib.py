@@ -30,7 +30,9 @@ EXPORTS = [
     "dwg_gwn_tree_terms", "dwg_gwn_tree_grid_terms", "dwg_knn_build", "dwg_knn_destroy", "dwg_knn_query",
     "dwg_marching_cubes", "dwg_diffuse_normals", "dwg_radii", "dwg_gauss_map", "dwg_transfer_normals",
     "dwg_outlier_filter", "dwg_reduce_sum", "dwg_bounds", "dwg_options_default", "dwg_launch",
-    "dwg_free",
+    "dwg_free", "dwg_mlp_num_params", "dwg_mlp_create", "dwg_mlp_load", "dwg_mlp_save", "dwg_mlp_destroy",
+    "dwg_mlp_get", "dwg_mlp_set", "dwg_mlp_patches", "dwg_mlp_forward", "dwg_mlp_loss_grad", "dwg_mlp_adam",
+    "dwg_mlp_augment", "dwg_gemm_bf16",
 ]
 
 
@@ -113,5 +115,20 @@ def load():
     L.dwg_launch.argtypes = [_vp, _vp, _vp, _i64, P(DwgOptions), ctypes.c_char_p, P(DwgResult)]
     L.dwg_free.argtypes = [_vp]
     L.dwg_free.restype = None
+    L.dwg_mlp_num_params.argtypes = []
+    L.dwg_mlp_num_params.restype = _i64
+    L.dwg_mlp_create.argtypes = [_vp, ctypes.c_uint64, P(_vp)]
+    L.dwg_mlp_load.argtypes = [_vp, ctypes.c_char_p, P(_vp)]
+    L.dwg_mlp_save.argtypes = [_vp, _vp, ctypes.c_char_p]
+    L.dwg_mlp_destroy.argtypes = [_vp]
+    L.dwg_mlp_destroy.restype = None
+    L.dwg_mlp_get.argtypes = [_vp, _vp, _vp, _vp]
+    L.dwg_mlp_set.argtypes = [_vp, _vp, _vp]
+    L.dwg_mlp_patches.argtypes = [_vp, _vp, _vp, _i64, _vp]
+    L.dwg_mlp_forward.argtypes = [_vp, _vp, _vp, _i64, _int, _vp]
+    L.dwg_mlp_loss_grad.argtypes = [_vp, _vp, _vp, _vp, _i64, P(ctypes.c_double)]
+    L.dwg_mlp_adam.argtypes = [_vp, _vp, _f, _f, _f, _f]
+    L.dwg_mlp_augment.argtypes = [_vp, _vp, _vp, _vp, _i64, P(_f), _vp, _vp]
+    L.dwg_gemm_bf16.argtypes = [_vp, _vp, _i64, _i64, _vp, _i64, _i64, _vp, _i64, _i64, _i64]
     _lib = L
     return L

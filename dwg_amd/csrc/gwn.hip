@@ -240,65 +240,120 @@ HostOctree build_octree_host(const double* P, const double* A, int64_t n, int ma
             code[i] = k;
         }
     });
+    // stable LSD radix sort of (code, index) by code, 11-bit digits
     T.perm.resize(n);
     std::iota(T.perm.begin(), T.perm.end(), 0);
-    std::stable_sort(T.perm.begin(), T.perm.end(), [&](int a, int b) { return code[a] < code[b]; });
-    std::vector<uint64_t> sc(n);
-    for (int64_t i = 0; i < n; ++i) sc[i] = code[T.perm[i]];
-    // nodes in octreeDWG pre-order
-    T.first_child.push_back(-1);
-    T.begin.push_back(0);
-    T.end.push_back(int(n));
-    T.depth.push_back(0);
-    T.cn.insert(T.cn.end(), {c0[0], c0[1], c0[2]});
-    T.w.push_back(w0);
-    std::function<void(int, int)> split = [&](int idx, int depth) {
-        const int b = T.begin[idx], e = T.end[idx];
-        if (!(e - b > leaf_size && depth < max_depth)) return;
-        const int m = int(T.first_child.size());
-        T.first_child[idx] = m;
-        const int shift = 3 * (max_depth - 1 - depth);
-        int cur = b;
-        const double pc[3] = {T.cn[3 * idx], T.cn[3 * idx + 1], T.cn[3 * idx + 2]};
-        const double q = T.w[idx] / 4;
-        for (int k = 0; k < 8; ++k) {
-            int stop = cur;
-            while (stop < e && int((sc[stop] >> shift) & 7) == k) ++stop;
-            T.first_child.push_back(-1);
-            T.begin.push_back(cur);
-            T.end.push_back(stop);
-            T.depth.push_back(depth + 1);
-            T.cn.insert(T.cn.end(), {pc[0] + ((k & 1) ? q : -q), pc[1] + ((k & 2) ? q : -q), pc[2] + ((k & 4) ? q : -q)});
-            T.w.push_back(T.w[idx] / 2);
-            cur = stop;
+    std::vector<uint64_t> sc(code);
+    {
+        std::vector<uint64_t> k2(n);
+        std::vector<int> v2(n);
+        const int bits = 3 * max_depth;
+        for (int shift = 0; shift < bits; shift += 11) {
+            std::vector<int64_t> cnt(2049, 0);
+            for (int64_t i = 0; i < n; ++i) ++cnt[((sc[i] >> shift) & 2047) + 1];
+            for (int d = 0; d < 2048; ++d) cnt[d + 1] += cnt[d];
+            for (int64_t i = 0; i < n; ++i) {
+                const int64_t dst = cnt[(sc[i] >> shift) & 2047]++;
+                k2[dst] = sc[i];
+                v2[dst] = T.perm[i];
+            }
+            sc.swap(k2);
+            T.perm.swap(v2);
         }
-        for (int k = 0; k < 8; ++k) split(m + k, depth + 1);
-    };
-    split(0, 0);
+    }
+    // positions and areas in sorted order, contiguous for the moment loops
+    std::vector<double> Ps(4 * n);
+    parallel_ranges(n, [&](int64_t b, int64_t e) {
+        for (int64_t i = b; i < e; ++i) {
+            const int p = T.perm[i];
+            Ps[4 * i] = P[3 * p];
+            Ps[4 * i + 1] = P[3 * p + 1];
+            Ps[4 * i + 2] = P[3 * p + 2];
+            Ps[4 * i + 3] = A ? A[p] : 2e-6;
+        }
+    });
+    // Nodes level by level (breadth-first numbering; the 8 children of a node
+    // stay contiguous, which is all the traversal relies on).  Each level is
+    // one parallel pass: split flags, a prefix sum for the child slots, then
+    // every split node cuts its sorted range at the octant digits.
+    T.first_child.assign(1, -1);
+    T.begin.assign(1, 0);
+    T.end.assign(1, int(n));
+    T.depth.assign(1, 0);
+    T.cn.assign({c0[0], c0[1], c0[2]});
+    T.w.assign(1, w0);
+    int64_t ls = 0, le = 1;
+    for (int d = 0; d < max_depth && ls < le; ++d) {
+        std::vector<int64_t> slot(le - ls + 1, 0);
+        for (int64_t i = ls; i < le; ++i) slot[i - ls + 1] = slot[i - ls] + ((T.end[i] - T.begin[i] > leaf_size) ? 1 : 0);
+        const int64_t nsplit = slot[le - ls];
+        if (nsplit == 0) break;
+        const int64_t total = le + 8 * nsplit;
+        T.first_child.resize(total, -1);
+        T.begin.resize(total);
+        T.end.resize(total);
+        T.depth.resize(total);
+        T.cn.resize(3 * total);
+        T.w.resize(total);
+        const int shift = 3 * (max_depth - 1 - d);
+        parallel_ranges(le - ls, [&](int64_t b0, int64_t e0) {
+            for (int64_t j = b0; j < e0; ++j) {
+                const int64_t idx = ls + j;
+                if (slot[j + 1] == slot[j]) continue;
+                const int64_t fc = le + 8 * slot[j];
+                T.first_child[idx] = int(fc);
+                const int b = T.begin[idx], e = T.end[idx];
+                const double q = T.w[idx] / 4;
+                int cur = b;
+                for (int k = 0; k < 8; ++k) {
+                    int stop = cur;
+                    while (stop < e && int((sc[stop] >> shift) & 7) == k) ++stop;
+                    const int64_t c = fc + k;
+                    T.begin[c] = cur;
+                    T.end[c] = stop;
+                    T.depth[c] = d + 1;
+                    T.cn[3 * c] = T.cn[3 * idx] + ((k & 1) ? q : -q);
+                    T.cn[3 * c + 1] = T.cn[3 * idx + 1] + ((k & 2) ? q : -q);
+                    T.cn[3 * c + 2] = T.cn[3 * idx + 2] + ((k & 4) ? q : -q);
+                    T.w[c] = T.w[idx] / 2;
+                    cur = stop;
+                }
+            }
+        });
+        ls = le;
+        le = total;
+    }
     const int64_t m = int64_t(T.first_child.size());
     T.cm.assign(3 * m, 0.0);
     T.r.assign(m, 0.0);
-    // CM = ΣA p / ΣA, R = max |p − CM| (fast_winding_number.cpp:59-110), FP64
-    parallel_ranges(m, [&](int64_t b, int64_t e) {
-        for (int64_t i = b; i < e; ++i) {
-            const int pb = T.begin[i], pe = T.end[i];
-            if (pb == pe) continue;
-            double at = 0, c[3] = {0, 0, 0};
-            for (int s = pb; s < pe; ++s) {
-                const int p = T.perm[s];
-                const double a = A ? A[p] : 2e-6;
-                at += a;
-                for (int d = 0; d < 3; ++d) c[d] += a * P[3 * p + d];
+    // CM = ΣA p / ΣA, R = max |p − CM| (fast_winding_number.cpp:59-110), FP64.
+    // Node costs are uneven (the root spans every point), so threads take
+    // blocks of nodes from a shared counter.
+    std::atomic<int64_t> next{0};
+    const int64_t blk = 2048;
+    parallel_ranges(int64_t(std::max(1u, std::thread::hardware_concurrency())) * 20000, [&](int64_t, int64_t) {
+        for (;;) {
+            const int64_t b = next.fetch_add(blk);
+            if (b >= m) return;
+            const int64_t e = std::min(m, b + blk);
+            for (int64_t i = b; i < e; ++i) {
+                const int pb = T.begin[i], pe = T.end[i];
+                if (pb == pe) continue;
+                double at = 0, c[3] = {0, 0, 0};
+                for (int s = pb; s < pe; ++s) {
+                    const double a = Ps[4 * s + 3];
+                    at += a;
+                    for (int d = 0; d < 3; ++d) c[d] += a * Ps[4 * s + d];
+                }
+                for (int d = 0; d < 3; ++d) c[d] /= at;
+                double rm = 0;
+                for (int s = pb; s < pe; ++s) {
+                    const double dx = Ps[4 * s] - c[0], dy = Ps[4 * s + 1] - c[1], dz = Ps[4 * s + 2] - c[2];
+                    rm = std::max(rm, std::sqrt(dx * dx + dy * dy + dz * dz));
+                }
+                for (int d = 0; d < 3; ++d) T.cm[3 * i + d] = c[d];
+                T.r[i] = rm;
             }
-            for (int d = 0; d < 3; ++d) c[d] /= at;
-            double rm = 0;
-            for (int s = pb; s < pe; ++s) {
-                const int p = T.perm[s];
-                const double dx = P[3 * p] - c[0], dy = P[3 * p + 1] - c[1], dz = P[3 * p + 2] - c[2];
-                rm = std::max(rm, std::sqrt(dx * dx + dy * dy + dz * dz));
-            }
-            for (int d = 0; d < 3; ++d) T.cm[3 * i + d] = c[d];
-            T.r[i] = rm;
         }
     });
     T.n_points = n;

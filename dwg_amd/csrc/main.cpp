@@ -103,6 +103,15 @@ int main(int argc, char* argv[]) {
         std::cerr << "no HIP device\n";
         return 3;
     }
+    dwg_mlp* mlp = nullptr;
+    if (!mlp_path.empty()) {
+        if (dwg_mlp_load(ctx, mlp_path.c_str(), &mlp) != DWG_OK) {
+            std::cerr << "cannot load MLP weights " << mlp_path << ": " << dwg_ctx_error(ctx) << "\n";
+            dwg_ctx_destroy(ctx);
+            return 4;
+        }
+        opt.mlp = mlp;
+    }
     dwg_result res;
     std::memset(&res, 0, sizeof(res));
     const int rc = dwg_launch(ctx, pts.data(), opt.input_normal == 1 ? nrm.data() : nullptr, n, &opt, out_base.c_str(),
@@ -114,6 +123,7 @@ int main(int argc, char* argv[]) {
         std::printf("Prepare Time = %.3f s\nIter Time = %.3f s\nFinal Time = %.3f s\nTotal Time (Wall time) = %.3f s\n",
                     res.time_setup_s, res.time_iter_s, res.time_final_s, res.time_total_s);
     }
+    if (mlp) dwg_mlp_destroy(mlp);
     dwg_ctx_destroy(ctx);
     return rc == DWG_OK ? 0 : 5;
 }

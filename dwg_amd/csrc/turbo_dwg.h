@@ -10,6 +10,10 @@
 
 #include "../../include/dwg_hip.h"
 
+namespace dwg {
+struct HostOctree;
+}
+
 namespace dwg_host {
 
 // Device array owned by the host driver.
@@ -58,18 +62,23 @@ public:
     ~TurboDWG();
 
     // TurboDWG::TurboDWG (TurboDWG.cu:14-83): sources, normals, areas, radii, grid.
+    // `prebuilt` (nullable): an octree already built over exactly these points
+    // with constant areas and leaf size 1.
     void SetSources(const std::vector<double>& pts, const std::vector<float>& nrm, const std::vector<float>& areas,
-                    const Grid& grid);
+                    const Grid& grid, const dwg::HostOctree* prebuilt = nullptr);
     // Radii of the sources from their k nearest neighbours (GetRadiiandAs /
     // TurboDWG::UpdateRadii).
     void UpdateRadii(int k);
     void GaussMapInit();
+    // learned initialisation: MLP normals from the k-NN patches of the sources
+    void MlpInit(const dwg_mlp* mlp);
     // TurboScreenedDWGLaunch (TurboDWG.cu:456-672); returns wall seconds.
     double TurboScreenedDWGLaunch(int total_iters, double* w_mean, int64_t* iter_tris, int64_t* iter_zeros);
     // GenerateMesh (TurboDWG.cu:808-924): transfer the source normals to the
     // points, one screened evaluation at the points and on the final grid, MC.
     void GenerateMesh(const std::vector<double>& points, const Grid& final_grid, int64_t* tri_count,
-                      std::vector<float>* tris, std::vector<float>* point_normals);
+                      std::vector<float>* tris, std::vector<float>* point_normals,
+                      const dwg::HostOctree* points_tree = nullptr);
     // GenerateMesh(grid) variant of --use_leaves_generate 1 (TurboDWG.cu:768-806).
     void GenerateMeshFromSources(const Grid& final_grid, int64_t* tri_count, std::vector<float>* tris);
 
@@ -88,7 +97,6 @@ private:
     dwg_ctx* ctx_;
     dwg_options opt_;
     int64_t n_ = 0;
-    std::vector<double> pts_host_;
     DevArray<float> pts_, nrm_, area_, radii_, w_src_, w_grid_, cent_, tri_nrm_;
     Grid grid_{};
     dwg_octree* tree_ = nullptr;
@@ -106,7 +114,7 @@ void NormalizePoints(const float* pts, int64_t n, std::vector<double>& out, doub
 // than min_num points; returns cell centres, cell ids.
 void GetOctreeLeaves(const std::vector<double>& P, int max_depth, int min_num, std::vector<double>& leaves,
                      std::vector<int>& leaf_node, std::vector<double>& leaf_width,
-                     std::vector<std::vector<int>>& leaf_points);
+                     std::vector<std::vector<int>>& leaf_points, dwg::HostOctree* tree_out = nullptr);
 
 // SmoothNormalstoOctreeLeaves (DWGProcess.h:430-472) + the whole-matrix
 // Frobenius normalisation of DWGProcess.h:537.

@@ -58,6 +58,7 @@ void DevArray<T>::download(T* h, int64_t n, void* stream) const {
 template struct DevArray<float>;
 template struct DevArray<int32_t>;
 template struct DevArray<int64_t>;
+template struct DevArray<uint16_t>;
 
 dwg_grid Grid::abi() const {
     dwg_grid g;
@@ -171,9 +172,8 @@ void TurboDWG::check(int rc, const char* what) const {
 }
 
 void TurboDWG::SetSources(const std::vector<double>& pts, const std::vector<float>& nrm,
-                          const std::vector<float>& areas, const Grid& grid) {
+                          const std::vector<float>& areas, const Grid& grid, const dwg::HostOctree* prebuilt) {
     n_ = int64_t(pts.size() / 3);
-    pts_host_ = pts;
     std::vector<float> pf(pts.begin(), pts.end());
     void* s = ctx_->stream;
     pts_.upload(pf.data(), 3 * n_, s);
@@ -189,7 +189,11 @@ void TurboDWG::SetSources(const std::vector<double>& pts, const std::vector<floa
     tree_ = nullptr;
     if (knn_) dwg_knn_destroy(knn_);
     knn_ = nullptr;
-    check(dwg_octree_build(ctx_, pts.data(), ad.data(), n_, opt_.max_depth, opt_.leaf_size, &tree_), "octree");
+    if (prebuilt && opt_.leaf_size == 1 && prebuilt->n_points == n_) {
+        tree_ = dwg::octree_upload(ctx_, *prebuilt);
+    } else {
+        check(dwg_octree_build(ctx_, pts.data(), ad.data(), n_, opt_.max_depth, opt_.leaf_size, &tree_), "octree");
+    }
     check(dwg_knn_build(ctx_, pts_.ptr, n_, 15, &knn_), "knn index");
 }
 
@@ -202,6 +206,12 @@ void TurboDWG::GaussMapInit() {
     check(dwg_bounds(ctx_, pts_.ptr, n_, lo, hi), "bounds");
     const float c[3] = {(lo[0] + hi[0]) / 2.0f, (lo[1] + hi[1]) / 2.0f, (lo[2] + hi[2]) / 2.0f};
     check(dwg_gauss_map(ctx_, pts_.ptr, n_, c, nrm_.ptr), "gauss map");
+}
+
+void TurboDWG::MlpInit(const dwg_mlp* mlp) {
+    DevArray<uint16_t> patches(48 * n_);
+    check(dwg_mlp_patches(ctx_, knn_, pts_.ptr, n_, patches.ptr), "mlp patches");
+    check(dwg_mlp_forward(ctx_, mlp, patches.ptr, n_, 1, nrm_.ptr), "mlp forward");
 }
 
 void TurboDWG::download_normals(std::vector<float>& out) const {
@@ -332,7 +342,8 @@ void TurboDWG::mesh_pass(const Grid& grid, float iso_scale, int64_t* tri_count, 
 }
 
 void TurboDWG::GenerateMesh(const std::vector<double>& points, const Grid& final_grid, int64_t* tri_count,
-                            std::vector<float>* tris, std::vector<float>* point_normals) {
+                            std::vector<float>* tris, std::vector<float>* point_normals,
+                            const dwg::HostOctree* points_tree) {
     // normals of the points from their normal_kw nearest sources (LoadNormals)
     const int64_t np = int64_t(points.size() / 3);
     std::vector<float> pf(points.begin(), points.end());
@@ -343,7 +354,8 @@ void TurboDWG::GenerateMesh(const std::vector<double>& points, const Grid& final
     std::vector<float> pn(3 * np);
     dn.download(pn.data(), 3 * np, ctx_->stream);
     std::vector<float> areas(np, 2e-6f);
-    SetSources(points, pn, areas, grid_);
+    // the final pass reuses the octree over the input points (TurboDWG.cu:869-870)
+    SetSources(points, pn, areas, grid_, points_tree);
     UpdateRadii(opt_.final_kw);
     mesh_pass(final_grid, 1.0f, tri_count, tris);
     if (point_normals) *point_normals = pn;
@@ -375,9 +387,10 @@ void NormalizePoints(const float* pts, int64_t n, std::vector<double>& out, doub
 
 void GetOctreeLeaves(const std::vector<double>& P, int max_depth, int min_num, std::vector<double>& leaves,
                      std::vector<int>& leaf_node, std::vector<double>& leaf_width,
-                     std::vector<std::vector<int>>& leaf_points) {
+                     std::vector<std::vector<int>>& leaf_points, dwg::HostOctree* tree_out) {
     const int64_t n = int64_t(P.size() / 3);
     dwg::HostOctree T = dwg::build_octree_host(P.data(), nullptr, n, max_depth, 1);
+    T.P = P.data();
     leaves.clear();
     leaf_node.clear();
     leaf_width.clear();
@@ -402,6 +415,7 @@ void GetOctreeLeaves(const std::vector<double>& P, int max_depth, int min_num, s
             if (T.end[c] > T.begin[c]) q.push_back(c);
         }
     }
+    if (tree_out) *tree_out = std::move(T);
 }
 
 void SmoothNormalstoOctreeLeaves(const std::vector<double>& leaves, const std::vector<double>& leaf_width,
@@ -462,12 +476,17 @@ void DWGLaunch(dwg_ctx* ctx, const float* pts, const float* nrm, int64_t n, cons
     // sources
     std::vector<double> S;
     std::vector<float> Sn;
+    dwg::HostOctree points_tree;  // octree over the input points, reused by the final pass
     if (opt.source_mode == 0) {
         std::vector<int> leaf_node;
         std::vector<double> leaf_w;
         std::vector<std::vector<int>> leaf_pts;
-        GetOctreeLeaves(P, opt.max_depth, opt.min_num, S, leaf_node, leaf_w, leaf_pts);
-        SmoothNormalstoOctreeLeaves(S, leaf_w, leaf_pts, P, Ns, Sn);
+        GetOctreeLeaves(P, opt.max_depth, opt.min_num, S, leaf_node, leaf_w, leaf_pts, &points_tree);
+        if (opt.input_normal == 3 || opt.input_normal == 4) {
+            Sn.assign(S.size(), 0.0f);  // overwritten by the initialisation below
+        } else {
+            SmoothNormalstoOctreeLeaves(S, leaf_w, leaf_pts, P, Ns, Sn);
+        }
     } else {
         S = P;
         Sn.assign(Ns.begin(), Ns.end());
@@ -482,6 +501,7 @@ void DWGLaunch(dwg_ctx* ctx, const float* pts, const float* nrm, int64_t n, cons
     dwg.SetSources(S, Sn, areas, grid);
     dwg.UpdateRadii(opt.kw);
     if (opt.input_normal == 3) dwg.GaussMapInit();
+    if (opt.input_normal == 4) dwg.MlpInit(opt.mlp);
     if (opt.verbose)
         std::printf("sources %lld, grid %d x %d x %d\n", (long long)ns, grid.res[0], grid.res[1], grid.res[2]);
     if (dwg_ctx_sync(ctx) != DWG_OK) throw dwg::Error(DWG_ERR_HIP, dwg_ctx_error(ctx));
@@ -508,7 +528,7 @@ void DWGLaunch(dwg_ctx* ctx, const float* pts, const float* nrm, int64_t n, cons
     } else {
         const Grid fg = voxel_grid(Pf.data(), n, opt.offset_2, opt.final_reso);
         if (opt.source_mode == 0) {
-            dwg.GenerateMesh(P, fg, &ntri, want_tris ? &tris : nullptr, &point_nrm);
+            dwg.GenerateMesh(P, fg, &ntri, want_tris ? &tris : nullptr, &point_nrm, &points_tree);
         } else {
             dwg.GenerateMeshFromSources(fg, &ntri, want_tris ? &tris : nullptr);
             dwg.download_normals(point_nrm);

@@ -261,3 +261,76 @@ def transfer_normals(ctx: Context, index: KnnIndex, src_nrm, q, k=10):
 def outlier_filter(ctx: Context, area, threshold):
     ctx.check(ctx.L.dwg_outlier_filter(ctx.h, _ptr(area), area.numel(), float(threshold)))
     return area
+
+
+class Mlp:
+    """dwg_mlp: the learned-initialisation network (BF16 on the matrix cores)."""
+
+    def __init__(self, ctx: Context, seed: int = 0, path: str | None = None):
+        self.ctx = ctx
+        h = ctypes.c_void_p()
+        if path is None:
+            ctx.check(ctx.L.dwg_mlp_create(ctx.h, int(seed), ctypes.byref(h)))
+        else:
+            ctx.check(ctx.L.dwg_mlp_load(ctx.h, path.encode(), ctypes.byref(h)))
+        self.h = h
+        self.n_params = int(ctx.L.dwg_mlp_num_params())
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            self.ctx.L.dwg_mlp_destroy(self.h)
+            self.h = None
+
+    def save(self, path):
+        self.ctx.check(self.ctx.L.dwg_mlp_save(self.ctx.h, self.h, path.encode()))
+
+    def params(self):
+        out = np.zeros(self.n_params, np.float32)
+        self.ctx.check(self.ctx.L.dwg_mlp_get(self.ctx.h, self.h, out.ctypes.data, None))
+        return out
+
+    def grad(self):
+        out = np.zeros(self.n_params, np.float32)
+        self.ctx.check(self.ctx.L.dwg_mlp_get(self.ctx.h, self.h, None, out.ctypes.data))
+        return out
+
+    def set_params(self, theta):
+        t = np.ascontiguousarray(theta, np.float32)
+        assert t.size == self.n_params
+        self.ctx.check(self.ctx.L.dwg_mlp_set(self.ctx.h, self.h, t.ctypes.data))
+
+    def patches(self, index: KnnIndex):
+        out = torch.empty((index.n, 48), device=self.ctx.device, dtype=torch.int16)
+        self.ctx.check(self.ctx.L.dwg_mlp_patches(self.ctx.h, index.h, _ptr(index.pts), index.n, _ptr(out)))
+        return out
+
+    def forward(self, patches, normalize=True):
+        n = patches.shape[0]
+        out = torch.empty((n, 3), device=self.ctx.device, dtype=torch.float32)
+        self.ctx.check(self.ctx.L.dwg_mlp_forward(self.ctx.h, self.h, _ptr(patches), n, 1 if normalize else 0,
+                                                  _ptr(out)))
+        return out
+
+    def loss_grad(self, patches, target):
+        loss = ctypes.c_double()
+        t = target.to(device=self.ctx.device, dtype=torch.float32).contiguous()
+        self.ctx.check(self.ctx.L.dwg_mlp_loss_grad(self.ctx.h, self.h, _ptr(patches), _ptr(t), patches.shape[0],
+                                                    ctypes.byref(loss)))
+        return loss.value
+
+    def adam(self, lr=1e-3, beta1=0.9, beta2=0.999, eps=1e-8):
+        self.ctx.check(self.ctx.L.dwg_mlp_adam(self.ctx.h, self.h, float(lr), float(beta1), float(beta2), float(eps)))
+
+    def augment(self, patches, target, index, rotation, out_patches, out_target):
+        R = (ctypes.c_float * 9)(*[float(v) for v in np.asarray(rotation, np.float32).ravel()])
+        self.ctx.check(self.ctx.L.dwg_mlp_augment(self.ctx.h, _ptr(patches), _ptr(target), _ptr(index),
+                                                  index.shape[0], R, _ptr(out_patches), _ptr(out_target)))
+
+
+def gemm_bf16(ctx: Context, a16, sa, b16, sb, m, n, k):
+    """C = A·B (FP32) with BF16 operands given as int16 bit patterns and
+    element strides sa = (s_ai, s_ak), sb = (s_bk, s_bj)."""
+    c = torch.empty((m, n), device=ctx.device, dtype=torch.float32)
+    ctx.check(ctx.L.dwg_gemm_bf16(ctx.h, _ptr(a16), int(sa[0]), int(sa[1]), _ptr(b16), int(sb[0]), int(sb[1]),
+                                  _ptr(c), int(m), int(n), int(k)))
+    return c

@@ -153,6 +153,41 @@ int dwg_reduce_sum(dwg_ctx* ctx, const float* d_vals, int64_t n, double* h_sum);
 /* Axis-aligned bounds of a point array (cukd computeBounds, builder_common.h:76-150). */
 int dwg_bounds(dwg_ctx* ctx, const float* d_pts, int64_t n, float* h_lo, float* h_hi);
 
+/* ---- learned normal initialisation --------------------------------------- */
+/* New (no reference counterpart; the reference initialises with random or
+ * Gauss-map normals, DWGProcess.h:264-281, TurboNormalInit.cu:3-27): a
+ * 48→128→128→3 GELU MLP over the 16-nearest-neighbour patch of a point
+ * (offsets scaled by the 16th-neighbour distance, distance order) predicts
+ * an oriented normal.  BF16 weights and activations, FP32 accumulation on
+ * the matrix cores, FP32 master weights and Adam state.  Parameter vector
+ * layout: W1[128][48], b1[128], W2[128][128], b2[128], W3[3][128], b3[3]. */
+int64_t dwg_mlp_num_params(void);
+int dwg_mlp_create(dwg_ctx* ctx, uint64_t seed, dwg_mlp** out);
+int dwg_mlp_load(dwg_ctx* ctx, const char* path, dwg_mlp** out);
+int dwg_mlp_save(dwg_ctx* ctx, const dwg_mlp* mlp, const char* path);
+void dwg_mlp_destroy(dwg_mlp* mlp);
+/* host copies of the FP32 parameters / last gradient (either may be NULL) */
+int dwg_mlp_get(dwg_ctx* ctx, const dwg_mlp* mlp, float* h_params, float* h_grad);
+int dwg_mlp_set(dwg_ctx* ctx, dwg_mlp* mlp, const float* h_params);
+/* n×48 BF16 patches (uint16 bit patterns) of the indexed points */
+int dwg_mlp_patches(dwg_ctx* ctx, const dwg_knn_index* index, const float* d_pts, int64_t n,
+                    uint16_t* d_patches);
+/* n×3 outputs, unit length when normalize != 0 */
+int dwg_mlp_forward(dwg_ctx* ctx, const dwg_mlp* mlp, const uint16_t* d_patches, int64_t n, int normalize,
+                    float* d_out);
+/* mean(1 − ŷ·t) over the batch and its gradient (kept in the MLP) */
+int dwg_mlp_loss_grad(dwg_ctx* ctx, dwg_mlp* mlp, const uint16_t* d_patches, const float* d_target,
+                      int64_t batch, double* h_loss);
+int dwg_mlp_adam(dwg_ctx* ctx, dwg_mlp* mlp, float lr, float beta1, float beta2, float eps);
+/* gathers batch rows d_index of (patches, targets) and rotates both by the
+ * row-major 3×3 h_rotation (training-time augmentation) */
+int dwg_mlp_augment(dwg_ctx* ctx, const uint16_t* d_patches, const float* d_target, const int64_t* d_index,
+                    int64_t batch, const float* h_rotation, uint16_t* d_patches_out, float* d_target_out);
+/* C[m×n] = A·B, BF16 operands with element strides, FP32 result (row-major
+ * C); the matrix-core product every MLP layer uses */
+int dwg_gemm_bf16(dwg_ctx* ctx, const uint16_t* d_a, int64_t s_ai, int64_t s_ak, const uint16_t* d_b,
+                  int64_t s_bk, int64_t s_bj, float* d_c, int64_t m, int64_t n, int64_t k);
+
 /* ---- whole reconstruction ------------------------------------------------ */
 /* Options of DWGLaunch (src/main.cpp:5-15); defaults of src/main.cpp:19-39
  * via dwg_options_default().  Fields after output_mode have no reference
