@@ -510,7 +510,7 @@ __device__ __forceinline__ int64_t xcd_block(int64_t b, int64_t nb) {
     return (b % 8) * per + b / 8;
 }
 
-template <bool SCREENED, bool COUNT, bool BF16>
+template <bool SCREENED, bool COUNT, bool BF16, bool GRID>
 __global__ __launch_bounds__(kTreeThreads) void gwn_tree_kernel(TreeArgs T, QueryArgs Q, float beta, float sigma,
                                                                  uint32_t flags, float* __restrict__ out,
                                                                  int64_t* __restrict__ terms_out) {
@@ -522,7 +522,7 @@ __global__ __launch_bounds__(kTreeThreads) void gwn_tree_kernel(TreeArgs T, Quer
     int64_t qi;
     bool valid;
     float qx, qy, qz;
-    if (Q.q) {
+    if (!GRID) {
         qi = wid * 64 + lane;
         valid = qi < Q.nq;
         qx = valid ? Q.q[3 * qi] : 0.f;
@@ -573,9 +573,17 @@ __global__ __launch_bounds__(kTreeThreads) void gwn_tree_kernel(TreeArgs T, Quer
         const bool act = (mask >> lane) & 1;
         const int fc = uniform(T.info[node].x);
         float part = 0.f;
+        // the 8 children are contiguous: fetch their records together
+        int4 nfs[8];
+        float4 crs[8];
+#pragma unroll
+        for (int c = 0; c < 8; ++c) nfs[c] = T.info[fc + c];
+#pragma unroll
+        for (int c = 0; c < 8; ++c) crs[c] = T.cm_r[fc + c];
+#pragma unroll
         for (int c = 0; c < 8; ++c) {
             const int ch = fc + c;
-            const int4 nf = T.info[ch];
+            const int4 nf = nfs[c];
             if (nf.z == nf.y) continue;
             if (nf.x < 0) {  // leaf child: summed point by point whether near or far
                 for (int j = nf.y; j < nf.z; ++j) {
@@ -585,7 +593,7 @@ __global__ __launch_bounds__(kTreeThreads) void gwn_tree_kernel(TreeArgs T, Quer
                 if (COUNT && act) nterms += nf.z - nf.y;
                 continue;
             }
-            const float4 cr = T.cm_r[ch];
+            const float4 cr = crs[c];
             const float dx = cr.x - qx, dy = cr.y - qy, dz = cr.z - qz;
             const float br = beta * cr.w;
             const bool far = dist2(dx, dy, dz) > br * br;
@@ -661,13 +669,21 @@ static void launch_tree(dwg_ctx* ctx, const dwg_octree* T, const QueryArgs& Q, f
     const char* name = count ? "gwn_tree_terms" : (Q.q ? (scr ? "gwn_tree_screened" : "gwn_tree_plain")
                                                        : (scr ? "gwn_tree_grid_screened" : "gwn_tree_grid_plain"));
     ProfileScope ps(ctx, name);
-#define DWG_TREE_LAUNCH(S, C, B) \
-    hipLaunchKernelGGL((gwn_tree_kernel<S, C, B>), grid, block, 0, ctx->stream, A, Q, beta, sigma, flags, out, terms)
-    if (count) DWG_TREE_LAUNCH(false, true, false);
-    else if (scr && bf) DWG_TREE_LAUNCH(true, false, true);
-    else if (scr) DWG_TREE_LAUNCH(true, false, false);
-    else if (bf) DWG_TREE_LAUNCH(false, false, true);
-    else DWG_TREE_LAUNCH(false, false, false);
+#define DWG_TREE_LAUNCH(S, C, B, G) \
+    hipLaunchKernelGGL((gwn_tree_kernel<S, C, B, G>), grid, block, 0, ctx->stream, A, Q, beta, sigma, flags, out, terms)
+#define DWG_TREE_LAUNCH_Q(S, C, B)       \
+    do {                                 \
+        if (Q.q)                         \
+            DWG_TREE_LAUNCH(S, C, B, false); \
+        else                             \
+            DWG_TREE_LAUNCH(S, C, B, true);  \
+    } while (0)
+    if (count) DWG_TREE_LAUNCH_Q(false, true, false);
+    else if (scr && bf) DWG_TREE_LAUNCH_Q(true, false, true);
+    else if (scr) DWG_TREE_LAUNCH_Q(true, false, false);
+    else if (bf) DWG_TREE_LAUNCH_Q(false, false, true);
+    else DWG_TREE_LAUNCH_Q(false, false, false);
+#undef DWG_TREE_LAUNCH_Q
 #undef DWG_TREE_LAUNCH
     check_launch();
 }
