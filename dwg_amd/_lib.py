@@ -25,7 +25,8 @@ DWG_FAR_BF16 = 8
 
 EXPORTS = [
     "dwg_ctx_create", "dwg_ctx_destroy", "dwg_ctx_error", "dwg_ctx_sync", "dwg_ctx_profile",
-    "dwg_ctx_profile_get", "dwg_ctx_profile_reset", "dwg_gwn_direct", "dwg_octree_build",
+    "dwg_ctx_profile_get", "dwg_ctx_profile_reset", "dwg_gwn_direct", "dwg_octree_build", "dwg_octree_build_device",
+    "dwg_octree_leaves",
     "dwg_octree_destroy", "dwg_octree_info", "dwg_octree_update", "dwg_gwn_tree", "dwg_gwn_tree_grid",
     "dwg_gwn_tree_terms", "dwg_gwn_tree_grid_terms", "dwg_knn_build", "dwg_knn_destroy", "dwg_knn_query",
     "dwg_marching_cubes", "dwg_diffuse_normals", "dwg_radii", "dwg_gauss_map", "dwg_transfer_normals",
@@ -90,7 +91,9 @@ def load():
     L.dwg_ctx_profile_reset.argtypes = [_vp]
     L.dwg_gwn_direct.argtypes = [_vp, _vp, _vp, _vp, _vp, _i64, _vp, _i64, _f, _u32, _vp, _vp]
     L.dwg_octree_build.argtypes = [_vp, _vp, _vp, _i64, _int, _int, P(_vp)]
+    L.dwg_octree_build_device.argtypes = [_vp, _vp, _vp, _i64, _int, _int, P(_vp)]
     L.dwg_octree_destroy.argtypes = [_vp]
+    L.dwg_octree_leaves.argtypes = [_vp, _vp, _int, _vp, _vp, P(_i64)]
     L.dwg_octree_destroy.restype = None
     L.dwg_octree_info.argtypes = [_vp, P(_i64), P(_i64), P(_int)]
     L.dwg_octree_update.argtypes = [_vp, _vp, _vp, _vp, _vp]

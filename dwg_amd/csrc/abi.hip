@@ -7,6 +7,7 @@
 #include "gwn.h"
 #include "knn.h"
 #include "mc.h"
+#include "octree.h"
 #include "scan.h"
 
 namespace {
@@ -139,14 +140,39 @@ int dwg_octree_build(dwg_ctx* ctx, const double* h_pts, const double* h_area, in
     *out = nullptr;
     return guarded(ctx, [&] {
         DWG_REQUIRE(h_pts, "null points");
+        DWG_REQUIRE(n > 0, "octree needs at least one point");
         set_device(ctx);
-        dwg::HostOctree H = dwg::build_octree_host(h_pts, h_area, n, max_depth, leaf_size);
-        H.P = h_pts;
-        *out = dwg::octree_upload(ctx, H);
+        double* p = static_cast<double*>(ctx->buf("octree_upload_pts", sizeof(double) * 3 * size_t(n)));
+        double* a = h_area ? static_cast<double*>(ctx->buf("octree_upload_area", sizeof(double) * size_t(n))) : nullptr;
+        DWG_HIP_CHECK(hipMemcpyAsync(p, h_pts, sizeof(double) * 3 * size_t(n), hipMemcpyHostToDevice, ctx->stream));
+        if (a) DWG_HIP_CHECK(hipMemcpyAsync(a, h_area, sizeof(double) * size_t(n), hipMemcpyHostToDevice, ctx->stream));
+        *out = dwg::octree_build(ctx, p, a, n, max_depth, leaf_size);
+    });
+}
+
+int dwg_octree_build_device(dwg_ctx* ctx, const double* d_pts, const double* d_area, int64_t n, int max_depth,
+                            int leaf_size, dwg_octree** out) {
+    if (!ctx || !out) return DWG_ERR_ARG;
+    *out = nullptr;
+    return guarded(ctx, [&] {
+        DWG_REQUIRE(d_pts, "null points");
+        set_device(ctx);
+        *out = dwg::octree_build(ctx, d_pts, d_area, n, max_depth, leaf_size);
     });
 }
 
 void dwg_octree_destroy(dwg_octree* tree) { delete tree; }
+
+int dwg_octree_leaves(dwg_ctx* ctx, const dwg_octree* tree, int min_num, double* d_centres, double* d_widths,
+                      int64_t* out_count) {
+    if (!ctx || !tree || !out_count) return DWG_ERR_ARG;
+    return guarded(ctx, [&] {
+        DWG_REQUIRE(d_centres, "null output");
+        DWG_REQUIRE(min_num >= 0, "min_num must be ≥ 0");
+        set_device(ctx);
+        *out_count = dwg::octree_leaves(ctx, tree, min_num, d_centres, d_widths, nullptr);
+    });
+}
 
 int dwg_octree_info(const dwg_octree* tree, int64_t* n_nodes, int64_t* n_leaves, int* max_level) {
     if (!tree) return DWG_ERR_ARG;

@@ -5,26 +5,13 @@
 
 namespace dwg {
 
-// Octree in octreeDWG's node numbering, points sorted into contiguous node
-// ranges.  P points to the caller's host doubles (not owned).
-struct HostOctree {
-    std::vector<int> first_child, begin, end, depth;
-    std::vector<double> cm, r;  // 3 per node, 1 per node
-    std::vector<double> cn, w;  // cell centre (3 per node) and cell width
-    std::vector<int> perm;      // packed slot → caller index
-    const double* P = nullptr;
-    int64_t n_points = 0, n_leaves = 0;
-    int max_level = 0;
-};
-
-HostOctree build_octree_host(const double* P, const double* A, int64_t n, int max_depth, int leaf_size);
-
 }  // namespace dwg
 
 struct dwg_octree {
     int64_t n_nodes = 0, n_points = 0, n_leaves = 0;
-    int max_level = 0;
+    int max_level = 0, max_depth = 0;
     bool updated = false;
+    std::vector<int64_t> level_start;  // nodes of depth d: [level_start[d], level_start[d+1])
     int4* info = nullptr;      // (first_child | −1, begin, end, depth)
     float4* cm_r = nullptr;    // (float(CM), R)
     float4* cm_lo = nullptr;   // (CM − float(CM), 0): residual of the rounded centre
@@ -34,12 +21,13 @@ struct dwg_octree {
     float4* sb = nullptr;      // (A·n/4π, A)
     int* perm = nullptr;       // tree slot → caller index
     dwg::Double3* an64 = nullptr;  // FP64 prefix sums of A·n (n+1)
+    double4* cell = nullptr;       // cell centre and width (breadth-first node order)
+    double4* cm64 = nullptr;       // (CM, ΣA) in FP64
     ~dwg_octree();
 };
 
 namespace dwg {
 
-dwg_octree* octree_upload(dwg_ctx* ctx, const HostOctree& H);
 void octree_update(dwg_ctx* ctx, dwg_octree* T, const float* nrm, const float* area, const float* radii);
 
 void gwn_direct(dwg_ctx* ctx, const float* pts, const float* nrm, const float* area, const float* radii, int64_t n,

@@ -16,11 +16,7 @@
 // Both are evaluated with numerator and denominator divided by 4π (A·n is
 // pre-scaled by 1/(4π) when sources are packed), which is the same value.
 #include <algorithm>
-#include <atomic>
 #include <cmath>
-#include <functional>
-#include <numeric>
-#include <thread>
 
 #include "gwn.h"
 #include "scan.h"
@@ -183,196 +179,14 @@ void gwn_direct(dwg_ctx* ctx, const float* pts, const float* nrm, const float* a
     check_launch();
 }
 
-// ---------------------------------------------------------------------------
-// octree build (host).  Split rule and child numbering of octreeDWG
-// (octree.cpp:171-335): the root cube is centred on the AABB with the
-// largest extent as width; a node with more than leaf_size points and depth
-// < max_depth gets 8 children, allocated contiguously in pre-order; octant
-// bit 1: x ≥ cx, 2: y ≥ cy, 4: z ≥ cz (exact double comparisons).  Points
-// are stored sorted by their octant path so every node owns a contiguous
-// range [begin, end) of the packed source array.
-// ---------------------------------------------------------------------------
-static void parallel_ranges(int64_t n, const std::function<void(int64_t, int64_t)>& fn) {
-    unsigned nt = std::max(1u, std::thread::hardware_concurrency());
-    if (n < 20000 || nt == 1) {
-        fn(0, n);
-        return;
-    }
-    nt = std::min(nt, 64u);
-    std::vector<std::thread> th;
-    int64_t chunk = (n + nt - 1) / nt;
-    for (unsigned t = 0; t < nt; ++t) {
-        int64_t b = t * chunk, e = std::min(n, b + chunk);
-        if (b < e) th.emplace_back([=, &fn]() { fn(b, e); });
-    }
-    for (auto& x : th) x.join();
-}
-
-HostOctree build_octree_host(const double* P, const double* A, int64_t n, int max_depth, int leaf_size) {
-    DWG_REQUIRE(n > 0, "octree needs at least one point");
-    DWG_REQUIRE(max_depth >= 0 && max_depth <= 21, "max_depth must be in [0, 21]");
-    DWG_REQUIRE(leaf_size >= 1, "leaf_size must be ≥ 1");
-    DWG_REQUIRE(n < (int64_t(1) << 31) - 1, "too many points");
-    HostOctree T;
-    double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
-    for (int64_t i = 0; i < n; ++i)
-        for (int d = 0; d < 3; ++d) {
-            lo[d] = std::min(lo[d], P[3 * i + d]);
-            hi[d] = std::max(hi[d], P[3 * i + d]);
-        }
-    const double c0[3] = {(lo[0] + hi[0]) / 2, (lo[1] + hi[1]) / 2, (lo[2] + hi[2]) / 2};
-    const double w0 = std::max({hi[0] - lo[0], hi[1] - lo[1], hi[2] - lo[2]});
-    // octant path codes, 3 bits per level, most significant = level 1
-    std::vector<uint64_t> code(n);
-    parallel_ranges(n, [&](int64_t b, int64_t e) {
-        for (int64_t i = b; i < e; ++i) {
-            double c[3] = {c0[0], c0[1], c0[2]}, w = w0;
-            uint64_t k = 0;
-            for (int d = 0; d < max_depth; ++d) {
-                int o = (P[3 * i] >= c[0] ? 1 : 0) | (P[3 * i + 1] >= c[1] ? 2 : 0) | (P[3 * i + 2] >= c[2] ? 4 : 0);
-                k = (k << 3) | uint64_t(o);
-                const double q = w / 4;
-                c[0] += (o & 1) ? q : -q;
-                c[1] += (o & 2) ? q : -q;
-                c[2] += (o & 4) ? q : -q;
-                w /= 2;
-            }
-            code[i] = k;
-        }
-    });
-    // stable LSD radix sort of (code, index) by code, 11-bit digits
-    T.perm.resize(n);
-    std::iota(T.perm.begin(), T.perm.end(), 0);
-    std::vector<uint64_t> sc(code);
-    {
-        std::vector<uint64_t> k2(n);
-        std::vector<int> v2(n);
-        const int bits = 3 * max_depth;
-        for (int shift = 0; shift < bits; shift += 11) {
-            std::vector<int64_t> cnt(2049, 0);
-            for (int64_t i = 0; i < n; ++i) ++cnt[((sc[i] >> shift) & 2047) + 1];
-            for (int d = 0; d < 2048; ++d) cnt[d + 1] += cnt[d];
-            for (int64_t i = 0; i < n; ++i) {
-                const int64_t dst = cnt[(sc[i] >> shift) & 2047]++;
-                k2[dst] = sc[i];
-                v2[dst] = T.perm[i];
-            }
-            sc.swap(k2);
-            T.perm.swap(v2);
-        }
-    }
-    // positions and areas in sorted order, contiguous for the moment loops
-    std::vector<double> Ps(4 * n);
-    parallel_ranges(n, [&](int64_t b, int64_t e) {
-        for (int64_t i = b; i < e; ++i) {
-            const int p = T.perm[i];
-            Ps[4 * i] = P[3 * p];
-            Ps[4 * i + 1] = P[3 * p + 1];
-            Ps[4 * i + 2] = P[3 * p + 2];
-            Ps[4 * i + 3] = A ? A[p] : 2e-6;
-        }
-    });
-    // Nodes level by level (breadth-first numbering; the 8 children of a node
-    // stay contiguous, which is all the traversal relies on).  Each level is
-    // one parallel pass: split flags, a prefix sum for the child slots, then
-    // every split node cuts its sorted range at the octant digits.
-    T.first_child.assign(1, -1);
-    T.begin.assign(1, 0);
-    T.end.assign(1, int(n));
-    T.depth.assign(1, 0);
-    T.cn.assign({c0[0], c0[1], c0[2]});
-    T.w.assign(1, w0);
-    int64_t ls = 0, le = 1;
-    for (int d = 0; d < max_depth && ls < le; ++d) {
-        std::vector<int64_t> slot(le - ls + 1, 0);
-        for (int64_t i = ls; i < le; ++i) slot[i - ls + 1] = slot[i - ls] + ((T.end[i] - T.begin[i] > leaf_size) ? 1 : 0);
-        const int64_t nsplit = slot[le - ls];
-        if (nsplit == 0) break;
-        const int64_t total = le + 8 * nsplit;
-        T.first_child.resize(total, -1);
-        T.begin.resize(total);
-        T.end.resize(total);
-        T.depth.resize(total);
-        T.cn.resize(3 * total);
-        T.w.resize(total);
-        const int shift = 3 * (max_depth - 1 - d);
-        parallel_ranges(le - ls, [&](int64_t b0, int64_t e0) {
-            for (int64_t j = b0; j < e0; ++j) {
-                const int64_t idx = ls + j;
-                if (slot[j + 1] == slot[j]) continue;
-                const int64_t fc = le + 8 * slot[j];
-                T.first_child[idx] = int(fc);
-                const int b = T.begin[idx], e = T.end[idx];
-                const double q = T.w[idx] / 4;
-                int cur = b;
-                for (int k = 0; k < 8; ++k) {
-                    int stop = cur;
-                    while (stop < e && int((sc[stop] >> shift) & 7) == k) ++stop;
-                    const int64_t c = fc + k;
-                    T.begin[c] = cur;
-                    T.end[c] = stop;
-                    T.depth[c] = d + 1;
-                    T.cn[3 * c] = T.cn[3 * idx] + ((k & 1) ? q : -q);
-                    T.cn[3 * c + 1] = T.cn[3 * idx + 1] + ((k & 2) ? q : -q);
-                    T.cn[3 * c + 2] = T.cn[3 * idx + 2] + ((k & 4) ? q : -q);
-                    T.w[c] = T.w[idx] / 2;
-                    cur = stop;
-                }
-            }
-        });
-        ls = le;
-        le = total;
-    }
-    const int64_t m = int64_t(T.first_child.size());
-    T.cm.assign(3 * m, 0.0);
-    T.r.assign(m, 0.0);
-    // CM = ΣA p / ΣA, R = max |p − CM| (fast_winding_number.cpp:59-110), FP64.
-    // Node costs are uneven (the root spans every point), so threads take
-    // blocks of nodes from a shared counter.
-    std::atomic<int64_t> next{0};
-    const int64_t blk = 2048;
-    parallel_ranges(int64_t(std::max(1u, std::thread::hardware_concurrency())) * 20000, [&](int64_t, int64_t) {
-        for (;;) {
-            const int64_t b = next.fetch_add(blk);
-            if (b >= m) return;
-            const int64_t e = std::min(m, b + blk);
-            for (int64_t i = b; i < e; ++i) {
-                const int pb = T.begin[i], pe = T.end[i];
-                if (pb == pe) continue;
-                double at = 0, c[3] = {0, 0, 0};
-                for (int s = pb; s < pe; ++s) {
-                    const double a = Ps[4 * s + 3];
-                    at += a;
-                    for (int d = 0; d < 3; ++d) c[d] += a * Ps[4 * s + d];
-                }
-                for (int d = 0; d < 3; ++d) c[d] /= at;
-                double rm = 0;
-                for (int s = pb; s < pe; ++s) {
-                    const double dx = Ps[4 * s] - c[0], dy = Ps[4 * s + 1] - c[1], dz = Ps[4 * s + 2] - c[2];
-                    rm = std::max(rm, std::sqrt(dx * dx + dy * dy + dz * dz));
-                }
-                for (int d = 0; d < 3; ++d) T.cm[3 * i + d] = c[d];
-                T.r[i] = rm;
-            }
-        }
-    });
-    T.n_points = n;
-    T.max_level = 0;
-    T.n_leaves = 0;
-    for (int64_t i = 0; i < m; ++i) {
-        T.max_level = std::max(T.max_level, T.depth[i]);
-        if (T.first_child[i] < 0 && T.end[i] > T.begin[i]) ++T.n_leaves;
-    }
-    return T;
-}
-
 }  // namespace dwg
 
 // ---------------------------------------------------------------------------
 // device octree
 // ---------------------------------------------------------------------------
 dwg_octree::~dwg_octree() {
-    for (void* p : {(void*)info, (void*)cm_r, (void*)cm_lo, (void*)ec, (void*)ec_bf16, (void*)sa, (void*)sb, (void*)perm, (void*)an64})
+    for (void* p : {(void*)info, (void*)cm_r, (void*)cm_lo, (void*)ec, (void*)ec_bf16, (void*)sa, (void*)sb, (void*)perm,
+                    (void*)an64, (void*)cell, (void*)cm64})
         if (p) (void)hipFree(p);
 }
 
@@ -395,54 +209,6 @@ __global__ void node_ec_kernel(const int4* __restrict__ info, const Double3* __r
         };
         ec_bf16[i] = make_uint2(bf(x) | (bf(y) << 16), bf(z));
     }
-}
-
-dwg_octree* octree_upload(dwg_ctx* ctx, const HostOctree& H) {
-    auto* T = new dwg_octree();
-    try {
-        const int64_t m = int64_t(H.first_child.size()), n = H.n_points;
-        T->n_nodes = m;
-        T->n_points = n;
-        T->n_leaves = H.n_leaves;
-        T->max_level = H.max_level;
-        std::vector<int4> info(m);
-        std::vector<float4> cmr(m), cml(m);
-        for (int64_t i = 0; i < m; ++i) {
-            info[i] = make_int4(H.first_child[i], H.begin[i], H.end[i], H.depth[i]);
-            const float cx = float(H.cm[3 * i]), cy = float(H.cm[3 * i + 1]), cz = float(H.cm[3 * i + 2]);
-            cmr[i] = make_float4(cx, cy, cz, float(H.r[i]));
-            // the dipole of a node next to the query is (CM − q)·EC with CM − q
-            // nearly tangent to EC; keeping the rounding residual of CM makes
-            // CM − q accurate to one rounding of the difference itself
-            cml[i] = make_float4(float(H.cm[3 * i] - double(cx)), float(H.cm[3 * i + 1] - double(cy)),
-                                 float(H.cm[3 * i + 2] - double(cz)), 0.f);
-        }
-        DWG_HIP_CHECK(hipMalloc(&T->info, sizeof(int4) * m));
-        DWG_HIP_CHECK(hipMalloc(&T->cm_r, sizeof(float4) * m));
-        DWG_HIP_CHECK(hipMalloc(&T->cm_lo, sizeof(float4) * m));
-        DWG_HIP_CHECK(hipMalloc(&T->ec, sizeof(float4) * m));
-        DWG_HIP_CHECK(hipMalloc(&T->ec_bf16, sizeof(uint2) * m));
-        DWG_HIP_CHECK(hipMalloc(&T->sa, sizeof(float4) * n));
-        DWG_HIP_CHECK(hipMalloc(&T->sb, sizeof(float4) * n));
-        DWG_HIP_CHECK(hipMalloc(&T->perm, sizeof(int) * n));
-        DWG_HIP_CHECK(hipMalloc(&T->an64, sizeof(Double3) * (n + 1)));
-        DWG_HIP_CHECK(hipMemcpyAsync(T->info, info.data(), sizeof(int4) * m, hipMemcpyHostToDevice, ctx->stream));
-        DWG_HIP_CHECK(hipMemcpyAsync(T->cm_r, cmr.data(), sizeof(float4) * m, hipMemcpyHostToDevice, ctx->stream));
-        DWG_HIP_CHECK(hipMemcpyAsync(T->cm_lo, cml.data(), sizeof(float4) * m, hipMemcpyHostToDevice, ctx->stream));
-        DWG_HIP_CHECK(hipMemcpyAsync(T->perm, H.perm.data(), sizeof(int) * n, hipMemcpyHostToDevice, ctx->stream));
-        // positions in tree order (x, y, z, s = 0.0015 until the first update)
-        std::vector<float4> sa(n);
-        for (int64_t i = 0; i < n; ++i) {
-            const int p = H.perm[i];
-            sa[i] = make_float4(float(H.P[3 * p]), float(H.P[3 * p + 1]), float(H.P[3 * p + 2]), kScreenDefault);
-        }
-        DWG_HIP_CHECK(hipMemcpyAsync(T->sa, sa.data(), sizeof(float4) * n, hipMemcpyHostToDevice, ctx->stream));
-        DWG_HIP_CHECK(hipStreamSynchronize(ctx->stream));
-    } catch (...) {
-        delete T;
-        throw;
-    }
-    return T;
 }
 
 __global__ void set_radius_kernel(float4* __restrict__ sa, const float* __restrict__ radii, const int* __restrict__ perm,

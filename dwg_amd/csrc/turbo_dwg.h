@@ -1,6 +1,7 @@
 // Host-side DWG driver: the reference's TurboDWG class and test3D entry
 // points (include/TurboDWG.h:29-75, include/DWGProcess.h:32-808) over the
-// libdwg_hip C ABI.  Eigen-free: matrices are row-major std::vector<double>.
+// libdwg_hip C ABI.  Eigen-free; point sets stay on the GPU from upload to
+// the final mesh (row-major n×3 device arrays).
 #pragma once
 
 #include <chrono>
@@ -9,10 +10,6 @@
 #include <vector>
 
 #include "../../include/dwg_hip.h"
-
-namespace dwg {
-struct HostOctree;
-}
 
 namespace dwg_host {
 
@@ -45,8 +42,10 @@ struct Grid {
     int64_t size() const { return int64_t(res[0]) * res[1] * res[2]; }
 };
 
-// igl::voxel_grid(V, offset, s, pad = 1) in closed form (voxel_grid.cpp:15-87).
+// igl::voxel_grid(V, offset, s, pad = 1) in closed form (voxel_grid.cpp:15-87):
+// from the points, or from their bounding box.
 Grid voxel_grid(const double* V, int64_t n, double offset, int s);
+Grid voxel_grid_bounds(const double lo[3], const double hi[3], double offset, int s);
 
 // Binary STL (TurboMC.cu:343-371) and the .xyz normals writer (TurboDWG.cu:912-923).
 void save_binary_stl(const std::string& path, const float* tris, int64_t ntri);
@@ -61,11 +60,13 @@ public:
     TurboDWG(dwg_ctx* ctx, const dwg_options& opt);
     ~TurboDWG();
 
-    // TurboDWG::TurboDWG (TurboDWG.cu:14-83): sources, normals, areas, radii, grid.
-    // `prebuilt` (nullable): an octree already built over exactly these points
-    // with constant areas and leaf size 1.
-    void SetSources(const std::vector<double>& pts, const std::vector<float>& nrm, const std::vector<float>& areas,
-                    const Grid& grid, const dwg::HostOctree* prebuilt = nullptr);
+    // TurboDWG::TurboDWG (TurboDWG.cu:14-83): sources (device FP64, n×3),
+    // normals (device, n×3; nullptr → zeros), areas 2e-6 (DWGProcess.h:619),
+    // radii 0.0015, grid.  `prebuilt` (nullable, ownership taken): an octree
+    // already built over exactly these points with constant areas and leaf
+    // size 1.
+    void SetSources(const double* d_pts, int64_t n, const float* d_nrm, const Grid& grid,
+                    dwg_octree* prebuilt = nullptr);
     // Radii of the sources from their k nearest neighbours (GetRadiiandAs /
     // TurboDWG::UpdateRadii).
     void UpdateRadii(int k);
@@ -76,9 +77,10 @@ public:
     double TurboScreenedDWGLaunch(int total_iters, double* w_mean, int64_t* iter_tris, int64_t* iter_zeros);
     // GenerateMesh (TurboDWG.cu:808-924): transfer the source normals to the
     // points, one screened evaluation at the points and on the final grid, MC.
-    void GenerateMesh(const std::vector<double>& points, const Grid& final_grid, int64_t* tri_count,
-                      std::vector<float>* tris, std::vector<float>* point_normals,
-                      const dwg::HostOctree* points_tree = nullptr);
+    // d_points: device FP64 (np×3); points_tree (nullable, ownership taken):
+    // their octree, reused as in TurboDWG.cu:869-870.
+    void GenerateMesh(const double* d_points, int64_t np, const Grid& final_grid, int64_t* tri_count,
+                      std::vector<float>* tris, std::vector<float>* point_normals, dwg_octree* points_tree = nullptr);
     // GenerateMesh(grid) variant of --use_leaves_generate 1 (TurboDWG.cu:768-806).
     void GenerateMeshFromSources(const Grid& final_grid, int64_t* tri_count, std::vector<float>* tris);
 
@@ -106,21 +108,23 @@ private:
 // test3D entry points (DWGProcess.h) over in-memory points.
 namespace test3D {
 
-// ReadPointandNormals normalisation (DWGProcess.h:53-97): centre at the mean,
-// divide by max_i 2|p_i − c|_∞.
-void NormalizePoints(const float* pts, int64_t n, std::vector<double>& out, double center[3], double* scale);
+// ReadPointandNormals normalisation (DWGProcess.h:53-97) on the GPU: centre
+// at the mean, divide by max_i 2|p_i − c|_∞.  d_pts: device floats (n×3);
+// outputs device FP64 and float copies.
+void NormalizePoints(dwg_ctx* ctx, const float* d_pts, int64_t n, double* d_out64, float* d_out32, double center[3],
+                     double* scale);
 
-// GetOctreeLeaves (DWGProcess.h:383-428): breadth-first leaf cells with more
-// than min_num points; returns cell centres, cell ids.
-void GetOctreeLeaves(const std::vector<double>& P, int max_depth, int min_num, std::vector<double>& leaves,
-                     std::vector<int>& leaf_node, std::vector<double>& leaf_width,
-                     std::vector<std::vector<int>>& leaf_points, dwg::HostOctree* tree_out = nullptr);
+// GetOctreeLeaves (DWGProcess.h:383-428) on the GPU: the octree over the
+// points (returned in *tree, caller owns) and its breadth-first leaf cells
+// with more than min_num points: centres (FP64), widths, node ids.
+int64_t GetOctreeLeaves(dwg_ctx* ctx, const double* d_P, int64_t n, int max_depth, int min_num, dwg_octree** tree,
+                        DevArray<double>& leaves, DevArray<double>& widths, DevArray<int32_t>& nodes);
 
 // SmoothNormalstoOctreeLeaves (DWGProcess.h:430-472) + the whole-matrix
-// Frobenius normalisation of DWGProcess.h:537.
-void SmoothNormalstoOctreeLeaves(const std::vector<double>& leaves, const std::vector<double>& leaf_width,
-                                 const std::vector<std::vector<int>>& leaf_points, const std::vector<double>& P,
-                                 const std::vector<double>& N, std::vector<float>& leaf_normals);
+// Frobenius normalisation of DWGProcess.h:537, on the GPU.
+void SmoothNormalstoOctreeLeaves(dwg_ctx* ctx, const dwg_octree* tree, const DevArray<int32_t>& nodes,
+                                 const DevArray<double>& leaves, const DevArray<double>& widths, int64_t nl,
+                                 const double* d_P, const double* d_N, float* d_leaf_normals);
 
 // GetRandomNs (DWGProcess.h:264-281), seeded.
 void GetRandomNs(int64_t n, uint64_t seed, std::vector<double>& N);

@@ -8,14 +8,17 @@
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
-#include <deque>
+#include <cstring>
 #include <fstream>
+#include <memory>
 #include <random>
 #include <sstream>
 #include <stdexcept>
 
 #include "common.h"
 #include "gwn.h"
+#include "knn.h"
+#include "octree.h"
 
 namespace dwg_host {
 
@@ -56,6 +59,7 @@ void DevArray<T>::download(T* h, int64_t n, void* stream) const {
 }
 
 template struct DevArray<float>;
+template struct DevArray<double>;
 template struct DevArray<int32_t>;
 template struct DevArray<int64_t>;
 template struct DevArray<uint16_t>;
@@ -72,13 +76,18 @@ dwg_grid Grid::abi() const {
 
 Grid voxel_grid(const double* V, int64_t n, double offset, int s) {
     if (n <= 0) throw dwg::Error(DWG_ERR_ARG, "voxel grid of no points");
-    if (s < 4) throw dwg::Error(DWG_ERR_ARG, "grid resolution must be ≥ 4");
     double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
     for (int64_t i = 0; i < n; ++i)
         for (int d = 0; d < 3; ++d) {
             lo[d] = std::min(lo[d], V[3 * i + d]);
             hi[d] = std::max(hi[d], V[3 * i + d]);
         }
+    return voxel_grid_bounds(lo, hi, offset, s);
+}
+
+Grid voxel_grid_bounds(const double lo_in[3], const double hi_in[3], double offset, int s) {
+    if (s < 4) throw dwg::Error(DWG_ERR_ARG, "grid resolution must be ≥ 4");
+    double lo[3] = {lo_in[0], lo_in[1], lo_in[2]}, hi[3] = {hi_in[0], hi_in[1], hi_in[2]};
     double ext[3], ctr[3];
     for (int d = 0; d < 3; ++d) {
         lo[d] -= offset;
@@ -171,28 +180,39 @@ void TurboDWG::check(int rc, const char* what) const {
     if (rc != DWG_OK) throw dwg::Error(rc, std::string(what) + ": " + dwg_ctx_error(ctx_));
 }
 
-void TurboDWG::SetSources(const std::vector<double>& pts, const std::vector<float>& nrm,
-                          const std::vector<float>& areas, const Grid& grid, const dwg::HostOctree* prebuilt) {
-    n_ = int64_t(pts.size() / 3);
-    std::vector<float> pf(pts.begin(), pts.end());
-    void* s = ctx_->stream;
-    pts_.upload(pf.data(), 3 * n_, s);
-    nrm_.upload(nrm.data(), 3 * n_, s);
-    area_.upload(areas.data(), n_, s);
-    std::vector<float> r(n_, 0.0015f);
-    radii_.upload(r.data(), n_, s);
+void TurboDWG::SetSources(const double* d_pts, int64_t n, const float* d_nrm, const Grid& grid,
+                          dwg_octree* prebuilt) {
+    std::unique_ptr<dwg_octree, void (*)(dwg_octree*)> owned(prebuilt, dwg_octree_destroy);
+    n_ = n;
+    hipStream_t s = ctx_->stream;
+    pts_.resize(3 * n_);
+    dwg::to_f32(ctx_, d_pts, 3 * n_, pts_.ptr);
+    nrm_.resize(3 * n_);
+    if (d_nrm) {
+        if (hipMemcpyAsync(nrm_.ptr, d_nrm, sizeof(float) * 3 * n_, hipMemcpyDeviceToDevice, s) != hipSuccess)
+            throw dwg::Error(DWG_ERR_HIP, "normal copy failed");
+    } else {
+        dwg::fill_u32(ctx_, nrm_.ptr, 0u, 3 * n_);
+    }
+    area_.resize(n_);
+    const float a0 = 2e-6f, r0 = 0.0015f;
+    uint32_t bits;
+    std::memcpy(&bits, &a0, 4);
+    dwg::fill_u32(ctx_, area_.ptr, bits, n_);
+    radii_.resize(n_);
+    std::memcpy(&bits, &r0, 4);
+    dwg::fill_u32(ctx_, radii_.ptr, bits, n_);
     w_src_.resize(n_);
     grid_ = grid;
     w_grid_.resize(grid_.size());
-    std::vector<double> ad(areas.begin(), areas.end());
     if (tree_) dwg_octree_destroy(tree_);
     tree_ = nullptr;
     if (knn_) dwg_knn_destroy(knn_);
     knn_ = nullptr;
-    if (prebuilt && opt_.leaf_size == 1 && prebuilt->n_points == n_) {
-        tree_ = dwg::octree_upload(ctx_, *prebuilt);
+    if (owned && opt_.leaf_size == 1 && owned->n_points == n_) {
+        tree_ = owned.release();
     } else {
-        check(dwg_octree_build(ctx_, pts.data(), ad.data(), n_, opt_.max_depth, opt_.leaf_size, &tree_), "octree");
+        check(dwg_octree_build_device(ctx_, d_pts, nullptr, n_, opt_.max_depth, opt_.leaf_size, &tree_), "octree");
     }
     check(dwg_knn_build(ctx_, pts_.ptr, n_, 15, &knn_), "knn index");
 }
@@ -341,24 +361,22 @@ void TurboDWG::mesh_pass(const Grid& grid, float iso_scale, int64_t* tri_count, 
     *tri_count = ntri;
 }
 
-void TurboDWG::GenerateMesh(const std::vector<double>& points, const Grid& final_grid, int64_t* tri_count,
-                            std::vector<float>* tris, std::vector<float>* point_normals,
-                            const dwg::HostOctree* points_tree) {
+void TurboDWG::GenerateMesh(const double* d_points, int64_t np, const Grid& final_grid, int64_t* tri_count,
+                            std::vector<float>* tris, std::vector<float>* point_normals, dwg_octree* points_tree) {
+    std::unique_ptr<dwg_octree, void (*)(dwg_octree*)> owned(points_tree, dwg_octree_destroy);
     // normals of the points from their normal_kw nearest sources (LoadNormals)
-    const int64_t np = int64_t(points.size() / 3);
-    std::vector<float> pf(points.begin(), points.end());
-    DevArray<float> dp, dn(3 * np);
-    dp.upload(pf.data(), 3 * np, ctx_->stream);
+    DevArray<float> dp(3 * np), dn(3 * np);
+    dwg::to_f32(ctx_, d_points, 3 * np, dp.ptr);
     check(dwg_transfer_normals(ctx_, knn_, nrm_.ptr, dp.ptr, np, opt_.normal_kw, dn.ptr), "transfer normals");
-    // the points become the sources (TurboDWG.cu:865-876)
-    std::vector<float> pn(3 * np);
-    dn.download(pn.data(), 3 * np, ctx_->stream);
-    std::vector<float> areas(np, 2e-6f);
-    // the final pass reuses the octree over the input points (TurboDWG.cu:869-870)
-    SetSources(points, pn, areas, grid_, points_tree);
+    // the points become the sources (TurboDWG.cu:865-876), reusing the octree
+    // over the input points (TurboDWG.cu:869-870)
+    SetSources(d_points, np, dn.ptr, grid_, owned.release());
     UpdateRadii(opt_.final_kw);
     mesh_pass(final_grid, 1.0f, tri_count, tris);
-    if (point_normals) *point_normals = pn;
+    if (point_normals) {
+        point_normals->resize(3 * np);
+        dn.download(point_normals->data(), 3 * np, ctx_->stream);
+    }
 }
 
 void TurboDWG::GenerateMeshFromSources(const Grid& final_grid, int64_t* tri_count, std::vector<float>* tris) {
@@ -369,75 +387,31 @@ void TurboDWG::GenerateMeshFromSources(const Grid& final_grid, int64_t* tri_coun
 // ---------------------------------------------------------------------------
 namespace test3D {
 
-void NormalizePoints(const float* pts, int64_t n, std::vector<double>& out, double center[3], double* scale) {
-    out.resize(3 * n);
-    double c[3] = {0, 0, 0};
-    for (int64_t i = 0; i < n; ++i)
-        for (int d = 0; d < 3; ++d) c[d] += double(pts[3 * i + d]);
-    for (int d = 0; d < 3; ++d) c[d] /= double(n);
-    double s = 0;
-    for (int64_t i = 0; i < n; ++i)
-        for (int d = 0; d < 3; ++d) s = std::max(s, 2.0 * std::fabs(double(pts[3 * i + d]) - c[d]));
-    if (s <= 0) s = 1;
-    for (int64_t i = 0; i < n; ++i)
-        for (int d = 0; d < 3; ++d) out[3 * i + d] = (double(pts[3 * i + d]) - c[d]) / s;
-    for (int d = 0; d < 3; ++d) center[d] = c[d];
-    *scale = s;
+void NormalizePoints(dwg_ctx* ctx, const float* d_pts, int64_t n, double* d_out64, float* d_out32, double center[3],
+                     double* scale) {
+    dwg::normalize_points(ctx, d_pts, n, d_out64, d_out32, center, scale);
 }
 
-void GetOctreeLeaves(const std::vector<double>& P, int max_depth, int min_num, std::vector<double>& leaves,
-                     std::vector<int>& leaf_node, std::vector<double>& leaf_width,
-                     std::vector<std::vector<int>>& leaf_points, dwg::HostOctree* tree_out) {
-    const int64_t n = int64_t(P.size() / 3);
-    dwg::HostOctree T = dwg::build_octree_host(P.data(), nullptr, n, max_depth, 1);
-    T.P = P.data();
-    leaves.clear();
-    leaf_node.clear();
-    leaf_width.clear();
-    leaf_points.clear();
-    std::deque<int> q{0};
-    while (!q.empty()) {
-        const int idx = q.front();
-        q.pop_front();
-        if (T.first_child[idx] < 0) {
-            if (T.end[idx] - T.begin[idx] > min_num) {
-                leaves.insert(leaves.end(), {T.cn[3 * idx], T.cn[3 * idx + 1], T.cn[3 * idx + 2]});
-                leaf_node.push_back(idx);
-                leaf_width.push_back(T.w[idx]);
-                std::vector<int> pts(T.perm.begin() + T.begin[idx], T.perm.begin() + T.end[idx]);
-                std::sort(pts.begin(), pts.end());
-                leaf_points.push_back(std::move(pts));
-            }
-            continue;
-        }
-        for (int k = 0; k < 8; ++k) {
-            const int c = T.first_child[idx] + k;
-            if (T.end[c] > T.begin[c]) q.push_back(c);
-        }
+int64_t GetOctreeLeaves(dwg_ctx* ctx, const double* d_P, int64_t n, int max_depth, int min_num, dwg_octree** tree,
+                        DevArray<double>& leaves, DevArray<double>& widths, DevArray<int32_t>& nodes) {
+    dwg_octree* T = dwg::octree_build(ctx, d_P, nullptr, n, max_depth, 1);
+    try {
+        leaves.resize(3 * T->n_leaves);
+        widths.resize(T->n_leaves);
+        nodes.resize(T->n_leaves);
+        const int64_t nl = dwg::octree_leaves(ctx, T, min_num, leaves.ptr, widths.ptr, nodes.ptr);
+        *tree = T;
+        return nl;
+    } catch (...) {
+        dwg_octree_destroy(T);
+        throw;
     }
-    if (tree_out) *tree_out = std::move(T);
 }
 
-void SmoothNormalstoOctreeLeaves(const std::vector<double>& leaves, const std::vector<double>& leaf_width,
-                                 const std::vector<std::vector<int>>& leaf_points, const std::vector<double>& P,
-                                 const std::vector<double>& N, std::vector<float>& leaf_normals) {
-    const size_t m = leaf_width.size();
-    std::vector<double> out(3 * m, 0.0);
-    // per component: Σ_j exp(−((P_jk − C_k)/W)²/4)·N_jk
-    for (size_t i = 0; i < m; ++i) {
-        const double W = leaf_width[i];
-        for (int p : leaf_points[i])
-            for (int k = 0; k < 3; ++k) {
-                const double x = (P[3 * p + k] - leaves[3 * i + k]) / W;
-                out[3 * i + k] += std::exp(-x * x / 4.0) * N[3 * p + k];
-            }
-    }
-    // whole-matrix Frobenius normalisation (DWGProcess.h:537)
-    double f = 0;
-    for (double v : out) f += v * v;
-    f = std::sqrt(f);
-    leaf_normals.resize(3 * m);
-    for (size_t i = 0; i < 3 * m; ++i) leaf_normals[i] = float(f > 0 ? out[i] / f : 0.0);
+void SmoothNormalstoOctreeLeaves(dwg_ctx* ctx, const dwg_octree* tree, const DevArray<int32_t>& nodes,
+                                 const DevArray<double>& leaves, const DevArray<double>& widths, int64_t nl,
+                                 const double* d_P, const double* d_N, float* d_leaf_normals) {
+    dwg::leaf_smooth_normals(ctx, tree, nodes.ptr, leaves.ptr, widths.ptr, nl, d_P, d_N, d_leaf_normals);
 }
 
 void GetRandomNs(int64_t n, uint64_t seed, std::vector<double>& N) {
@@ -459,46 +433,64 @@ void DWGLaunch(dwg_ctx* ctx, const float* pts, const float* nrm, int64_t n, cons
     if (opt.input_normal == 1 && !nrm) throw dwg::Error(DWG_ERR_ARG, "--in_normal 1 needs input normals");
     if (opt.input_normal == 4 && !opt.mlp) throw dwg::Error(DWG_ERR_ARG, "--in_normal 4 needs MLP weights");
     const auto t0 = Clock::now();
-    std::vector<double> P;
+    hipStream_t stream = ctx->stream;
+    // the input goes to the GPU once; everything up to the final mesh stays there
+    DevArray<float> P32(3 * n);
+    DevArray<double> P(3 * n);
     double center[3] = {0, 0, 0}, scale = 1;
-    if (opt.normalize_input) {
-        NormalizePoints(pts, n, P, center, &scale);
-    } else {
-        P.assign(pts, pts + 3 * n);
+    {
+        DevArray<float> raw;
+        raw.upload(pts, 3 * n, stream);
+        if (opt.normalize_input) {
+            NormalizePoints(ctx, raw.ptr, n, P.ptr, P32.ptr, center, &scale);
+        } else {
+            std::swap(P32, raw);
+            dwg::to_f64(ctx, P32.ptr, 3 * n, P.ptr);
+        }
     }
-    // initial point normals (DWGProcess.h:488-510)
-    std::vector<double> Ns;
-    if (opt.input_normal == 1) {
-        Ns.assign(nrm, nrm + 3 * n);
-    } else {
-        GetRandomNs(n, opt.seed, Ns);
+    // initial point normals (DWGProcess.h:488-510); the Gauss-map and learned
+    // initialisations overwrite the source normals, so they need none
+    DevArray<double> Ns;
+    if (opt.input_normal == 0 || opt.input_normal == 1) {
+        std::vector<double> h;
+        if (opt.input_normal == 1) h.assign(nrm, nrm + 3 * n);
+        else GetRandomNs(n, opt.seed, h);
+        Ns.upload(h.data(), 3 * n, stream);
     }
     // sources
-    std::vector<double> S;
-    std::vector<float> Sn;
-    dwg::HostOctree points_tree;  // octree over the input points, reused by the final pass
+    std::unique_ptr<dwg_octree, void (*)(dwg_octree*)> points_tree(nullptr, dwg_octree_destroy);
+    DevArray<double> leaves, leaf_w;
+    DevArray<int32_t> leaf_node;
+    DevArray<float> Sn;
+    const double* S = P.ptr;
+    int64_t ns = n;
     if (opt.source_mode == 0) {
-        std::vector<int> leaf_node;
-        std::vector<double> leaf_w;
-        std::vector<std::vector<int>> leaf_pts;
-        GetOctreeLeaves(P, opt.max_depth, opt.min_num, S, leaf_node, leaf_w, leaf_pts, &points_tree);
-        if (opt.input_normal == 3 || opt.input_normal == 4) {
-            Sn.assign(S.size(), 0.0f);  // overwritten by the initialisation below
-        } else {
-            SmoothNormalstoOctreeLeaves(S, leaf_w, leaf_pts, P, Ns, Sn);
+        dwg_octree* T = nullptr;
+        ns = GetOctreeLeaves(ctx, P.ptr, n, opt.max_depth, opt.min_num, &T, leaves, leaf_w, leaf_node);
+        points_tree.reset(T);
+        S = leaves.ptr;
+        if (Ns.ptr) {
+            Sn.resize(3 * ns);
+            SmoothNormalstoOctreeLeaves(ctx, T, leaf_node, leaves, leaf_w, ns, P.ptr, Ns.ptr, Sn.ptr);
         }
-    } else {
-        S = P;
-        Sn.assign(Ns.begin(), Ns.end());
+    } else if (Ns.ptr) {
+        Sn.resize(3 * n);
+        dwg::to_f32(ctx, Ns.ptr, 3 * n, Sn.ptr);
     }
-    const int64_t ns = int64_t(S.size() / 3);
-    std::vector<float> areas(ns, 2e-6f);  // DWGProcess.h:619
+    if (ns <= 0) throw dwg::Error(DWG_ERR_ARG, "no octree leaf holds more than min_num points");
     // the iteration grid is built over the float input points (DWGProcess.h:662-672)
-    std::vector<double> Pf(P.size());
-    for (size_t i = 0; i < P.size(); ++i) Pf[i] = double(float(P[i]));
-    const Grid grid = voxel_grid(Pf.data(), n, opt.offset_1, opt.grid_reso);
+    double plo[3], phi[3];
+    {
+        float lo[3], hi[3];
+        dwg::bounds(ctx, P32.ptr, n, lo, hi);
+        for (int d = 0; d < 3; ++d) {
+            plo[d] = lo[d];
+            phi[d] = hi[d];
+        }
+    }
+    const Grid grid = voxel_grid_bounds(plo, phi, opt.offset_1, opt.grid_reso);
     TurboDWG dwg(ctx, opt);
-    dwg.SetSources(S, Sn, areas, grid);
+    dwg.SetSources(S, ns, Sn.ptr, grid);
     dwg.UpdateRadii(opt.kw);
     if (opt.input_normal == 3) dwg.GaussMapInit();
     if (opt.input_normal == 4) dwg.MlpInit(opt.mlp);
@@ -522,13 +514,15 @@ void DWGLaunch(dwg_ctx* ctx, const float* pts, const float* nrm, int64_t n, cons
     std::vector<float> tris, point_nrm;
     const bool want_tris = (res && res->want_mesh) || (out_base && (opt.output_mode == 1 || opt.output_mode == 3));
     if (opt.use_leaves_generate == 1) {
-        const Grid fg = voxel_grid(S.data(), ns, 0.01, opt.final_reso);
+        double slo[3], shi[3];
+        dwg::bounds_f64(ctx, S, ns, slo, shi);
+        const Grid fg = voxel_grid_bounds(slo, shi, 0.01, opt.final_reso);
         dwg.GenerateMeshFromSources(fg, &ntri, want_tris ? &tris : nullptr);
         dwg.download_normals(point_nrm);
     } else {
-        const Grid fg = voxel_grid(Pf.data(), n, opt.offset_2, opt.final_reso);
+        const Grid fg = voxel_grid_bounds(plo, phi, opt.offset_2, opt.final_reso);
         if (opt.source_mode == 0) {
-            dwg.GenerateMesh(P, fg, &ntri, want_tris ? &tris : nullptr, &point_nrm, &points_tree);
+            dwg.GenerateMesh(P.ptr, n, fg, &ntri, want_tris ? &tris : nullptr, &point_nrm, points_tree.release());
         } else {
             dwg.GenerateMeshFromSources(fg, &ntri, want_tris ? &tris : nullptr);
             dwg.download_normals(point_nrm);
@@ -539,8 +533,10 @@ void DWGLaunch(dwg_ctx* ctx, const float* pts, const float* nrm, int64_t n, cons
         if (opt.output_mode == 1 || opt.output_mode == 3) save_binary_stl(std::string(out_base) + ".stl", tris.data(), ntri);
         if (opt.output_mode == 2 || opt.output_mode == 3) {
             const int64_t np = int64_t(point_nrm.size() / 3);
-            const std::vector<double>& src = (opt.use_leaves_generate == 1) ? S : P;
-            std::vector<float> spf(src.begin(), src.end());
+            DevArray<float> f(3 * np);
+            dwg::to_f32(ctx, (opt.use_leaves_generate == 1) ? S : P.ptr, 3 * np, f.ptr);
+            std::vector<float> spf(3 * np);
+            f.download(spf.data(), 3 * np, stream);
             save_xyz(std::string(out_base) + ".xyz", spf.data(), point_nrm.data(), np);
         }
     }

@@ -140,6 +140,17 @@ class Octree:
         self.ctx.L.dwg_octree_info(self.h, ctypes.byref(m), ctypes.byref(l), ctypes.byref(d))
         return {"nodes": m.value, "leaves": l.value, "depth": d.value}
 
+    def leaves(self, min_num=0):
+        """Centres and widths of the leaf cells holding more than min_num
+        points, breadth-first (GetOctreeLeaves)."""
+        c = self.ctx
+        nl = self.info()["leaves"]
+        cen = torch.empty((max(nl, 1), 3), device=c.device, dtype=torch.float64)
+        wid = torch.empty(max(nl, 1), device=c.device, dtype=torch.float64)
+        cnt = ctypes.c_int64()
+        c.check(c.L.dwg_octree_leaves(c.h, self.h, int(min_num), _ptr(cen), _ptr(wid), ctypes.byref(cnt)))
+        return cen[:cnt.value].cpu().numpy(), wid[:cnt.value].cpu().numpy()
+
     def update(self, nrm, area, radii=None):
         c = self.ctx
         self._nrm, self._area = c.f32(nrm), c.f32(area)

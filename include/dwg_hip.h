@@ -76,12 +76,21 @@ int dwg_gwn_direct(dwg_ctx* ctx, const float* d_pts, const float* d_nrm, const f
  * leaf_size points and depth < max_depth is split into 8 children,
  * octree.cpp:171-335; the reference uses leaf_size 1) plus the order-0
  * moments CM, R (fast_winding_number.cpp:54-119) and the GPU upload
- * (TurboLibiGWN::PreLibiGWN, TurboLibiGWN.cu:132-231).  h_pts: host
- * doubles (n×3); h_area: host doubles (n) or NULL for the constant 2e-6
- * (DWGProcess.h:619).  Point order of every other call stays the caller's. */
+ * (TurboLibiGWN::PreLibiGWN, TurboLibiGWN.cu:132-231); the whole build runs
+ * on the GPU.  h_pts: host doubles (n×3); h_area: host doubles (n) or NULL
+ * for the constant 2e-6 (DWGProcess.h:619).  Point order of every other
+ * call stays the caller's. */
 int dwg_octree_build(dwg_ctx* ctx, const double* h_pts, const double* h_area, int64_t n,
                      int max_depth, int leaf_size, dwg_octree** out);
+/* The same from device arrays (d_pts n×3 FP64, d_area n FP64 or NULL). */
+int dwg_octree_build_device(dwg_ctx* ctx, const double* d_pts, const double* d_area, int64_t n,
+                            int max_depth, int leaf_size, dwg_octree** out);
 void dwg_octree_destroy(dwg_octree* tree);
+/* GetOctreeLeaves (DWGProcess.h:383-428): centres (device FP64, room for
+ * n_leaves×3) and widths (device FP64 or NULL) of the leaf cells holding more
+ * than min_num ≥ 0 points, in breadth-first order; the count in *out_count. */
+int dwg_octree_leaves(dwg_ctx* ctx, const dwg_octree* tree, int min_num, double* d_centres, double* d_widths,
+                      int64_t* out_count);
 int dwg_octree_info(const dwg_octree* tree, int64_t* n_nodes, int64_t* n_leaves, int* max_level);
 /* Per-iteration refresh of the far-field moment EC = Σ A·n per node and of
  * the packed sources (TurboLibiGWN::UpdateMassNormals, TurboLibiGWN.cu:1051-1059).
