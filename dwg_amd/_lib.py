@@ -25,14 +25,14 @@ DWG_FAR_BF16 = 8
 
 EXPORTS = [
     "dwg_ctx_create", "dwg_ctx_destroy", "dwg_ctx_error", "dwg_ctx_sync", "dwg_ctx_profile",
-    "dwg_ctx_profile_get", "dwg_ctx_profile_reset", "dwg_gwn_direct", "dwg_octree_build", "dwg_octree_build_device",
+    "dwg_ctx_profile_get", "dwg_ctx_profile_reset", "dwg_ctx_counter_get", "dwg_gwn_direct", "dwg_octree_build", "dwg_octree_build_device",
     "dwg_octree_leaves",
     "dwg_octree_destroy", "dwg_octree_info", "dwg_octree_update", "dwg_gwn_tree", "dwg_gwn_tree_grid",
     "dwg_gwn_tree_terms", "dwg_gwn_tree_grid_terms", "dwg_knn_build", "dwg_knn_destroy", "dwg_knn_query",
     "dwg_marching_cubes", "dwg_diffuse_normals", "dwg_radii", "dwg_gauss_map", "dwg_transfer_normals",
     "dwg_outlier_filter", "dwg_reduce_sum", "dwg_bounds", "dwg_options_default", "dwg_launch",
     "dwg_free", "dwg_mlp_num_params", "dwg_mlp_create", "dwg_mlp_load", "dwg_mlp_save", "dwg_mlp_destroy",
-    "dwg_mlp_get", "dwg_mlp_set", "dwg_mlp_patches", "dwg_mlp_forward", "dwg_mlp_loss_grad", "dwg_mlp_adam",
+    "dwg_mlp_get", "dwg_mlp_set", "dwg_mlp_patches", "dwg_mlp_patches_from", "dwg_mlp_init", "dwg_mlp_forward", "dwg_mlp_loss_grad", "dwg_mlp_adam",
     "dwg_mlp_augment", "dwg_gemm_bf16",
 ]
 
@@ -89,6 +89,7 @@ def load():
     L.dwg_ctx_profile.argtypes = [_vp, _int]
     L.dwg_ctx_profile_get.argtypes = [_vp, ctypes.c_char_p, P(ctypes.c_double), P(_i64)]
     L.dwg_ctx_profile_reset.argtypes = [_vp]
+    L.dwg_ctx_counter_get.argtypes = [_vp, ctypes.c_char_p, P(_i64)]
     L.dwg_gwn_direct.argtypes = [_vp, _vp, _vp, _vp, _vp, _i64, _vp, _i64, _f, _u32, _vp, _vp]
     L.dwg_octree_build.argtypes = [_vp, _vp, _vp, _i64, _int, _int, P(_vp)]
     L.dwg_octree_build_device.argtypes = [_vp, _vp, _vp, _i64, _int, _int, P(_vp)]
@@ -129,7 +130,9 @@ def load():
     L.dwg_mlp_set.argtypes = [_vp, _vp, _vp]
     L.dwg_mlp_patches.argtypes = [_vp, _vp, _vp, _i64, _vp]
     L.dwg_mlp_forward.argtypes = [_vp, _vp, _vp, _i64, _int, _vp]
-    L.dwg_mlp_loss_grad.argtypes = [_vp, _vp, _vp, _vp, _i64, P(ctypes.c_double)]
+    L.dwg_mlp_patches_from.argtypes = [_vp, _vp, _vp, _i64, _vp, _i64, _vp]
+    L.dwg_mlp_init.argtypes = [_vp, _vp, _vp, _i64, _vp, _vp]
+    L.dwg_mlp_loss_grad.argtypes = [_vp, _vp, _vp, _vp, _i64, _int, P(ctypes.c_double)]
     L.dwg_mlp_adam.argtypes = [_vp, _vp, _f, _f, _f, _f]
     L.dwg_mlp_augment.argtypes = [_vp, _vp, _vp, _vp, _i64, P(_f), _vp, _vp]
     L.dwg_gemm_bf16.argtypes = [_vp, _vp, _i64, _i64, _vp, _i64, _i64, _vp, _i64, _i64, _i64]

@@ -109,6 +109,13 @@ int dwg_ctx_profile_get(dwg_ctx* ctx, const char* name, double* total_ms, int64_
     });
 }
 
+int dwg_ctx_counter_get(dwg_ctx* ctx, const char* name, int64_t* value) {
+    if (!ctx || !name || !value) return DWG_ERR_ARG;
+    auto it = ctx->counters.find(name);
+    *value = it == ctx->counters.end() ? 0 : it->second;
+    return DWG_OK;
+}
+
 int dwg_ctx_profile_reset(dwg_ctx* ctx) {
     if (!ctx) return DWG_ERR_ARG;
     return guarded(ctx, [&] {
@@ -119,6 +126,7 @@ int dwg_ctx_profile_reset(dwg_ctx* ctx) {
                 (void)hipEventDestroy(ab.second);
             }
         ctx->times.clear();
+        ctx->counters.clear();
     });
 }
 

@@ -71,12 +71,16 @@ struct dwg_ctx {
     bool profile = false;
     std::string profile_suffix;  // appended to recorded kernel names (phase tag)
     std::map<std::string, dwg::KernelTimes> times;
+    std::map<std::string, int64_t> counters;  // event counts recorded while profiling
     // named scratch buffers owned by the context
     std::map<std::string, dwg::DevBuf> scratch;
     // pinned host staging for small read-backs
     void* pinned = nullptr;
 
     void* buf(const std::string& name, size_t bytes) { return scratch[name].ensure(bytes); }
+    void count(const std::string& name, int64_t v) {
+        if (profile) counters[name + profile_suffix] += v;
+    }
 };
 
 namespace dwg {

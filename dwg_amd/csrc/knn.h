@@ -20,6 +20,7 @@ struct dwg_knn_index {
         int64_t ncells = 0;
         int64_t* cell_start = nullptr;  // ncells + 1
         float4* pts = nullptr;          // (x, y, z, index bits) bucketed by cell
+        int* empty = nullptr;           // coarse level: Chebyshev distance (cells) to the nearest occupied cell
     } lvl[2];
     ~dwg_knn_index();
 };

@@ -19,6 +19,7 @@ dwg_knn_index::~dwg_knn_index() {
     for (auto& L : lvl) {
         if (L.cell_start) (void)hipFree(L.cell_start);
         if (L.pts) (void)hipFree(L.pts);
+        if (L.empty) (void)hipFree(L.empty);
     }
 }
 
@@ -152,6 +153,48 @@ static void bucket(dwg_ctx* ctx, const float* p, int64_t n, dwg_knn_index::Level
     check_launch();
 }
 
+// Chebyshev distance (in cells) from every cell to the nearest occupied one,
+// as three separable passes: along x the distance to the nearest occupied
+// cell of the row, then d(y) = min_y' max(|y − y'|, d_prev(y')) along y and z.
+__global__ void empty_dist_x_kernel(const int64_t* __restrict__ cell_start, int dx, int dy, int dz,
+                                    int* __restrict__ out) {
+    const int64_t c = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (c >= int64_t(dx) * dy * dz) return;
+    const int x = int(c % dx);
+    const int64_t row = c - x;
+    int best = 1 << 28;
+    for (int x2 = 0; x2 < dx; ++x2)
+        if (cell_start[row + x2 + 1] > cell_start[row + x2]) best = min(best, abs(x2 - x));
+    out[c] = best;
+}
+
+__global__ void empty_dist_pass_kernel(const int* __restrict__ in, int dx, int dy, int dz, int axis,
+                                       int* __restrict__ out) {
+    const int64_t c = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (c >= int64_t(dx) * dy * dz) return;
+    const int x = int(c % dx), y = int((c / dx) % dy), z = int(c / (int64_t(dx) * dy));
+    const int len = axis == 1 ? dy : dz, pos = axis == 1 ? y : z;
+    const int64_t stride = axis == 1 ? int64_t(dx) : int64_t(dx) * dy;
+    const int64_t base = c - int64_t(pos) * stride;
+    int best = 1 << 28;
+    for (int t = 0; t < len; ++t) best = min(best, max(abs(t - pos), in[base + int64_t(t) * stride]));
+    out[c] = best;
+    (void)x;
+}
+
+static void empty_distance(dwg_ctx* ctx, dwg_knn_index::Level& L) {
+    const int dx = L.G.dim[0], dy = L.G.dim[1], dz = L.G.dim[2];
+    DWG_HIP_CHECK(hipMalloc(&L.empty, sizeof(int) * L.ncells));
+    int* tmp = static_cast<int*>(ctx->buf("knn_empty_tmp", sizeof(int) * L.ncells));
+    const unsigned g = grid_for(L.ncells, 256);
+    hipLaunchKernelGGL(empty_dist_x_kernel, dim3(g), dim3(256), 0, ctx->stream, L.cell_start, dx, dy, dz, L.empty);
+    check_launch();
+    hipLaunchKernelGGL(empty_dist_pass_kernel, dim3(g), dim3(256), 0, ctx->stream, L.empty, dx, dy, dz, 1, tmp);
+    check_launch();
+    hipLaunchKernelGGL(empty_dist_pass_kernel, dim3(g), dim3(256), 0, ctx->stream, tmp, dx, dy, dz, 2, L.empty);
+    check_launch();
+}
+
 dwg_knn_index* knn_build(dwg_ctx* ctx, const float* p, int64_t n, int k_hint) {
     DWG_REQUIRE(n > 0 && n < (int64_t(1) << 31) - 1, "kNN index needs 1 ≤ n < 2^31 points");
     float lo[3], hi[3];
@@ -189,6 +232,7 @@ dwg_knn_index* knn_build(dwg_ctx* ctx, const float* p, int64_t n, int k_hint) {
         bucket(ctx, p, n, I->lvl[0], true);
         I->lvl[1].G = grid_args(lo, hi, cell * 8.0);
         bucket(ctx, p, n, I->lvl[1], true);
+        empty_distance(ctx, I->lvl[1]);
     } catch (...) {
         delete I;
         throw;
@@ -225,14 +269,17 @@ __device__ __forceinline__ uint64_t kth_of(const uint64_t (&best)[K], int k) {
 template <int K>
 __device__ bool search_level(const KnnGridArgs& G, const int64_t* __restrict__ cell_start,
                              const float4* __restrict__ pts, float qx, float qy, float qz, int k,
-                             const uint8_t* __restrict__ skip, uint64_t (&best)[K], int rmax) {
+                             const uint8_t* __restrict__ skip, uint64_t (&best)[K], int rmax,
+                             const int* __restrict__ empty) {
     uint64_t kth = ~0ull;
 #pragma unroll
     for (int j = 0; j < K; ++j) best[j] = ~0ull;
     const int c0x = cell_coord(qx, G.lo[0], G.inv_cell, G.dim[0]);
     const int c0y = cell_coord(qy, G.lo[1], G.inv_cell, G.dim[1]);
     const int c0z = cell_coord(qz, G.lo[2], G.inv_cell, G.dim[2]);
-    for (int r = 0; r <= rmax; ++r) {
+    // every cell closer (Chebyshev) than empty[c0] holds no point: start there
+    const int r0 = empty ? min(empty[c0x + int64_t(G.dim[0]) * (c0y + int64_t(G.dim[1]) * c0z)], rmax) : 0;
+    for (int r = r0; r <= rmax; ++r) {
         const int zlo = max(c0z - r, 0), zhi = min(c0z + r, G.dim[2] - 1);
         const int ylo = max(c0y - r, 0), yhi = min(c0y + r, G.dim[1] - 1);
         for (int z = zlo; z <= zhi; ++z) {
@@ -280,22 +327,31 @@ __device__ bool search_level(const KnnGridArgs& G, const int64_t* __restrict__ c
     return false;
 }
 
+// One query per lane; `qlist` (nullable) lists the queries to answer
+// (nq entries), otherwise queries 0..nq−1.
 template <int K>
 __global__ __launch_bounds__(256) void knn_query_kernel(KnnGridArgs G0, const int64_t* __restrict__ cs0,
                                                         const float4* __restrict__ p0, KnnGridArgs G1,
                                                         const int64_t* __restrict__ cs1,
-                                                        const float4* __restrict__ p1, const float* __restrict__ q,
-                                                        int64_t nq, int k, const uint8_t* __restrict__ skip,
+                                                        const float4* __restrict__ p1, const int* __restrict__ empty1,
+                                                        const float* __restrict__ q, int64_t nq,
+                                                        const int* __restrict__ qlist, int k,
+                                                        const uint8_t* __restrict__ skip,
                                                         const uint8_t* __restrict__ qmask, int32_t* __restrict__ out_idx,
                                                         float* __restrict__ out_d2) {
-    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (i >= nq) return;
+    const int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (t >= nq) return;
+    const int64_t i = qlist ? int64_t(qlist[t]) : t;
     if (qmask && !qmask[i]) return;
     const float qx = q[3 * i], qy = q[3 * i + 1], qz = q[3 * i + 2];
     uint64_t best[K];
-    if (!search_level<K>(G0, cs0, p0, qx, qy, qz, k, skip, best, 2)) {
+    // a query whose coarse cell is empty skips the fine shells
+    const int c1 = cell_coord(qx, G1.lo[0], G1.inv_cell, G1.dim[0]) +
+                   G1.dim[0] * (cell_coord(qy, G1.lo[1], G1.inv_cell, G1.dim[1]) +
+                                G1.dim[1] * cell_coord(qz, G1.lo[2], G1.inv_cell, G1.dim[2]));
+    if (empty1[c1] > 0 || !search_level<K>(G0, cs0, p0, qx, qy, qz, k, skip, best, 2, nullptr)) {
         const int maxr = max(G1.dim[0], max(G1.dim[1], G1.dim[2]));
-        (void)search_level<K>(G1, cs1, p1, qx, qy, qz, k, skip, best, maxr);
+        (void)search_level<K>(G1, cs1, p1, qx, qy, qz, k, skip, best, maxr, empty1);
     }
     for (int j = 0; j < K; ++j) {
         if (j >= k) break;
@@ -305,17 +361,176 @@ __global__ __launch_bounds__(256) void knn_query_kernel(KnnGridArgs G0, const in
     }
 }
 
-void knn_query(dwg_ctx* ctx, const dwg_knn_index* I, const float* q, int64_t nq, int k, const uint8_t* skip,
-               const uint8_t* qmask, int32_t* out_idx, float* out_d2, const char* tag) {
-    DWG_REQUIRE(I, "null kNN index");
-    DWG_REQUIRE(k >= 1 && k <= 32, "k must be in [1, 32]");
-    DWG_REQUIRE(nq >= 0, "negative query count");
+// ---------------------------------------------------------------------------
+// Tiled queries.  Queries are bucketed by tile (kTile³ fine cells); one
+// workgroup per non-empty tile stages the points of the tile plus a kHalo
+// ring of cells in LDS and every lane (one query each) scans the same
+// candidates in lockstep, so LDS reads are broadcasts and no lane diverges.
+// A query is final when its k-th key is closer than every face of the
+// staged box that is not on the grid boundary; the rest (sparse or clustered
+// regions) go to the per-lane shell search above.  Both paths keep the k
+// smallest (d², index) keys, so the result does not depend on the path.
+// ---------------------------------------------------------------------------
+constexpr int kTile = 4;
+constexpr int kHalo = 2;
+constexpr int kRegion = kTile + 2 * kHalo;
+constexpr int kRows = kRegion * kRegion;
+constexpr int kStage = 2048;
+constexpr int kTileThreads = 256;
+
+struct TileArgs {
+    int tdim[3];  // tiles per axis
+};
+
+__device__ __forceinline__ int tile_of_point(const KnnGridArgs& G, const TileArgs& T, float x, float y, float z) {
+    const int cx = cell_coord(x, G.lo[0], G.inv_cell, G.dim[0]) / kTile;
+    const int cy = cell_coord(y, G.lo[1], G.inv_cell, G.dim[1]) / kTile;
+    const int cz = cell_coord(z, G.lo[2], G.inv_cell, G.dim[2]) / kTile;
+    return cx + T.tdim[0] * (cy + T.tdim[1] * cz);
+}
+
+__global__ void tile_count_kernel(const float* __restrict__ q, int64_t nq, const uint8_t* __restrict__ qmask,
+                                  KnnGridArgs G, TileArgs T, int* __restrict__ tile_of,
+                                  unsigned long long* __restrict__ count) {
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= nq) return;
+    if (qmask && !qmask[i]) {
+        tile_of[i] = -1;
+        return;
+    }
+    const int t = tile_of_point(G, T, q[3 * i], q[3 * i + 1], q[3 * i + 2]);
+    tile_of[i] = t;
+    atomicAdd(&count[t], 1ull);
+}
+
+__global__ void tile_scatter_kernel(const int* __restrict__ tile_of, int64_t nq, unsigned long long* __restrict__ cursor,
+                                    int* __restrict__ order) {
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= nq) return;
+    const int t = tile_of[i];
+    if (t < 0) return;
+    order[atomicAdd(&cursor[t], 1ull)] = int(i);
+}
+
+__global__ void tile_flags_kernel(const int64_t* __restrict__ start, int64_t ntiles, int64_t* __restrict__ flags) {
+    const int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (t > ntiles) return;
+    flags[t] = (t < ntiles && start[t + 1] > start[t]) ? 1 : 0;
+}
+
+__global__ void tile_list_kernel(const int64_t* __restrict__ flags, int64_t ntiles, int* __restrict__ list) {
+    const int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (t >= ntiles || flags[t + 1] == flags[t]) return;
+    list[flags[t]] = int(t);
+}
+
+template <int K>
+__global__ __launch_bounds__(kTileThreads) void knn_tile_kernel(
+    KnnGridArgs G, TileArgs T, const int64_t* __restrict__ cell_start, const float4* __restrict__ pts,
+    const int* __restrict__ tiles, const int64_t* __restrict__ tile_start, const int* __restrict__ order,
+    const float* __restrict__ q, int k, const uint8_t* __restrict__ skip, int32_t* __restrict__ out_idx,
+    float* __restrict__ out_d2, int* __restrict__ redo, unsigned int* __restrict__ n_redo) {
+    __shared__ float4 s_pts[kStage];
+    __shared__ int64_t s_rb[kRows];
+    __shared__ int s_pref[kRows + 1];
+    const int tile = tiles[blockIdx.x];
+    const int tx = tile % T.tdim[0], ty = (tile / T.tdim[0]) % T.tdim[1], tz = tile / (T.tdim[0] * T.tdim[1]);
+    const int x0 = max(tx * kTile - kHalo, 0), x1 = min(tx * kTile + kTile - 1 + kHalo, G.dim[0] - 1);
+    const int y0 = max(ty * kTile - kHalo, 0), y1 = min(ty * kTile + kTile - 1 + kHalo, G.dim[1] - 1);
+    const int z0 = max(tz * kTile - kHalo, 0), z1 = min(tz * kTile + kTile - 1 + kHalo, G.dim[2] - 1);
+    const int ny = y1 - y0 + 1, nrows = ny * (z1 - z0 + 1);
+    // each (y, z) row of cells x0..x1 is one contiguous range of points
+    if (threadIdx.x < nrows) {
+        const int y = y0 + threadIdx.x % ny, z = z0 + threadIdx.x / ny;
+        const int64_t c = x0 + int64_t(G.dim[0]) * (y + int64_t(G.dim[1]) * z);
+        const int64_t b = cell_start[c], e = cell_start[c + (x1 - x0) + 1];
+        s_rb[threadIdx.x] = b;
+        s_pref[threadIdx.x + 1] = int(e - b);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        s_pref[0] = 0;
+        for (int r = 0; r < nrows; ++r) s_pref[r + 1] += s_pref[r];
+    }
+    __syncthreads();
+    const int total = s_pref[nrows];
+    // faces of the staged box that are not on the grid boundary
+    const float flx = x0 > 0 ? G.lo[0] + float(x0) * G.cell : -INFINITY;
+    const float fhx = x1 + 1 < G.dim[0] ? G.lo[0] + float(x1 + 1) * G.cell : INFINITY;
+    const float fly = y0 > 0 ? G.lo[1] + float(y0) * G.cell : -INFINITY;
+    const float fhy = y1 + 1 < G.dim[1] ? G.lo[1] + float(y1 + 1) * G.cell : INFINITY;
+    const float flz = z0 > 0 ? G.lo[2] + float(z0) * G.cell : -INFINITY;
+    const float fhz = z1 + 1 < G.dim[2] ? G.lo[2] + float(z1 + 1) * G.cell : INFINITY;
+    const int64_t qs = tile_start[tile], qe = tile_start[tile + 1];
+    for (int64_t qb = qs; qb < qe; qb += kTileThreads) {
+        const int64_t t = qb + threadIdx.x;
+        const bool active = t < qe;
+        const int qi = active ? order[t] : 0;
+        const float qx = active ? q[3 * int64_t(qi)] : 0.f, qy = active ? q[3 * int64_t(qi) + 1] : 0.f,
+                    qz = active ? q[3 * int64_t(qi) + 2] : 0.f;
+        uint64_t best[K];
+#pragma unroll
+        for (int j = 0; j < K; ++j) best[j] = ~0ull;
+        uint64_t kth = ~0ull;
+        for (int cb = 0; cb < total; cb += kStage) {
+            const int cnt = min(kStage, total - cb);
+            __syncthreads();
+            for (int j = threadIdx.x; j < cnt; j += kTileThreads) {
+                const int g = cb + j;
+                int lo = 0, hi = nrows - 1;  // last row with s_pref[row] <= g
+                while (lo < hi) {
+                    const int mid = (lo + hi + 1) >> 1;
+                    if (s_pref[mid] <= g) lo = mid;
+                    else hi = mid - 1;
+                }
+                float4 p = pts[s_rb[lo] + (g - s_pref[lo])];
+                if (skip && skip[__float_as_int(p.w)]) p.w = __int_as_float(-1);
+                s_pts[j] = p;
+            }
+            __syncthreads();
+            if (active) {
+                for (int j = 0; j < cnt; ++j) {
+                    const float4 p = s_pts[j];
+                    const int idx = __float_as_int(p.w);
+                    if (idx < 0) continue;
+                    const float d2 = dist2(qx - p.x, qy - p.y, qz - p.z);
+                    const uint64_t key = (uint64_t(__float_as_uint(d2)) << 32) | uint32_t(idx);
+                    if (key < kth) {
+                        knn_insert<K>(best, k, key);
+                        kth = kth_of<K>(best, k);
+                    }
+                }
+            }
+        }
+        if (!active) continue;
+        const float dmin = fminf(fminf(fminf(qx - flx, fhx - qx), fminf(qy - fly, fhy - qy)), fminf(qz - flz, fhz - qz));
+        bool exact = dmin == INFINITY;
+        if (!exact && kth != ~0ull) {
+            const float kd2 = __uint_as_float(uint32_t(kth >> 32));
+            const float safe = dmin * (1.0f - 1e-5f) - 1e-6f * G.cell;
+            exact = safe > 0.f && kd2 < safe * safe;
+        }
+        if (!exact) {
+            redo[atomicAdd(n_redo, 1u)] = qi;
+            continue;
+        }
+        for (int j = 0; j < K; ++j) {
+            if (j >= k) break;
+            const uint64_t b = best[j];
+            out_idx[int64_t(qi) * k + j] = b == ~0ull ? -1 : int32_t(uint32_t(b));
+            if (out_d2) out_d2[int64_t(qi) * k + j] = b == ~0ull ? INFINITY : __uint_as_float(uint32_t(b >> 32));
+        }
+    }
+}
+
+static void launch_per_lane(dwg_ctx* ctx, const dwg_knn_index* I, const float* q, int64_t nq, const int* qlist, int k,
+                            const uint8_t* skip, const uint8_t* qmask, int32_t* out_idx, float* out_d2) {
     if (nq == 0) return;
     dim3 grid(grid_for(nq, 256)), block(256);
-    ProfileScope ps(ctx, tag ? tag : "knn_query");
-#define DWG_KNN(KK) \
-    hipLaunchKernelGGL(knn_query_kernel<KK>, grid, block, 0, ctx->stream, I->lvl[0].G, I->lvl[0].cell_start,   \
-                       I->lvl[0].pts, I->lvl[1].G, I->lvl[1].cell_start, I->lvl[1].pts, q, nq, k, skip, qmask, \
+#define DWG_KNN(KK)                                                                                                  \
+    hipLaunchKernelGGL(knn_query_kernel<KK>, grid, block, 0, ctx->stream, I->lvl[0].G, I->lvl[0].cell_start,        \
+                       I->lvl[0].pts, I->lvl[1].G, I->lvl[1].cell_start, I->lvl[1].pts, I->lvl[1].empty, q, nq, qlist, \
+                       k, skip, qmask,                                                                            \
                        out_idx, out_d2)
     if (k <= 8) DWG_KNN(8);
     else if (k <= 10) DWG_KNN(10);
@@ -323,6 +538,71 @@ void knn_query(dwg_ctx* ctx, const dwg_knn_index* I, const float* q, int64_t nq,
     else DWG_KNN(32);
 #undef DWG_KNN
     check_launch();
+}
+
+void knn_query(dwg_ctx* ctx, const dwg_knn_index* I, const float* q, int64_t nq, int k, const uint8_t* skip,
+               const uint8_t* qmask, int32_t* out_idx, float* out_d2, const char* tag) {
+    DWG_REQUIRE(I, "null kNN index");
+    DWG_REQUIRE(k >= 1 && k <= 32, "k must be in [1, 32]");
+    DWG_REQUIRE(nq >= 0, "negative query count");
+    DWG_REQUIRE(nq < (int64_t(1) << 31) - 1, "too many queries");
+    if (nq == 0) return;
+    ProfileScope ps(ctx, tag ? tag : "knn_query");
+    if (nq < 4096) {  // too few to pay for the bucketing
+        launch_per_lane(ctx, I, q, nq, nullptr, k, skip, qmask, out_idx, out_d2);
+        return;
+    }
+    const KnnGridArgs& G = I->lvl[0].G;
+    TileArgs T;
+    for (int d = 0; d < 3; ++d) T.tdim[d] = (G.dim[d] + kTile - 1) / kTile;
+    const int64_t ntiles = int64_t(T.tdim[0]) * T.tdim[1] * T.tdim[2];
+    int* tile_of = static_cast<int*>(ctx->buf("knnt_tile_of", sizeof(int) * nq));
+    int64_t* start = static_cast<int64_t*>(ctx->buf("knnt_start", sizeof(int64_t) * (ntiles + 1)));
+    unsigned long long* cursor = static_cast<unsigned long long*>(ctx->buf("knnt_cursor", sizeof(uint64_t) * ntiles));
+    int* order = static_cast<int*>(ctx->buf("knnt_order", sizeof(int) * nq));
+    int64_t* flags = static_cast<int64_t*>(ctx->buf("knnt_flags", sizeof(int64_t) * (ntiles + 1)));
+    int* list = static_cast<int*>(ctx->buf("knnt_list", sizeof(int) * ntiles));
+    int* redo = static_cast<int*>(ctx->buf("knnt_redo", sizeof(int) * nq + 64));
+    unsigned int* n_redo = static_cast<unsigned int*>(ctx->buf("knnt_nredo", 64));
+    fill_u32(ctx, start, 0, 2 * (ntiles + 1));
+    fill_u32(ctx, n_redo, 0, 1);
+    hipLaunchKernelGGL(tile_count_kernel, dim3(grid_for(nq, 256)), dim3(256), 0, ctx->stream, q, nq, qmask, G, T,
+                       tile_of, reinterpret_cast<unsigned long long*>(start));
+    check_launch();
+    exclusive_scan<int64_t>(ctx, start, ntiles);
+    hipLaunchKernelGGL(copy_i64_kernel, dim3(grid_for(ntiles, 256)), dim3(256), 0, ctx->stream, start, cursor, ntiles);
+    check_launch();
+    hipLaunchKernelGGL(tile_scatter_kernel, dim3(grid_for(nq, 256)), dim3(256), 0, ctx->stream, tile_of, nq, cursor,
+                       order);
+    check_launch();
+    hipLaunchKernelGGL(tile_flags_kernel, dim3(grid_for(ntiles + 1, 256)), dim3(256), 0, ctx->stream, start, ntiles,
+                       flags);
+    check_launch();
+    exclusive_scan<int64_t>(ctx, flags, ntiles);
+    hipLaunchKernelGGL(tile_list_kernel, dim3(grid_for(ntiles, 256)), dim3(256), 0, ctx->stream, flags, ntiles, list);
+    check_launch();
+    const int64_t nlist = read_scalar<int64_t>(ctx, flags + ntiles);
+    const std::string base = tag ? tag : "knn_query";
+    if (ctx->profile) ctx->count(base + ":queries", read_scalar<int64_t>(ctx, start + ntiles));
+    if (nlist > 0) {
+        const std::string name = base + ":tiles";
+        ProfileScope pt(ctx, name.c_str());
+#define DWG_KNNT(KK)                                                                                                  \
+    hipLaunchKernelGGL(knn_tile_kernel<KK>, dim3(unsigned(nlist)), dim3(kTileThreads), 0, ctx->stream, G, T,         \
+                       I->lvl[0].cell_start, I->lvl[0].pts, list, start, order, q, k, skip, out_idx, out_d2, redo, \
+                       n_redo)
+        if (k <= 8) DWG_KNNT(8);
+        else if (k <= 10) DWG_KNNT(10);
+        else if (k <= 16) DWG_KNNT(16);
+        else DWG_KNNT(32);
+#undef DWG_KNNT
+        check_launch();
+    }
+    const int64_t nredo = int64_t(read_scalar<unsigned int>(ctx, n_redo));
+    ctx->count(base + ":redo", nredo);
+    const std::string name = base + ":redo";
+    ProfileScope pr(ctx, name.c_str());
+    launch_per_lane(ctx, I, q, nredo, redo, k, skip, nullptr, out_idx, out_d2);
 }
 
 }  // namespace dwg

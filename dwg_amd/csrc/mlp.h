@@ -16,6 +16,9 @@ constexpr int64_t kOffB2 = kOffW2 + int64_t(kHid) * kHid;
 constexpr int64_t kOffW3 = kOffB2 + kHid;
 constexpr int64_t kOffB3 = kOffW3 + int64_t(kOut) * kHid;
 constexpr int64_t kParams = kOffB3 + kOut;  // 23,171
+// points per training cloud: larger clouds take patches from a subsample of
+// this size
+constexpr int64_t kTrainPoints = 50000;
 
 }  // namespace dwg
 

@@ -2,18 +2,26 @@
 // MLP with GELU over k-NN patches, trained and run in BF16 with FP32
 // accumulation on the matrix cores.
 //
-// Patches: the 16 nearest neighbours of a point (exact kNN, (d², index)
-// order, the point itself dropped), offsets p_j − p_i divided by the distance
-// to the 16th neighbour, flattened to 48 values in distance order.
+// Patches: the 16 nearest neighbours of a point among the patch points
+// (exact kNN, (d², index) order; the first neighbour at distance 0 — the
+// point itself when it is a patch point — is dropped), offsets p_j − p_i
+// divided by the distance to the 16th neighbour, flattened to 48 values in
+// distance order.  A cloud larger than the training clouds (50,000 points)
+// takes its patch points from a hashed subsample of that size, so patches
+// span the same fraction of the shape as in training.
 //
 // Every product is one launch of gemm_bf16: C = A·B with arbitrary element
 // strides for A (M×K), B (K×N) and C (M×N), BF16 operands, FP32
 // accumulation, one 32×32 output tile per wave on v_mfma_f32_32x32x8_bf16.
 // GELU is the tanh form of Hendrycks & Gimpel (2016).  The loss is the mean
-// of 1 − ŷ·t over the batch (ŷ = y/|y|, t the unit target normal).
+// over the batch of 1 − ŷ·t (oriented) or 1 − (ŷ·t)² (normal line, sign
+// free), ŷ = y/|y|, t the unit target normal.  The initialisation orients
+// the predicted line away from the bounding-box centre, as the Gauss map
+// does (TurboNormalInit.cu:3-44).
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <memory>
 #include <random>
 
 #include "knn.h"
@@ -43,7 +51,8 @@ __device__ __forceinline__ float from_bf16(uint16_t h) { return __uint_as_float(
 // patches
 // ---------------------------------------------------------------------------
 __global__ void patches_kernel(const int32_t* __restrict__ knn, const float* __restrict__ d2, int kk,
-                               const float* __restrict__ p, int64_t n, uint16_t* __restrict__ out) {
+                               const float* __restrict__ ps, const float* __restrict__ q, int64_t n,
+                               uint16_t* __restrict__ out) {
     const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i >= n) return;
     int sel[kPatchK];
@@ -51,13 +60,13 @@ __global__ void patches_kernel(const int32_t* __restrict__ knn, const float* __r
     float last_d2 = 0.f;
     bool self_skipped = false;
     for (int j = 0; j < kk && m < kPatchK; ++j) {
-        const int32_t q = knn[i * kk + j];
-        if (q < 0) break;
-        if (!self_skipped && q == int32_t(i)) {
+        const int32_t c = knn[i * kk + j];
+        if (c < 0) break;
+        if (!self_skipped && d2[i * kk + j] == 0.f) {
             self_skipped = true;
             continue;
         }
-        sel[m++] = q;
+        sel[m++] = c;
         last_d2 = d2[i * kk + j];
     }
     const float r = last_d2 > 0.f ? sqrtf(last_d2) : 1.f;
@@ -65,11 +74,46 @@ __global__ void patches_kernel(const int32_t* __restrict__ knn, const float* __r
     for (int j = 0; j < kPatchK; ++j) {
         float v[3] = {0.f, 0.f, 0.f};
         if (j < m) {
-            const int64_t q = sel[j];
-            for (int d = 0; d < 3; ++d) v[d] = (p[3 * q + d] - p[3 * i + d]) * inv;
+            const int64_t c = sel[j];
+            for (int d = 0; d < 3; ++d) v[d] = (ps[3 * c + d] - q[3 * i + d]) * inv;
         }
         for (int d = 0; d < 3; ++d) out[i * kIn + 3 * j + d] = to_bf16(v[d]);
     }
+}
+
+// keep point i of the patch subsample when its hashed index falls in the
+// first 1/stride of the range
+__global__ void subsample_flags_kernel(int64_t n, uint32_t stride, int64_t* __restrict__ flags) {
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i > n) return;
+    if (i == n) {
+        flags[i] = 0;
+        return;
+    }
+    uint32_t h = uint32_t(i) * 0x9E3779B1u;
+    h ^= h >> 16;
+    h *= 0x85EBCA6Bu;
+    h ^= h >> 13;
+    flags[i] = (h % stride) == 0 ? 1 : 0;
+}
+
+__global__ void subsample_gather_kernel(const int64_t* __restrict__ slot, const float* __restrict__ p, int64_t n,
+                                        float* __restrict__ out) {
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n || slot[i + 1] == slot[i]) return;
+    const int64_t j = slot[i];
+    for (int d = 0; d < 3; ++d) out[3 * j + d] = p[3 * i + d];
+}
+
+// orient the predicted line away from c (n·(p − c) ≥ 0)
+__global__ void orient_kernel(float* __restrict__ nrm, const float* __restrict__ p, int64_t n, float cx, float cy,
+                              float cz) {
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float d = nrm[3 * i] * (p[3 * i] - cx) + nrm[3 * i + 1] * (p[3 * i + 1] - cy) +
+                    nrm[3 * i + 2] * (p[3 * i + 2] - cz);
+    if (d < 0.f)
+        for (int k = 0; k < 3; ++k) nrm[3 * i + k] = -nrm[3 * i + k];
 }
 
 // ---------------------------------------------------------------------------
@@ -177,8 +221,9 @@ __global__ void add_bias_normalize_kernel(float* __restrict__ y, const float* __
     for (int c = 0; c < 3; ++c) y[3 * i + c] = v[c];
 }
 
-// loss = mean(1 − ŷ·t);  dL/dy = −(t − (ŷ·t)ŷ)/(|y|·B)
-__global__ void loss_kernel(const float* __restrict__ y, const float* __restrict__ t, int64_t rows,
+// oriented:  loss = mean(1 − c),  dL/dy = −(t − c ŷ)/(|y|·B)
+// line:      loss = mean(1 − c²), dL/dy = −2c (t − c ŷ)/(|y|·B),   c = ŷ·t
+__global__ void loss_kernel(const float* __restrict__ y, const float* __restrict__ t, int64_t rows, int mode,
                             float* __restrict__ dy, uint16_t* __restrict__ dy16, double* __restrict__ loss_part) {
     __shared__ double lds[256];
     const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -189,8 +234,8 @@ __global__ void loss_kernel(const float* __restrict__ y, const float* __restrict
         const float h0 = y0 / ny, h1 = y1 / ny, h2 = y2 / ny;
         const float t0 = t[3 * i], t1 = t[3 * i + 1], t2 = t[3 * i + 2];
         const float c = h0 * t0 + h1 * t1 + h2 * t2;
-        l = 1.0 - double(c);
-        const float s = -1.0f / (ny * float(rows));
+        l = mode ? 1.0 - double(c) * double(c) : 1.0 - double(c);
+        const float s = (mode ? -2.0f * c : -1.0f) / (ny * float(rows));
         const float g[3] = {s * (t0 - c * h0), s * (t1 - c * h1), s * (t2 - c * h2)};
         for (int k = 0; k < 3; ++k) {
             dy[3 * i + k] = g[k];
@@ -360,15 +405,21 @@ void mlp_save(dwg_ctx* ctx, const dwg_mlp* m, const char* path) {
     DWG_REQUIRE(ok, std::string("cannot write ") + path);
 }
 
-void mlp_patches(dwg_ctx* ctx, const dwg_knn_index* I, const float* p, int64_t n, uint16_t* out) {
-    DWG_REQUIRE(I && I->n == n, "kNN index does not match the points");
+void mlp_patches_from(dwg_ctx* ctx, const dwg_knn_index* I, const float* src, int64_t m, const float* q, int64_t n,
+                      uint16_t* out) {
+    DWG_REQUIRE(I && I->n == m, "kNN index does not match the patch points");
+    if (n == 0) return;
     const int kk = kPatchK + 1;
     int32_t* knn = static_cast<int32_t*>(ctx->buf("mlp_knn", sizeof(int32_t) * n * kk));
     float* d2 = static_cast<float*>(ctx->buf("mlp_d2", sizeof(float) * n * kk));
-    knn_query(ctx, I, p, n, kk, nullptr, nullptr, knn, d2, "knn_query_patches");
+    knn_query(ctx, I, q, n, kk, nullptr, nullptr, knn, d2, "knn_query_patches");
     ProfileScope ps(ctx, "mlp_patches");
-    hipLaunchKernelGGL(patches_kernel, dim3(grid_for(n, 256)), dim3(256), 0, ctx->stream, knn, d2, kk, p, n, out);
+    hipLaunchKernelGGL(patches_kernel, dim3(grid_for(n, 256)), dim3(256), 0, ctx->stream, knn, d2, kk, src, q, n, out);
     check_launch();
+}
+
+void mlp_patches(dwg_ctx* ctx, const dwg_knn_index* I, const float* p, int64_t n, uint16_t* out) {
+    mlp_patches_from(ctx, I, p, n, p, n, out);
 }
 
 struct Acts {
@@ -414,7 +465,7 @@ void mlp_forward(dwg_ctx* ctx, const dwg_mlp* m, const uint16_t* x, int64_t n, f
     }
 }
 
-double mlp_loss_grad(dwg_ctx* ctx, dwg_mlp* m, const uint16_t* x, const float* target, int64_t B) {
+double mlp_loss_grad(dwg_ctx* ctx, dwg_mlp* m, const uint16_t* x, const float* target, int64_t B, int mode) {
     DWG_REQUIRE(B > 0, "empty batch");
     ProfileScope ps(ctx, "mlp_train");
     Acts a = acts(ctx, B);
@@ -426,7 +477,7 @@ double mlp_loss_grad(dwg_ctx* ctx, dwg_mlp* m, const uint16_t* x, const float* t
     uint16_t* dy16 = static_cast<uint16_t*>(ctx->buf("mlp_dy16", sizeof(uint16_t) * B * kOut));
     const unsigned lb = grid_for(B, 256);
     double* lpart = static_cast<double*>(ctx->buf("mlp_loss", sizeof(double) * (lb + 1)));
-    hipLaunchKernelGGL(loss_kernel, dim3(lb), dim3(256), 0, ctx->stream, a.y, target, B, dy, dy16, lpart);
+    hipLaunchKernelGGL(loss_kernel, dim3(lb), dim3(256), 0, ctx->stream, a.y, target, B, mode, dy, dy16, lpart);
     check_launch();
     float* G = m->grad;
     const uint16_t* W = m->bf16;
@@ -486,6 +537,33 @@ void mlp_get(dwg_ctx* ctx, const dwg_mlp* m, float* h_params, float* h_grad) {
 }
 
 void mlp_put(dwg_ctx* ctx, dwg_mlp* m, const float* h_params) { mlp_set(ctx, m, h_params); }
+
+void mlp_init(dwg_ctx* ctx, const dwg_mlp* m, const float* p, int64_t n, const float c[3], float* out) {
+    DWG_REQUIRE(n > 0, "no points");
+    ProfileScope prof(ctx, "mlp_init");
+    const float* src = p;
+    int64_t ns = n;
+    if (n > kTrainPoints) {
+        const uint32_t stride = uint32_t((n + kTrainPoints - 1) / kTrainPoints);
+        int64_t* slot = static_cast<int64_t*>(ctx->buf("mlp_sub_slot", sizeof(int64_t) * (n + 1)));
+        hipLaunchKernelGGL(subsample_flags_kernel, dim3(grid_for(n + 1, 256)), dim3(256), 0, ctx->stream, n, stride,
+                           slot);
+        check_launch();
+        exclusive_scan<int64_t>(ctx, slot, n);
+        ns = read_scalar<int64_t>(ctx, slot + n);
+        DWG_REQUIRE(ns > kPatchK, "patch subsample too small");
+        float* sub = static_cast<float*>(ctx->buf("mlp_sub_pts", sizeof(float) * 3 * ns));
+        hipLaunchKernelGGL(subsample_gather_kernel, dim3(grid_for(n, 256)), dim3(256), 0, ctx->stream, slot, p, n, sub);
+        check_launch();
+        src = sub;
+    }
+    std::unique_ptr<dwg_knn_index> index(knn_build(ctx, src, ns, kPatchK + 1));
+    uint16_t* patches = static_cast<uint16_t*>(ctx->buf("mlp_init_patches", sizeof(uint16_t) * kIn * n));
+    mlp_patches_from(ctx, index.get(), src, ns, p, n, patches);
+    mlp_forward(ctx, m, patches, n, out, 1);
+    hipLaunchKernelGGL(orient_kernel, dim3(grid_for(n, 256)), dim3(256), 0, ctx->stream, out, p, n, c[0], c[1], c[2]);
+    check_launch();
+}
 
 void gemm_bf16_test(dwg_ctx* ctx, const uint16_t* A, int64_t sAi, int64_t sAk, const uint16_t* B, int64_t sBk,
                     int64_t sBj, float* C, int64_t M, int64_t N, int64_t K) {
@@ -554,6 +632,18 @@ int dwg_mlp_patches(dwg_ctx* ctx, const dwg_knn_index* index, const float* d_pts
     return mlp_guard(ctx, [&] { dwg::mlp_patches(ctx, index, d_pts, n, d_patches); });
 }
 
+int dwg_mlp_patches_from(dwg_ctx* ctx, const dwg_knn_index* index, const float* d_src, int64_t n_src,
+                         const float* d_q, int64_t nq, uint16_t* d_patches) {
+    if (!index || !d_src || (nq > 0 && (!d_q || !d_patches))) return DWG_ERR_ARG;
+    return mlp_guard(ctx, [&] { dwg::mlp_patches_from(ctx, index, d_src, n_src, d_q, nq, d_patches); });
+}
+
+int dwg_mlp_init(dwg_ctx* ctx, const dwg_mlp* mlp, const float* d_pts, int64_t n, const float* h_center,
+                 float* d_out) {
+    if (!mlp || !d_pts || !h_center || !d_out) return DWG_ERR_ARG;
+    return mlp_guard(ctx, [&] { dwg::mlp_init(ctx, mlp, d_pts, n, h_center, d_out); });
+}
+
 int dwg_mlp_forward(dwg_ctx* ctx, const dwg_mlp* mlp, const uint16_t* d_patches, int64_t n, int normalize,
                     float* d_out) {
     if (!mlp || (n > 0 && (!d_patches || !d_out))) return DWG_ERR_ARG;
@@ -563,10 +653,10 @@ int dwg_mlp_forward(dwg_ctx* ctx, const dwg_mlp* mlp, const uint16_t* d_patches,
 }
 
 int dwg_mlp_loss_grad(dwg_ctx* ctx, dwg_mlp* mlp, const uint16_t* d_patches, const float* d_target, int64_t batch,
-                      double* h_loss) {
-    if (!mlp || !d_patches || !d_target) return DWG_ERR_ARG;
+                      int mode, double* h_loss) {
+    if (!mlp || !d_patches || !d_target || mode < 0 || mode > 1) return DWG_ERR_ARG;
     return mlp_guard(ctx, [&] {
-        const double l = dwg::mlp_loss_grad(ctx, mlp, d_patches, d_target, batch);
+        const double l = dwg::mlp_loss_grad(ctx, mlp, d_patches, d_target, batch, mode);
         if (h_loss) *h_loss = l;
     });
 }

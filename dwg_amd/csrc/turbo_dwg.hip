@@ -229,9 +229,12 @@ void TurboDWG::GaussMapInit() {
 }
 
 void TurboDWG::MlpInit(const dwg_mlp* mlp) {
-    DevArray<uint16_t> patches(48 * n_);
-    check(dwg_mlp_patches(ctx_, knn_, pts_.ptr, n_, patches.ptr), "mlp patches");
-    check(dwg_mlp_forward(ctx_, mlp, patches.ptr, n_, 1, nrm_.ptr), "mlp forward");
+    // the predicted normal line, oriented like the Gauss map: away from the
+    // bounding-box centre (TurboNormalInit.cu:3-44)
+    float lo[3], hi[3];
+    check(dwg_bounds(ctx_, pts_.ptr, n_, lo, hi), "bounds");
+    const float c[3] = {(lo[0] + hi[0]) / 2.0f, (lo[1] + hi[1]) / 2.0f, (lo[2] + hi[2]) / 2.0f};
+    check(dwg_mlp_init(ctx_, mlp, pts_.ptr, n_, c, nrm_.ptr), "mlp init");
 }
 
 void TurboDWG::download_normals(std::vector<float>& out) const {
@@ -337,7 +340,9 @@ void TurboDWG::mesh_pass(const Grid& grid, float iso_scale, int64_t* tri_count, 
         explicit Tag(dwg_ctx* x) : c(x) { c->profile_suffix = ":final"; }
         ~Tag() { c->profile_suffix.clear(); }
     } tag(ctx_);
-    const uint32_t flags = DWG_SCREENED | (opt_.use_radii ? DWG_USE_RADII : 0u) | opt_.precision;
+    // the mesh that is returned comes from FP32 far-field moments; BF16 is
+    // confined to the iterations
+    const uint32_t flags = DWG_SCREENED | (opt_.use_radii ? DWG_USE_RADII : 0u) | (opt_.precision & ~DWG_FAR_BF16);
     const dwg_grid g = grid.abi();
     DevArray<float> wg(grid.size());
     const float* rad = opt_.use_radii ? radii_.ptr : nullptr;

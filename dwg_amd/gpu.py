@@ -78,6 +78,11 @@ class Context:
     def profile_reset(self):
         self.check(self.L.dwg_ctx_profile_reset(self.h))
 
+    def counter(self, name):
+        v = ctypes.c_int64()
+        self.check(self.L.dwg_ctx_counter_get(self.h, name.encode(), ctypes.byref(v)))
+        return v.value
+
     def f32(self, x):
         return _f32(x, self.device)
 
@@ -315,6 +320,25 @@ class Mlp:
         self.ctx.check(self.ctx.L.dwg_mlp_patches(self.ctx.h, index.h, _ptr(index.pts), index.n, _ptr(out)))
         return out
 
+    def patches_from(self, index: KnnIndex, q):
+        """Patches of the points q over the points of `index`."""
+        q = self.ctx.f32(q)
+        nq = q.numel() // 3
+        out = torch.empty((nq, 48), device=self.ctx.device, dtype=torch.int16)
+        self.ctx.check(self.ctx.L.dwg_mlp_patches_from(self.ctx.h, index.h, _ptr(index.pts), index.n, _ptr(q), nq,
+                                                       _ptr(out)))
+        return out
+
+    def init_normals(self, pts, center):
+        """Learned initialisation (dwg_mlp_init): normal line oriented away
+        from `center`."""
+        p = self.ctx.f32(pts)
+        n = p.numel() // 3
+        out = torch.empty((n, 3), device=self.ctx.device, dtype=torch.float32)
+        c = (ctypes.c_float * 3)(*[float(v) for v in center])
+        self.ctx.check(self.ctx.L.dwg_mlp_init(self.ctx.h, self.h, _ptr(p), n, c, _ptr(out)))
+        return out
+
     def forward(self, patches, normalize=True):
         n = patches.shape[0]
         out = torch.empty((n, 3), device=self.ctx.device, dtype=torch.float32)
@@ -322,11 +346,13 @@ class Mlp:
                                                   _ptr(out)))
         return out
 
-    def loss_grad(self, patches, target):
+    def loss_grad(self, patches, target, line=False):
+        """Loss and gradient: 1 − ŷ·t (oriented) or, with line=True,
+        1 − (ŷ·t)² (normal line, sign free)."""
         loss = ctypes.c_double()
         t = target.to(device=self.ctx.device, dtype=torch.float32).contiguous()
         self.ctx.check(self.ctx.L.dwg_mlp_loss_grad(self.ctx.h, self.h, _ptr(patches), _ptr(t), patches.shape[0],
-                                                    ctypes.byref(loss)))
+                                                    1 if line else 0, ctypes.byref(loss)))
         return loss.value
 
     def adam(self, lr=1e-3, beta1=0.9, beta2=0.999, eps=1e-8):

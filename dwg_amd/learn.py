@@ -4,13 +4,18 @@ Training data: DWG's own converged normals.  Each training shape is the
 union-of-32-primitives generator at 50,000 points (seeds 1000, 1001, …);
 plain DWG (Gauss-map init, 40 iterations, points as sources) runs on the
 GPU and its final point normals are the targets for the 16-neighbour
-patches of the same points.  Training is Adam on mean(1 − ŷ·t) with one
-random rotation applied to each batch (patches and targets together).
+patches of the same points.  Training is Adam on the normal-line loss
+mean(1 − (ŷ·t)²) with one random rotation applied to each batch (patches
+and targets together): a local patch fixes the line of the normal but not
+reliably its side, so the initialisation takes the side from the Gauss-map
+hemisphere (away from the bounding-box centre) and the DWG iterations
+resolve the rest.
 
 Evaluation on held-out shapes (seeds 2000, …): the plain method's final
-mean normal angle against the analytic normals, and the angle the learned
-initialisation reaches after k DWG iterations, k = 0, 2, 4, …; the result
-is the smallest k within 0.5° of the plain method and its wall time.
+mean normal angle against the analytic normals (Gauss-map start, 40
+iterations), and the angle reached after k DWG iterations from the learned
+start and from the Gauss-map start, k = 0, 2, 4, …; the result is the
+smallest k within 0.5° of the plain method and its wall time.
 
     python -m dwg_amd.learn --shapes 512 --steps 6000 --out weights.bin --eval 64
 """
@@ -66,7 +71,7 @@ def random_rotation(rng):
                      [2 * (x * z - y * w), 2 * (y * z + x * w), 1 - 2 * (x * x + y * y)]], np.float32)
 
 
-def train(ctx, net, X, T, steps=6000, batch=8192, lr=2e-3, seed=0, augment=True):
+def train(ctx, net, X, T, steps=6000, batch=8192, lr=2e-3, seed=0, augment=True, line=True):
     rng = np.random.default_rng(seed)
     n = X.shape[0]
     xb = torch.empty((batch, 48), dtype=torch.int16, device=X.device)
@@ -77,7 +82,7 @@ def train(ctx, net, X, T, steps=6000, batch=8192, lr=2e-3, seed=0, augment=True)
         sel = torch.from_numpy(rng.integers(0, n, batch)).to(X.device)
         R = random_rotation(rng) if augment else np.eye(3, dtype=np.float32)
         net.augment(X, T, sel, R, xb, tb)
-        loss = net.loss_grad(xb, tb)
+        loss = net.loss_grad(xb, tb, line=line)
         # constant learning rate for 60% of the steps, then cosine decay
         f = max(0.0, (step - 0.6 * steps) / (0.4 * steps))
         net.adam(lr=lr * (0.02 + 0.98 * 0.5 * (1 + np.cos(np.pi * f))))
@@ -87,7 +92,7 @@ def train(ctx, net, X, T, steps=6000, batch=8192, lr=2e-3, seed=0, augment=True)
     return hist
 
 
-def evaluate(ctx, net, seeds, n=50_000, ks=(0, 2, 4, 6, 8, 10, 12, 16, 20), tol_deg=0.5):
+def evaluate(ctx, net, seeds, n=50_000, ks=(0, 2, 4, 6, 8, 10, 12, 16, 20, 24, 30), tol_deg=0.5):
     rows = []
     for s in seeds:
         P, N = shapes.union(n, 32, s)
@@ -102,6 +107,8 @@ def evaluate(ctx, net, seeds, n=50_000, ks=(0, 2, 4, 6, 8, 10, 12, 16, 20), tol_
         reach = [(k, a, t) for k, a, t in row["learned"] if a <= plain + tol_deg]
         row["k_reach"] = reach[0][0] if reach else None
         row["t_reach"] = reach[0][2] if reach else None
+        reach = [(k, a, t) for k, a, t in row["plain_k"] if a <= plain + tol_deg]
+        row["k_reach_plain"] = reach[0][0] if reach else None
         rows.append(row)
         log(f"eval seed {s}: plain {plain:.2f}° in {base.time_total_s:.3f} s; learned "
             + ", ".join(f"k={k}:{a:.2f}°" for k, a, _ in row["learned"]) + "; plain "
@@ -111,7 +118,9 @@ def evaluate(ctx, net, seeds, n=50_000, ks=(0, 2, 4, 6, 8, 10, 12, 16, 20), tol_
 
 def summary(rows):
     ks = [r["k_reach"] for r in rows if r["k_reach"] is not None]
+    kp = [r["k_reach_plain"] for r in rows if r["k_reach_plain"] is not None]
     return {"shapes": len(rows), "reached": len(ks), "median_k": float(np.median(ks)) if ks else None,
+            "reached_gauss": len(kp), "median_k_gauss": float(np.median(kp)) if kp else None,
             "plain_mean_deg": float(np.mean([r["plain_deg"] for r in rows])),
             "learned_k0_mean_deg": float(np.mean([r["learned"][0][1] for r in rows])),
             "median_plain_s": float(np.median([r["plain_s"] for r in rows])),

@@ -60,6 +60,9 @@ int dwg_ctx_profile(dwg_ctx* ctx, int enable);
 /* Sum of recorded durations (ms) and launch count of kernel `name`; syncs. */
 int dwg_ctx_profile_get(dwg_ctx* ctx, const char* name, double* total_ms, int64_t* launches);
 int dwg_ctx_profile_reset(dwg_ctx* ctx);
+/* Event count `name` recorded while profiling (e.g. "knn_query_centroids:redo",
+ * queries that left the tiled kNN path); 0 if none. */
+int dwg_ctx_counter_get(dwg_ctx* ctx, const char* name, int64_t* value);
 
 /* ---- winding-number sums ------------------------------------------------ */
 /* Direct all-pairs sums w(q) = Σ_j term(p_j − q) over n sources at nq
@@ -181,12 +184,22 @@ int dwg_mlp_set(dwg_ctx* ctx, dwg_mlp* mlp, const float* h_params);
 /* n×48 BF16 patches (uint16 bit patterns) of the indexed points */
 int dwg_mlp_patches(dwg_ctx* ctx, const dwg_knn_index* index, const float* d_pts, int64_t n,
                     uint16_t* d_patches);
+/* patches of nq query points d_q over the n_src points d_src of `index`
+ * (the first neighbour at distance 0 is dropped) */
+int dwg_mlp_patches_from(dwg_ctx* ctx, const dwg_knn_index* index, const float* d_src, int64_t n_src,
+                         const float* d_q, int64_t nq, uint16_t* d_patches);
+/* Learned initialisation of n point normals: patches over the points (over
+ * a 50,000-point hashed subsample for larger clouds), MLP normal line,
+ * oriented away from h_center (the Gauss-map hemisphere, TurboNormalInit.cu:3-44). */
+int dwg_mlp_init(dwg_ctx* ctx, const dwg_mlp* mlp, const float* d_pts, int64_t n, const float* h_center,
+                 float* d_out);
 /* n×3 outputs, unit length when normalize != 0 */
 int dwg_mlp_forward(dwg_ctx* ctx, const dwg_mlp* mlp, const uint16_t* d_patches, int64_t n, int normalize,
                     float* d_out);
-/* mean(1 − ŷ·t) over the batch and its gradient (kept in the MLP) */
+/* loss over the batch and its gradient (kept in the MLP): mode 0 the
+ * oriented loss mean(1 − ŷ·t), mode 1 the normal-line loss mean(1 − (ŷ·t)²) */
 int dwg_mlp_loss_grad(dwg_ctx* ctx, dwg_mlp* mlp, const uint16_t* d_patches, const float* d_target,
-                      int64_t batch, double* h_loss);
+                      int64_t batch, int mode, double* h_loss);
 int dwg_mlp_adam(dwg_ctx* ctx, dwg_mlp* mlp, float lr, float beta1, float beta2, float eps);
 /* gathers batch rows d_index of (patches, targets) and rotates both by the
  * row-major 3×3 h_rotation (training-time augmentation) */

@@ -53,16 +53,17 @@ def forward(theta, X):
     return y, (X, z1, h1, z2, h2)
 
 
-def loss_grad(theta, X, T):
-    """mean(1 − ŷ·t) and its gradient with respect to theta (FP64)."""
+def loss_grad(theta, X, T, line=False):
+    """mean(1 − ŷ·t), or with line=True mean(1 − (ŷ·t)²), and its gradient
+    with respect to theta (FP64)."""
     y, (X, z1, h1, z2, h2) = forward(theta, X)
     W1, b1, W2, b2, W3, b3 = unpack(theta)
     B = len(X)
     ny = np.linalg.norm(y, axis=1, keepdims=True)
     yh = y / ny
     c = (yh * T).sum(1, keepdims=True)
-    loss = float((1 - c).mean())
-    dy = -(T - c * yh) / ny / B
+    loss = float((1 - c * c).mean()) if line else float((1 - c).mean())
+    dy = -(2 * c if line else 1.0) * (T - c * yh) / ny / B
     gW3 = dy.T @ h2
     gb3 = dy.sum(0)
     dz2 = (dy @ W3) * gelu_grad(z2)
@@ -75,10 +76,11 @@ def loss_grad(theta, X, T):
     return loss, g
 
 
-def loss_only(theta, X, T):
+def loss_only(theta, X, T, line=False):
     y, _ = forward(theta, X)
     yh = y / np.linalg.norm(y, axis=1, keepdims=True)
-    return float((1 - (yh * T).sum(1)).mean())
+    c = (yh * T).sum(1)
+    return float((1 - c * c).mean()) if line else float((1 - c).mean())
 
 
 def patches(P):
@@ -105,6 +107,27 @@ def patches(P):
         inv = np.float32(1) / np.float32(r)
         off = (P[sel] - P[i]) * inv
         out[i, : 3 * len(sel)] = off.ravel()
+    return out
+
+
+def patches_from(S, Q):
+    """16-neighbour patches of the points Q over the points S (float32
+    arithmetic): 17 nearest by (d², index), the first at distance 0 dropped
+    (else the 17th), offsets over the distance to the 16th kept neighbour."""
+    S = np.ascontiguousarray(S, np.float32)
+    Q = np.ascontiguousarray(Q, np.float32)
+    idx, d2 = knn_brute(S, Q, K + 1, return_d2=True)
+    out = np.zeros((len(Q), IN), np.float32)
+    for i in range(len(Q)):
+        keep = list(range(K + 1))
+        if d2[i, 0] == 0:
+            keep.pop(0)
+        keep = [j for j in keep[:K] if idx[i, j] >= 0]
+        last = d2[i, keep[-1]] if keep else np.float32(0)
+        r = np.sqrt(np.float32(last)) if last > 0 else np.float32(1)
+        inv = np.float32(1) / np.float32(r)
+        off = (S[idx[i, keep]] - Q[i]) * inv
+        out[i, : 3 * len(keep)] = off.ravel()
     return out
 
 

@@ -1,0 +1,64 @@
+"""Exact kNN at sizes that take the tiled (LDS-staged) query path (≥ 4096
+queries), bit-exact against the oracle's brute-force ranking by
+(float d², index): k up to 32, queries outside the point bounds, a dense
+cluster that overflows one LDS stage, exact duplicates (ties), a skipped
+region large enough to send queries to the per-lane fallback, and a query
+mask.
+"""
+import numpy as np
+import pytest
+import torch
+
+import oracle as O
+from dwg_amd import gpu, shapes
+
+pytestmark = pytest.mark.gpu
+
+
+def _noisy_sphere(n, seed, sigma=0.002):
+    rng = np.random.default_rng(seed)
+    P, _ = shapes.sphere(n, seed)
+    return (0.5 * P + rng.normal(0, sigma, P.shape)).astype(np.float32)
+
+
+@pytest.mark.parametrize("k", [4, 10, 15, 17, 32])
+def test_knn_tiled_bit_exact(ctx, k):
+    P = _noisy_sphere(30_000, 20 + k)
+    rng = np.random.default_rng(k)
+    Q = np.concatenate([P[rng.integers(0, len(P), 5000)] + rng.normal(0, 0.004, (5000, 3)),
+                        rng.uniform(-0.6, 0.6, (1500, 3)),      # inside the box, off the surface
+                        rng.uniform(-1.5, 1.5, (500, 3))]).astype(np.float32)  # partly outside the box
+    idx = gpu.KnnIndex(ctx, P, k_hint=k)
+    got, d2 = idx.query(Q, k, d2=True)
+    ref, rd2 = O.knn_brute(P, Q, k, return_d2=True)
+    assert np.array_equal(got.cpu().numpy(), ref)
+    assert np.array_equal(d2.cpu().numpy(), rd2)
+
+
+def test_knn_tiled_dense_cluster_and_duplicates(ctx):
+    rng = np.random.default_rng(41)
+    P = _noisy_sphere(20_000, 40)
+    blob = (rng.normal(0, 0.002, (6000, 3)) + [0.5, 0.0, 0.0]).astype(np.float32)  # thousands per cell
+    P = np.concatenate([P, blob, P[:800]]).astype(np.float32)  # exact duplicates tie on d²
+    Q = np.concatenate([P[::3], blob[:2000] + rng.normal(0, 0.001, (2000, 3)).astype(np.float32)]).astype(np.float32)
+    idx = gpu.KnnIndex(ctx, P, k_hint=15)
+    got, d2 = idx.query(Q, 15, d2=True)
+    ref, rd2 = O.knn_brute(P, Q, 15, return_d2=True)
+    assert np.array_equal(got.cpu().numpy(), ref)
+    assert np.array_equal(d2.cpu().numpy(), rd2)
+
+
+def test_knn_tiled_skip_region_and_mask(ctx):
+    P = _noisy_sphere(25_000, 42)
+    rng = np.random.default_rng(43)
+    # a whole cap is skipped (as zero normals cluster): its queries must
+    # reach far beyond the staged tiles
+    skip = (P[:, 0] > 0.3).astype(np.uint8)
+    skip |= (rng.uniform(size=len(P)) < 0.1).astype(np.uint8)
+    qmask = (rng.uniform(size=len(P)) < 0.7).astype(np.uint8)
+    idx = gpu.KnnIndex(ctx, P, k_hint=10)
+    got = idx.query(P, 10, skip=torch.from_numpy(skip), qmask=torch.from_numpy(qmask)).cpu().numpy()
+    ref = O.knn_brute(P, P, 10, skip=skip)
+    sel = qmask == 1
+    assert np.array_equal(got[sel], ref[sel])
+    assert np.all(got[~sel] == -1)

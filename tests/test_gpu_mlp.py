@@ -154,3 +154,47 @@ def test_learned_init_runs_in_the_pipeline(ctx):
     r = pipeline.reconstruct(ctx, P, mlp=net, input_normal=4, grid_reso=64, final_reso=64, total_iter=2,
                              source_mode=1)
     assert r.tris is not None and len(r.tris) > 0
+
+
+def test_line_gradient_matches_oracle(ctx):
+    theta = _rand_theta(21)
+    net = gpu.Mlp(ctx, 0)
+    net.set_params(theta)
+    rng = np.random.default_rng(22)
+    X = M.to_bf16(rng.normal(size=(512, 48)).astype(np.float32))
+    T = rng.normal(size=(512, 3))
+    T /= np.linalg.norm(T, axis=1, keepdims=True)
+    loss = net.loss_grad(_bits(X), torch.from_numpy(T.astype(np.float32)), line=True)
+    g = net.grad().astype(np.float64)
+    theta_b = np.concatenate([M.to_bf16(a.astype(np.float32)).ravel() if a.ndim == 2 else a.ravel()
+                              for a in M.unpack(theta)])
+    ref_loss, ref_g = M.loss_grad(theta_b, X.astype(np.float64), T, line=True)
+    assert loss == pytest.approx(ref_loss, abs=1e-3)
+    cos = g @ ref_g / (np.linalg.norm(g) * np.linalg.norm(ref_g))
+    assert cos > 0.999, cos
+    assert np.linalg.norm(g - ref_g) < 0.03 * np.linalg.norm(ref_g)
+
+
+def test_patches_from_subsample_matches_oracle(ctx):
+    rng = np.random.default_rng(23)
+    P, _ = shapes.sphere(6000, 24)
+    P = (P + rng.normal(0, 0.002, P.shape)).astype(np.float32)
+    S = P[::7].copy()
+    net = gpu.Mlp(ctx, 0)
+    idx = gpu.KnnIndex(ctx, S, k_hint=17)
+    got = net.patches_from(idx, P).cpu().numpy().view(np.uint16)
+    want = M.bf16_bits(M.patches_from(S, P))
+    diff = np.abs(got.astype(np.int32) - want.astype(np.int32))
+    assert (diff > 1).sum() == 0 and (diff == 1).mean() < 1e-3
+
+
+def test_learned_init_orientation(ctx):
+    # 120k points: patches from the hashed 50k subsample; every output is a
+    # unit vector on the outer side of the centre
+    P, _ = shapes.union(120_000, 32, 25)
+    net = gpu.Mlp(ctx, 26)
+    lo, hi = P.min(0), P.max(0)
+    c = (lo + hi) / 2
+    n = net.init_normals(P, c).cpu().numpy()
+    assert np.allclose(np.linalg.norm(n, axis=1), 1.0, atol=1e-5)
+    assert np.all((n * (P - c)).sum(1) >= -1e-7)
