@@ -86,7 +86,8 @@ def run_once(ctx, P, opts, profile=False, count_terms=False, mlp=None):
 
 KERNELS = ["gwn_tree_grid_screened", "gwn_tree_screened", "octree_pack", "octree_ec_scan", "octree_ec", "mc_count",
            "mc_emit", "knn_query_centroids", "knn_query_zero_fill", "knn_query_radii", "knn_query_transfer",
-           "knn_build", "diffuse_scatter", "diffuse_normalize", "diffuse_zero_fill", "radii", "gauss_map"]
+           "knn_query_patches", "knn_build", "diffuse_nonzero_index", "diffuse_normalize", "diffuse_zero_fill", "radii",
+           "gauss_map", "mlp_init", "mlp_forward"]
 
 
 def cpu_baseline(P, grid_reso, iters, budget_s=20.0):
@@ -169,7 +170,19 @@ def main():
     ap.add_argument("--seed", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-quality", action="store_true")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run a device-to-device copy of MIB MiB (HBM counter calibration for rocprofv3 --pmc)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        import torch
+        a = torch.empty(args.copy_calibration << 18, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        a.fill_(1.0)
+        torch.cuda.synchronize()
+        b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_read": a.numel() * 4, "copy_bytes_written": b.numel() * 4}))
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU only: --gpus {args.gpus} is not supported in this build "
               "(every box has one GPU; the sharding design is described in DESIGN.md)", file=sys.stderr)

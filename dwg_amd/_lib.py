@@ -27,9 +27,9 @@ EXPORTS = [
     "dwg_ctx_create", "dwg_ctx_destroy", "dwg_ctx_error", "dwg_ctx_sync", "dwg_ctx_profile",
     "dwg_ctx_profile_get", "dwg_ctx_profile_reset", "dwg_ctx_counter_get", "dwg_gwn_direct", "dwg_octree_build", "dwg_octree_build_device",
     "dwg_octree_leaves",
-    "dwg_octree_destroy", "dwg_octree_info", "dwg_octree_update", "dwg_gwn_tree", "dwg_gwn_tree_grid",
+    "dwg_octree_destroy", "dwg_octree_info", "dwg_octree_update", "dwg_gwn_tree", "dwg_gwn_tree_self", "dwg_gwn_tree_grid",
     "dwg_gwn_tree_terms", "dwg_gwn_tree_grid_terms", "dwg_knn_build", "dwg_knn_destroy", "dwg_knn_query",
-    "dwg_marching_cubes", "dwg_diffuse_normals", "dwg_radii", "dwg_gauss_map", "dwg_transfer_normals",
+    "dwg_marching_cubes", "dwg_diffuse_normals", "dwg_radii", "dwg_point_areas", "dwg_pca_normals", "dwg_gauss_map", "dwg_transfer_normals",
     "dwg_outlier_filter", "dwg_reduce_sum", "dwg_bounds", "dwg_options_default", "dwg_launch",
     "dwg_free", "dwg_mlp_num_params", "dwg_mlp_create", "dwg_mlp_load", "dwg_mlp_save", "dwg_mlp_destroy",
     "dwg_mlp_get", "dwg_mlp_set", "dwg_mlp_patches", "dwg_mlp_patches_from", "dwg_mlp_init", "dwg_mlp_forward", "dwg_mlp_loss_grad", "dwg_mlp_adam",
@@ -99,6 +99,7 @@ def load():
     L.dwg_octree_info.argtypes = [_vp, P(_i64), P(_i64), P(_int)]
     L.dwg_octree_update.argtypes = [_vp, _vp, _vp, _vp, _vp]
     L.dwg_gwn_tree.argtypes = [_vp, _vp, _vp, _i64, _f, _f, _u32, _vp]
+    L.dwg_gwn_tree_self.argtypes = [_vp, _vp, _f, _f, _u32, _vp]
     L.dwg_gwn_tree_grid.argtypes = [_vp, _vp, P(DwgGrid), _f, _f, _u32, _vp]
     L.dwg_gwn_tree_terms.argtypes = [_vp, _vp, _vp, _i64, _f, _vp]
     L.dwg_gwn_tree_grid_terms.argtypes = [_vp, _vp, P(DwgGrid), _f, P(_i64)]
@@ -109,6 +110,8 @@ def load():
     L.dwg_marching_cubes.argtypes = [_vp, _vp, P(DwgGrid), _f, _i64, _vp, _vp, _vp, P(_i64)]
     L.dwg_diffuse_normals.argtypes = [_vp, _vp, _vp, _i64, _vp, _vp, _i64, _int, _int, _int, _vp, P(_i64)]
     L.dwg_radii.argtypes = [_vp, _vp, _vp, _i64, _int, _f, _f, _vp]
+    L.dwg_point_areas.argtypes = [_vp, _vp, _vp, _i64, _int, _vp, _vp]
+    L.dwg_pca_normals.argtypes = [_vp, _vp, _vp, _i64, _int, _vp, _vp]
     L.dwg_gauss_map.argtypes = [_vp, _vp, _i64, P(_f), _vp]
     L.dwg_transfer_normals.argtypes = [_vp, _vp, _vp, _vp, _i64, _int, _vp]
     L.dwg_outlier_filter.argtypes = [_vp, _vp, _i64, _f]

@@ -209,6 +209,15 @@ int dwg_gwn_tree(dwg_ctx* ctx, const dwg_octree* tree, const float* d_q, int64_t
     });
 }
 
+int dwg_gwn_tree_self(dwg_ctx* ctx, const dwg_octree* tree, float beta, float sigma, uint32_t flags, float* d_out) {
+    if (!ctx || !tree) return DWG_ERR_ARG;
+    return guarded(ctx, [&] {
+        DWG_REQUIRE(d_out, "null output");
+        set_device(ctx);
+        dwg::gwn_tree_self(ctx, tree, beta, sigma, flags, d_out);
+    });
+}
+
 int dwg_gwn_tree_grid(dwg_ctx* ctx, const dwg_octree* tree, const dwg_grid* grid, float beta, float sigma,
                       uint32_t flags, float* d_out) {
     if (!ctx || !tree) return DWG_ERR_ARG;
@@ -299,6 +308,26 @@ int dwg_radii(dwg_ctx* ctx, const dwg_knn_index* index, const float* d_pts, int6
         DWG_REQUIRE(k >= 1 && k <= 32, "k must be in [1, 32]");
         set_device(ctx);
         dwg::radii(ctx, index, d_pts, n, k, rmin, rmax, d_out);
+    });
+}
+
+int dwg_pca_normals(dwg_ctx* ctx, const dwg_knn_index* index, const float* d_pts, int64_t n, int k,
+                    const float* h_viewpoint, float* d_out) {
+    if (!ctx || !index) return DWG_ERR_ARG;
+    return guarded(ctx, [&] {
+        DWG_REQUIRE(n == 0 || (d_pts && d_out && h_viewpoint), "null arrays");
+        set_device(ctx);
+        if (n > 0) dwg::pca_normals(ctx, index, d_pts, n, k, h_viewpoint, d_out);
+    });
+}
+
+int dwg_point_areas(dwg_ctx* ctx, const dwg_knn_index* index, const float* d_pts, int64_t n, int k,
+                    const float* d_nrm, float* d_out) {
+    if (!ctx || !index) return DWG_ERR_ARG;
+    return guarded(ctx, [&] {
+        DWG_REQUIRE(n == 0 || (d_pts && d_out), "null arrays");
+        set_device(ctx);
+        if (n > 0) dwg::point_areas(ctx, index, d_pts, n, k, d_nrm, d_out);
     });
 }
 

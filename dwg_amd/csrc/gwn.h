@@ -13,7 +13,10 @@ struct dwg_octree {
     bool updated = false;
     std::vector<int64_t> level_start;  // nodes of depth d: [level_start[d], level_start[d+1])
     int4* info = nullptr;      // (first_child | −1, begin, end, depth)
-    float4* cm_r = nullptr;    // (float(CM), R)
+    int2* kid = nullptr;       // traversal record: (first_child, 0) internal, (−count, begin) leaf
+    float4* cm_r = nullptr;    // (float(CM), (β·R)² for the β of the last evaluation)
+    float* radius = nullptr;   // float(R)
+    float far_beta = -1.f;     // β that cm_r.w was computed for
     float4* cm_lo = nullptr;   // (CM − float(CM), 0): residual of the rounded centre
     float4* ec = nullptr;      // (EC/4π, 0), refreshed per iteration
     uint2* ec_bf16 = nullptr;  // bf16 copy of EC/4π
@@ -23,6 +26,10 @@ struct dwg_octree {
     dwg::Double3* an64 = nullptr;  // FP64 prefix sums of A·n (n+1)
     double4* cell = nullptr;       // cell centre and width (breadth-first node order)
     double4* cm64 = nullptr;       // (CM, ΣA) in FP64
+    // 8 words per node, read by the traversal for all 8 children of a node
+    // in one vector load: internal node (EC/4π, 0, CM − float(CM), 0);
+    // leaf with one point: that point's (A·n/4π, A, x, y, z, s)
+    float4* payload = nullptr;
     ~dwg_octree();
 };
 
@@ -34,6 +41,8 @@ void gwn_direct(dwg_ctx* ctx, const float* pts, const float* nrm, const float* a
                 const float* q, int64_t nq, float sigma, uint32_t flags, float* out_w, float* out_g);
 void gwn_tree(dwg_ctx* ctx, const dwg_octree* T, const float* q, int64_t nq, float beta, float sigma, uint32_t flags,
               float* out);
+// sums at the tree's own points (evaluated in tree order), out in caller order
+void gwn_tree_self(dwg_ctx* ctx, const dwg_octree* T, float beta, float sigma, uint32_t flags, float* out);
 void gwn_tree_grid(dwg_ctx* ctx, const dwg_octree* T, const dwg_grid* g, float beta, float sigma, uint32_t flags,
                    float* out);
 void gwn_tree_terms(dwg_ctx* ctx, const dwg_octree* T, const float* q, int64_t nq, float beta, int64_t* terms);

@@ -185,21 +185,31 @@ void gwn_direct(dwg_ctx* ctx, const float* pts, const float* nrm, const float* a
 // device octree
 // ---------------------------------------------------------------------------
 dwg_octree::~dwg_octree() {
-    for (void* p : {(void*)info, (void*)cm_r, (void*)cm_lo, (void*)ec, (void*)ec_bf16, (void*)sa, (void*)sb, (void*)perm,
-                    (void*)an64, (void*)cell, (void*)cm64})
+    for (void* p : {(void*)info, (void*)kid, (void*)cm_r, (void*)cm_lo, (void*)ec, (void*)ec_bf16, (void*)sa, (void*)sb, (void*)perm,
+                    (void*)an64, (void*)cell, (void*)cm64, (void*)payload, (void*)radius})
         if (p) (void)hipFree(p);
 }
 
 namespace dwg {
 
+// EC/4π per node (FP64 prefix-sum difference, one rounding), into ec and
+// the node payload: words 0-3 of an internal node; a leaf with one point
+// carries that point's (A·n/4π, A) and (x, y, z, s) instead.
 __global__ void node_ec_kernel(const int4* __restrict__ info, const Double3* __restrict__ scan, int64_t m,
-                               float4* __restrict__ ec, uint2* __restrict__ ec_bf16) {
+                               const float4* __restrict__ sa, const float4* __restrict__ sb,
+                               float4* __restrict__ ec, uint2* __restrict__ ec_bf16, float4* __restrict__ payload) {
     int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i >= m) return;
     const int4 nf = info[i];
     const Double3 s = scan[nf.z] - scan[nf.y];
     const float x = float(s.x * double(kInv4Pi)), y = float(s.y * double(kInv4Pi)), z = float(s.z * double(kInv4Pi));
     ec[i] = make_float4(x, y, z, 0.f);
+    if (nf.x >= 0) {
+        payload[2 * i] = make_float4(x, y, z, 0.f);
+    } else if (nf.z - nf.y == 1) {
+        payload[2 * i] = sb[nf.y];
+        payload[2 * i + 1] = sa[nf.y];
+    }
     if (ec_bf16) {
         // round-to-nearest-even bf16 of each component
         auto bf = [](float f) -> uint32_t {
@@ -233,8 +243,8 @@ void octree_update(dwg_ctx* ctx, dwg_octree* T, const float* nrm, const float* a
         exclusive_scan<Double3>(ctx, T->an64, n);
     }
     ProfileScope ps(ctx, "octree_ec");
-    hipLaunchKernelGGL(node_ec_kernel, dim3(grid_for(m, 256)), dim3(256), 0, ctx->stream, T->info, T->an64, m, T->ec,
-                       T->ec_bf16);
+    hipLaunchKernelGGL(node_ec_kernel, dim3(grid_for(m, 256)), dim3(256), 0, ctx->stream, T->info, T->an64, m, T->sa,
+                       T->sb, T->ec, T->ec_bf16, T->payload);
     check_launch();
     T->updated = true;
 }
@@ -248,16 +258,41 @@ constexpr int kStack = 7 * 21 + 8;
 
 struct TreeArgs {
     const int4* __restrict__ info;
+    const int2* __restrict__ kid;
     const float4* __restrict__ cm_r;
-    const float4* __restrict__ cm_lo;
-    const float4* __restrict__ ec;
-    const uint2* __restrict__ ec_bf16;
+    const float* __restrict__ payload;  // 8 words per node (see dwg_octree::payload)
     const float4* __restrict__ sa;
     const float4* __restrict__ sb;
 };
 
+// Wave-uniform reads of node records go through the scalar data cache: a
+// pointer into the constant address space makes each such load an s_load
+// (one copy into SGPRs).  Through a generic pointer the compiler emits a
+// vector load, which returns 64 copies of the same bytes through the
+// vector memory path.
+typedef int NativeInt2 __attribute__((ext_vector_type(2)));
+typedef float NativeFloat4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ int2 scalar_load(const int2* p, int64_t i) {
+    const NativeInt2 v = ((const __attribute__((address_space(4))) NativeInt2*)p)[i];
+    return make_int2(v.x, v.y);
+}
+
+__device__ __forceinline__ float4 scalar_load(const float4* p, int64_t i) {
+    const NativeFloat4 v = ((const __attribute__((address_space(4))) NativeFloat4*)p)[i];
+    return make_float4(v.x, v.y, v.z, v.w);
+}
+
+__device__ __forceinline__ float bf16_round(float f) {
+    uint32_t u = __float_as_uint(f);
+    u += 0x7FFFu + ((u >> 16) & 1u);
+    return __uint_as_float(u & 0xFFFF0000u);
+}
+
 struct QueryArgs {
-    const float* q;  // explicit points (nullptr → grid)
+    const float* q;          // explicit points
+    const int* self_perm;    // the tree's own points in tree order, results to out[perm[slot]]
+    bool grid;               // implicit grid (q and self_perm unused)
     int64_t nq;
     float ox, oy, oz, h;
     int rx, ry, rz;
@@ -265,8 +300,6 @@ struct QueryArgs {
     int64_t n_bricks;
 };
 
-__device__ __forceinline__ float bf16_lo(uint32_t u) { return __uint_as_float(u << 16); }
-__device__ __forceinline__ float bf16_hi(uint32_t u) { return __uint_as_float(u & 0xFFFF0000u); }
 
 // Block index remap so each XCD (blocks are dealt round-robin over 8 XCDs)
 // gets a contiguous run of spatially adjacent bricks / query chunks.
@@ -282,6 +315,7 @@ __global__ __launch_bounds__(kTreeThreads) void gwn_tree_kernel(TreeArgs T, Quer
                                                                  int64_t* __restrict__ terms_out) {
     __shared__ int s_node[kTreeWaves][kStack];
     __shared__ uint64_t s_mask[kTreeWaves][kStack];
+    __shared__ float4 s_pay[kTreeWaves][16];  // the 8 children's payloads of the node being opened
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int64_t blk = xcd_block(blockIdx.x, gridDim.x);
     const int64_t wid = blk * kTreeWaves + wave;
@@ -291,9 +325,17 @@ __global__ __launch_bounds__(kTreeThreads) void gwn_tree_kernel(TreeArgs T, Quer
     if (!GRID) {
         qi = wid * 64 + lane;
         valid = qi < Q.nq;
-        qx = valid ? Q.q[3 * qi] : 0.f;
-        qy = valid ? Q.q[3 * qi + 1] : 0.f;
-        qz = valid ? Q.q[3 * qi + 2] : 0.f;
+        if (Q.self_perm) {  // consecutive tree slots: spatially coherent waves
+            const float4 a = valid ? T.sa[qi] : make_float4(0.f, 0.f, 0.f, 0.f);
+            qx = a.x;
+            qy = a.y;
+            qz = a.z;
+            qi = valid ? int64_t(Q.self_perm[qi]) : 0;
+        } else {
+            qx = valid ? Q.q[3 * qi] : 0.f;
+            qy = valid ? Q.q[3 * qi + 1] : 0.f;
+            qz = valid ? Q.q[3 * qi + 2] : 0.f;
+        }
     } else {
         const int64_t br = wid;
         const int bxi = int(br % Q.bx), byi = int((br / Q.bx) % Q.by), bzi = int(br / (int64_t(Q.bx) * Q.by));
@@ -304,7 +346,7 @@ __global__ __launch_bounds__(kTreeThreads) void gwn_tree_kernel(TreeArgs T, Quer
         qy = __builtin_fmaf(float(y), Q.h, Q.oy);
         qz = __builtin_fmaf(float(z), Q.h, Q.oz);
     }
-    const uint64_t all = __ballot(valid);
+    const uint64_t all = __builtin_amdgcn_ballot_w64(valid);
     if (all == 0) return;
     const float sl2e = sigma * kLog2e;
     float sum = 0.f, comp = 0.f;
@@ -321,70 +363,79 @@ __global__ __launch_bounds__(kTreeThreads) void gwn_tree_kernel(TreeArgs T, Quer
         if (kahan) kahan_add(sum, comp, part); else sum += part;
     };
 
+    // the stack holds (first child of a node to open, lanes that open it)
     int sp = 0;
     const int4 root = T.info[0];
     if (root.x < 0) {
         leaf_sum(root.y, root.z, valid);
     } else {
         if (lane == 0) {
-            s_node[wave][0] = 0;
+            s_node[wave][0] = root.x;
             s_mask[wave][0] = all;
         }
         sp = 1;
     }
     while (sp > 0) {
         --sp;
-        const int node = uniform(s_node[wave][sp]);
+        const int fc = uniform(s_node[wave][sp]);
         const uint64_t mask = uniform64(s_mask[wave][sp]);
         const bool act = (mask >> lane) & 1;
-        const int fc = uniform(T.info[node].x);
         float part = 0.f;
-        // the 8 children are contiguous: fetch their records together
-        int4 nfs[8];
+        // the 8 children are contiguous: their records come in one batch of
+        // scalar loads, their payloads in one coalesced vector load (word
+        // 8c + w of child c in lane 8c + w), so a node costs one memory
+        // round trip however many of its children are far or leaves
+        int2 nks[8];
         float4 crs[8];
 #pragma unroll
-        for (int c = 0; c < 8; ++c) nfs[c] = T.info[fc + c];
+        for (int c = 0; c < 8; ++c) nks[c] = scalar_load(T.kid, fc + c);
 #pragma unroll
-        for (int c = 0; c < 8; ++c) crs[c] = T.cm_r[fc + c];
+        for (int c = 0; c < 8; ++c) crs[c] = scalar_load(T.cm_r, fc + c);
+        // payload word 8c + w of child c: loaded by lane 8c + w, then read
+        // back by the whole wave from LDS at a uniform address (a broadcast)
+        reinterpret_cast<float*>(s_pay[wave])[lane] = T.payload[int64_t(fc) * 8 + lane];
+        __builtin_amdgcn_wave_barrier();
 #pragma unroll
         for (int c = 0; c < 8; ++c) {
-            const int ch = fc + c;
-            const int4 nf = nfs[c];
-            if (nf.z == nf.y) continue;
-            if (nf.x < 0) {  // leaf child: summed point by point whether near or far
-                for (int j = nf.y; j < nf.z; ++j) {
-                    const float t = gwn_term<SCREENED>(qx, qy, qz, T.sa[j], T.sb[j], sigma, sl2e);
+            const int2 nk = nks[c];
+            if (nk.x == 0) continue;  // empty
+            if (nk.x < 0) {  // leaf child: summed point by point whether near or far
+                if (nk.x == -1) {
+                    const float4 b = s_pay[wave][2 * c];
+                    const float4 a = s_pay[wave][2 * c + 1];
+                    const float t = gwn_term<SCREENED>(qx, qy, qz, a, b, sigma, sl2e);
                     part += act ? t : 0.f;
+                } else {
+                    for (int j = nk.y; j < nk.y - nk.x; ++j) {
+                        const float t = gwn_term<SCREENED>(qx, qy, qz, scalar_load(T.sa, j), scalar_load(T.sb, j), sigma, sl2e);
+                        part += act ? t : 0.f;
+                    }
                 }
-                if (COUNT && act) nterms += nf.z - nf.y;
+                if (COUNT && act) nterms += -nk.x;
                 continue;
             }
+            // cm_r.w holds (β·R)², rounded as float(float(β·R)²), for this β
             const float4 cr = crs[c];
             const float dx = cr.x - qx, dy = cr.y - qy, dz = cr.z - qz;
-            const float br = beta * cr.w;
-            const bool far = dist2(dx, dy, dz) > br * br;
-            const bool near = act && !far;
-            const uint64_t nm = __ballot(near);
-            if (__ballot(act && far)) {
-                float4 e4;
-                if (BF16) {
-                    const uint2 u = T.ec_bf16[ch];
-                    e4 = make_float4(bf16_lo(u.x), bf16_hi(u.x), bf16_lo(u.y), 0.f);
-                } else {
-                    e4 = T.ec[ch];
-                }
+            const bool far = dist2(dx, dy, dz) > cr.w;
+            // lane masks straight from the compare: far for the lanes that
+            // opened this node, near ones open the child
+            const uint64_t farm = __builtin_amdgcn_ballot_w64(far);
+            const uint64_t nm = mask & ~farm;
+            if (mask & farm) {
+                float4 e4 = s_pay[wave][2 * c];
+                if (BF16) e4 = make_float4(bf16_round(e4.x), bf16_round(e4.y), bf16_round(e4.z), 0.f);
+                const float4 lo = s_pay[wave][2 * c + 1];
                 // d = (CM_f − q) + δ: the first difference is exact for nearby
                 // values, so d carries a single rounding
-                const float4 lo = T.cm_lo[ch];
-                const float t = gwn_term_d<SCREENED>((cr.x - qx) + lo.x, (cr.y - qy) + lo.y, (cr.z - qz) + lo.z,
-                                                     kScreenDefault, e4, sigma, sl2e);
+                const float t = gwn_term_d<SCREENED>(dx + lo.x, dy + lo.y, dz + lo.z, kScreenDefault, e4, sigma, sl2e);
                 // the plain dipole has no r > 1e-10 guard in the reference either
                 part += (act && far) ? t : 0.f;
                 if (COUNT && act && far) nterms += 1;
             }
             if (nm) {
                 if (lane == 0) {
-                    s_node[wave][sp] = ch;
+                    s_node[wave][sp] = nk.x;
                     s_mask[wave][sp] = nm;
                 }
                 ++sp;
@@ -405,9 +456,16 @@ static QueryArgs make_points(const float* q, int64_t nq) {
     return Q;
 }
 
+static QueryArgs make_self(const dwg_octree* T) {
+    QueryArgs Q{};
+    Q.self_perm = T->perm;
+    Q.nq = T->n_points;
+    return Q;
+}
+
 static QueryArgs make_grid(const dwg_grid* g) {
     QueryArgs Q{};
-    Q.q = nullptr;
+    Q.grid = true;
     Q.ox = g->origin[0];
     Q.oy = g->origin[1];
     Q.oz = g->origin[2];
@@ -423,23 +481,39 @@ static QueryArgs make_grid(const dwg_grid* g) {
     return Q;
 }
 
-static void launch_tree(dwg_ctx* ctx, const dwg_octree* T, const QueryArgs& Q, float beta, float sigma, uint32_t flags,
-                        float* out, int64_t* terms, bool count) {
-    DWG_REQUIRE(T && T->updated, "octree has no moments yet: call dwg_octree_update first");
-    DWG_REQUIRE(T->max_level <= 21, "octree deeper than the traversal stack");
-    const int64_t waves = Q.q ? (Q.nq + 63) / 64 : Q.n_bricks;
+// cm_r.w = (β·R)² for the traversal's far test d² > (β·R)², with the
+// rounding sequence float(β·R) then its square (the order the oracle uses)
+__global__ void far_radius_kernel(const float* __restrict__ r, int64_t m, float beta, float4* __restrict__ cm_r) {
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= m) return;
+    const float br = beta * r[i];
+    cm_r[i].w = br * br;
+}
+
+static void launch_tree(dwg_ctx* ctx, const dwg_octree* Tc, const QueryArgs& Q, float beta, float sigma,
+                        uint32_t flags, float* out, int64_t* terms, bool count) {
+    DWG_REQUIRE(Tc && Tc->updated, "octree has no moments yet: call dwg_octree_update first");
+    DWG_REQUIRE(Tc->max_level <= 21, "octree deeper than the traversal stack");
+    auto* T = const_cast<dwg_octree*>(Tc);  // the far radii follow β; the tree itself is unchanged
+    if (!(T->far_beta == beta)) {
+        hipLaunchKernelGGL(far_radius_kernel, dim3(grid_for(T->n_nodes, 256)), dim3(256), 0, ctx->stream, T->radius,
+                           T->n_nodes, beta, T->cm_r);
+        check_launch();
+        T->far_beta = beta;
+    }
+    const int64_t waves = Q.grid ? Q.n_bricks : (Q.nq + 63) / 64;
     if (waves == 0) return;
-    TreeArgs A{T->info, T->cm_r, T->cm_lo, T->ec, T->ec_bf16, T->sa, T->sb};
+    TreeArgs A{T->info, T->kid, T->cm_r, reinterpret_cast<const float*>(T->payload), T->sa, T->sb};
     dim3 grid(grid_for(waves, kTreeWaves)), block(kTreeThreads);
     const bool scr = flags & DWG_SCREENED, bf = flags & DWG_FAR_BF16;
-    const char* name = count ? "gwn_tree_terms" : (Q.q ? (scr ? "gwn_tree_screened" : "gwn_tree_plain")
-                                                       : (scr ? "gwn_tree_grid_screened" : "gwn_tree_grid_plain"));
+    const char* name = count ? "gwn_tree_terms" : (!Q.grid ? (scr ? "gwn_tree_screened" : "gwn_tree_plain")
+                                                           : (scr ? "gwn_tree_grid_screened" : "gwn_tree_grid_plain"));
     ProfileScope ps(ctx, name);
 #define DWG_TREE_LAUNCH(S, C, B, G) \
     hipLaunchKernelGGL((gwn_tree_kernel<S, C, B, G>), grid, block, 0, ctx->stream, A, Q, beta, sigma, flags, out, terms)
 #define DWG_TREE_LAUNCH_Q(S, C, B)       \
     do {                                 \
-        if (Q.q)                         \
+        if (!Q.grid)                     \
             DWG_TREE_LAUNCH(S, C, B, false); \
         else                             \
             DWG_TREE_LAUNCH(S, C, B, true);  \
@@ -458,6 +532,11 @@ void gwn_tree(dwg_ctx* ctx, const dwg_octree* T, const float* q, int64_t nq, flo
               float* out) {
     DWG_REQUIRE(nq >= 0 && (q || nq == 0), "bad query array");
     launch_tree(ctx, T, make_points(q, nq), beta, sigma, flags, out, nullptr, false);
+}
+
+void gwn_tree_self(dwg_ctx* ctx, const dwg_octree* T, float beta, float sigma, uint32_t flags, float* out) {
+    DWG_REQUIRE(T, "null octree");
+    launch_tree(ctx, T, make_self(T), beta, sigma, flags, out, nullptr, false);
 }
 
 void gwn_tree_grid(dwg_ctx* ctx, const dwg_octree* T, const dwg_grid* g, float beta, float sigma, uint32_t flags,

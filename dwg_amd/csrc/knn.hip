@@ -240,27 +240,60 @@ dwg_knn_index* knn_build(dwg_ctx* ctx, const float* p, int64_t n, int k_hint) {
     return I;
 }
 
-template <int K>
-__device__ __forceinline__ void knn_insert(uint64_t (&best)[K], int k, uint64_t key) {
-    uint64_t v = key;
-#pragma unroll
-    for (int i = 0; i < K; ++i) {
-        if (i < k) {
-            const uint64_t b = best[i];
-            const bool sw = v < b;
-            best[i] = sw ? v : b;
-            v = sw ? b : v;
+dwg_knn_index* knn_build_like(dwg_ctx* ctx, const dwg_knn_index* like, const float* p, int64_t n) {
+    DWG_REQUIRE(like && n > 0 && n < (int64_t(1) << 31) - 1, "bad kNN index request");
+    auto* I = new dwg_knn_index();
+    try {
+        I->n = n;
+        ProfileScope ps(ctx, "knn_build");
+        for (int l = 0; l < 2; ++l) {
+            I->lvl[l].G = like->lvl[l].G;
+            bucket(ctx, p, n, I->lvl[l], true);
         }
+        empty_distance(ctx, I->lvl[1]);
+    } catch (...) {
+        delete I;
+        throw;
+    }
+    return I;
+}
+
+// The k best keys of a query live in registers as a descending list:
+// best[0] is the k-th smallest key so far (the insertion threshold), the
+// slots k..K−1 hold 0 and never move (every key is ≥ 0).  Every access uses
+// a constant index, so the list is never demoted to scratch memory.
+template <int K>
+__device__ __forceinline__ void knn_reset(uint64_t (&best)[K], int k) {
+#pragma unroll
+    for (int j = 0; j < K; ++j) best[j] = j < k ? ~0ull : 0ull;
+}
+
+// key < best[0]: it replaces the largest key and sinks to its place
+template <int K>
+__device__ __forceinline__ void knn_insert(uint64_t (&best)[K], uint64_t key) {
+    best[0] = key;
+#pragma unroll
+    for (int i = 0; i < K - 1; ++i) {
+        const uint64_t a = best[i], b = best[i + 1];
+        const bool sw = a < b;
+        best[i] = sw ? b : a;
+        best[i + 1] = sw ? a : b;
     }
 }
 
+// writes the k keys in ascending order (index −1 / d² ∞ where fewer were found)
 template <int K>
-__device__ __forceinline__ uint64_t kth_of(const uint64_t (&best)[K], int k) {
-    uint64_t v = best[0];
+__device__ __forceinline__ void knn_store(const uint64_t (&best)[K], int k, int64_t row, int32_t* __restrict__ out_idx,
+                                          float* __restrict__ out_d2) {
 #pragma unroll
-    for (int j = 1; j < K; ++j)
-        if (j == k - 1) v = best[j];
-    return v;
+    for (int j = 0; j < K; ++j) {
+        if (j < k) {
+            const uint64_t b = best[j];
+            const int64_t o = row * k + (k - 1 - j);
+            out_idx[o] = b == ~0ull ? -1 : int32_t(uint32_t(b));
+            if (out_d2) out_d2[o] = b == ~0ull ? INFINITY : __uint_as_float(uint32_t(b >> 32));
+        }
+    }
 }
 
 // Visits shells r = 0..rmax of cells around q on one grid level, keeping
@@ -271,9 +304,7 @@ __device__ bool search_level(const KnnGridArgs& G, const int64_t* __restrict__ c
                              const float4* __restrict__ pts, float qx, float qy, float qz, int k,
                              const uint8_t* __restrict__ skip, uint64_t (&best)[K], int rmax,
                              const int* __restrict__ empty) {
-    uint64_t kth = ~0ull;
-#pragma unroll
-    for (int j = 0; j < K; ++j) best[j] = ~0ull;
+    knn_reset<K>(best, k);
     const int c0x = cell_coord(qx, G.lo[0], G.inv_cell, G.dim[0]);
     const int c0y = cell_coord(qy, G.lo[1], G.inv_cell, G.dim[1]);
     const int c0z = cell_coord(qz, G.lo[2], G.inv_cell, G.dim[2]);
@@ -300,14 +331,12 @@ __device__ bool search_level(const KnnGridArgs& G, const int64_t* __restrict__ c
                         if (skip && skip[idx]) continue;
                         const float d2 = dist2(qx - p.x, qy - p.y, qz - p.z);
                         const uint64_t key = (uint64_t(__float_as_uint(d2)) << 32) | uint32_t(idx);
-                        if (key < kth) {
-                            knn_insert<K>(best, k, key);
-                            kth = kth_of<K>(best, k);
-                        }
+                        if (key < best[0]) knn_insert<K>(best, key);
                     }
                 }
             }
         }
+        const uint64_t kth = best[0];
         // distance from q to the nearest face of the searched block that is
         // not on the grid boundary; no unvisited point can be closer
         float dmin = INFINITY;
@@ -353,12 +382,7 @@ __global__ __launch_bounds__(256) void knn_query_kernel(KnnGridArgs G0, const in
         const int maxr = max(G1.dim[0], max(G1.dim[1], G1.dim[2]));
         (void)search_level<K>(G1, cs1, p1, qx, qy, qz, k, skip, best, maxr, empty1);
     }
-    for (int j = 0; j < K; ++j) {
-        if (j >= k) break;
-        const uint64_t b = best[j];
-        out_idx[i * k + j] = b == ~0ull ? -1 : int32_t(uint32_t(b));
-        if (out_d2) out_d2[i * k + j] = b == ~0ull ? INFINITY : __uint_as_float(uint32_t(b >> 32));
-    }
+    knn_store<K>(best, k, i, out_idx, out_d2);
 }
 
 // ---------------------------------------------------------------------------
@@ -371,12 +395,15 @@ __global__ __launch_bounds__(256) void knn_query_kernel(KnnGridArgs G0, const in
 // regions) go to the per-lane shell search above.  Both paths keep the k
 // smallest (d², index) keys, so the result does not depend on the path.
 // ---------------------------------------------------------------------------
+constexpr double kScatterFix = 1099511627776.0;  // 2^40
 constexpr int kTile = 4;
-constexpr int kHalo = 2;
-constexpr int kRegion = kTile + 2 * kHalo;
-constexpr int kRows = kRegion * kRegion;
 constexpr int kStage = 2048;
 constexpr int kTileThreads = 256;
+// halos of the tiled passes: most queries close within 2 cells; the ones
+// that do not (off-surface centroids) get passes with 5 and 9 cells
+constexpr int kHalo1 = 2;
+constexpr int kHalo2 = 5;
+constexpr int kHalo3 = 9;
 
 struct TileArgs {
     int tdim[3];  // tiles per axis
@@ -389,27 +416,29 @@ __device__ __forceinline__ int tile_of_point(const KnnGridArgs& G, const TileArg
     return cx + T.tdim[0] * (cy + T.tdim[1] * cz);
 }
 
-__global__ void tile_count_kernel(const float* __restrict__ q, int64_t nq, const uint8_t* __restrict__ qmask,
-                                  KnnGridArgs G, TileArgs T, int* __restrict__ tile_of,
-                                  unsigned long long* __restrict__ count) {
-    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (i >= nq) return;
+// query t of the pass is qlist[t] (or t); masked queries are left out
+__global__ void tile_count_kernel(const float* __restrict__ q, int64_t nq, const int* __restrict__ qlist,
+                                  const uint8_t* __restrict__ qmask, KnnGridArgs G, TileArgs T,
+                                  int* __restrict__ tile_of, unsigned long long* __restrict__ count) {
+    const int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (t >= nq) return;
+    const int64_t i = qlist ? int64_t(qlist[t]) : t;
     if (qmask && !qmask[i]) {
-        tile_of[i] = -1;
+        tile_of[t] = -1;
         return;
     }
-    const int t = tile_of_point(G, T, q[3 * i], q[3 * i + 1], q[3 * i + 2]);
-    tile_of[i] = t;
-    atomicAdd(&count[t], 1ull);
+    const int tl = tile_of_point(G, T, q[3 * i], q[3 * i + 1], q[3 * i + 2]);
+    tile_of[t] = tl;
+    atomicAdd(&count[tl], 1ull);
 }
 
-__global__ void tile_scatter_kernel(const int* __restrict__ tile_of, int64_t nq, unsigned long long* __restrict__ cursor,
-                                    int* __restrict__ order) {
-    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (i >= nq) return;
-    const int t = tile_of[i];
-    if (t < 0) return;
-    order[atomicAdd(&cursor[t], 1ull)] = int(i);
+__global__ void tile_scatter_kernel(const int* __restrict__ tile_of, int64_t nq, const int* __restrict__ qlist,
+                                    unsigned long long* __restrict__ cursor, int* __restrict__ order) {
+    const int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (t >= nq) return;
+    const int tl = tile_of[t];
+    if (tl < 0) return;
+    order[atomicAdd(&cursor[tl], 1ull)] = qlist ? qlist[t] : int(t);
 }
 
 __global__ void tile_flags_kernel(const int64_t* __restrict__ start, int64_t ntiles, int64_t* __restrict__ flags) {
@@ -424,33 +453,80 @@ __global__ void tile_list_kernel(const int64_t* __restrict__ flags, int64_t ntil
     list[flags[t]] = int(t);
 }
 
-template <int K>
+// Staged candidates are ordered by ring: the rows of cells that cross the
+// tile first, then the halo rings outwards, so a query meets its nearest
+// points early and its k-th key (the insertion threshold) tightens fast.
+// SCATTER: besides its neighbours, every closed query adds its value
+// (3 floats) in 2^-40 fixed point into acc[neighbour] — the normal scatter
+// of the diffusion (TurboDWG.cu:179-204).  The tile gathers these sums in an
+// LDS hash table keyed by point index and flushes each key once with 64-bit
+// global atomics; integer sums are associative, so the result equals
+// per-pair atomics bit for bit.
+template <int K, int HALO, bool SCATTER>
 __global__ __launch_bounds__(kTileThreads) void knn_tile_kernel(
     KnnGridArgs G, TileArgs T, const int64_t* __restrict__ cell_start, const float4* __restrict__ pts,
     const int* __restrict__ tiles, const int64_t* __restrict__ tile_start, const int* __restrict__ order,
     const float* __restrict__ q, int k, const uint8_t* __restrict__ skip, int32_t* __restrict__ out_idx,
-    float* __restrict__ out_d2, int* __restrict__ redo, unsigned int* __restrict__ n_redo) {
+    float* __restrict__ out_d2, int* __restrict__ redo, unsigned int* __restrict__ n_redo, KnnScatter sc) {
+    constexpr int kRegion = kTile + 2 * HALO;
+    constexpr int kRows = kRegion * kRegion;
+    constexpr int kHash = SCATTER ? 512 : 1;
+    __shared__ int s_hkey[kHash];
+    __shared__ unsigned long long s_hacc[kHash][3];
+    if (SCATTER) {
+        for (int h = threadIdx.x; h < kHash; h += kTileThreads) {
+            s_hkey[h] = -1;
+            s_hacc[h][0] = s_hacc[h][1] = s_hacc[h][2] = 0ull;
+        }
+    }
     __shared__ float4 s_pts[kStage];
     __shared__ int64_t s_rb[kRows];
+    __shared__ int s_cnt[kRows];
     __shared__ int s_pref[kRows + 1];
+    __shared__ int s_ring[HALO + 2];
     const int tile = tiles[blockIdx.x];
     const int tx = tile % T.tdim[0], ty = (tile / T.tdim[0]) % T.tdim[1], tz = tile / (T.tdim[0] * T.tdim[1]);
-    const int x0 = max(tx * kTile - kHalo, 0), x1 = min(tx * kTile + kTile - 1 + kHalo, G.dim[0] - 1);
-    const int y0 = max(ty * kTile - kHalo, 0), y1 = min(ty * kTile + kTile - 1 + kHalo, G.dim[1] - 1);
-    const int z0 = max(tz * kTile - kHalo, 0), z1 = min(tz * kTile + kTile - 1 + kHalo, G.dim[2] - 1);
+    const int x0 = max(tx * kTile - HALO, 0), x1 = min(tx * kTile + kTile - 1 + HALO, G.dim[0] - 1);
+    const int y0 = max(ty * kTile - HALO, 0), y1 = min(ty * kTile + kTile - 1 + HALO, G.dim[1] - 1);
+    const int z0 = max(tz * kTile - HALO, 0), z1 = min(tz * kTile + kTile - 1 + HALO, G.dim[2] - 1);
     const int ny = y1 - y0 + 1, nrows = ny * (z1 - z0 + 1);
-    // each (y, z) row of cells x0..x1 is one contiguous range of points
-    if (threadIdx.x < nrows) {
-        const int y = y0 + threadIdx.x % ny, z = z0 + threadIdx.x / ny;
-        const int64_t c = x0 + int64_t(G.dim[0]) * (y + int64_t(G.dim[1]) * z);
-        const int64_t b = cell_start[c], e = cell_start[c + (x1 - x0) + 1];
-        s_rb[threadIdx.x] = b;
-        s_pref[threadIdx.x + 1] = int(e - b);
+    if (threadIdx.x < HALO + 2) s_ring[threadIdx.x] = 0;
+    __syncthreads();
+    // each (y, z) row of cells x0..x1 is one contiguous range of points;
+    // a thread takes rows r0 = threadIdx.x and r0 + 256 (kRows ≤ 512)
+    static_assert(kRows <= 2 * kTileThreads, "at most two rows per thread");
+    int ring[2] = {0, 0}, rank[2] = {0, 0}, rc[2] = {0, 0};
+    int64_t rb[2] = {0, 0};
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        const int r = threadIdx.x + h * kTileThreads;
+        if (r < nrows) {
+            const int y = y0 + r % ny, z = z0 + r / ny;
+            const int64_t c = x0 + int64_t(G.dim[0]) * (y + int64_t(G.dim[1]) * z);
+            rb[h] = cell_start[c];
+            rc[h] = int(cell_start[c + (x1 - x0) + 1] - rb[h]);
+            const int dy = max(max(ty * kTile - y, y - (ty * kTile + kTile - 1)), 0);
+            const int dz = max(max(tz * kTile - z, z - (tz * kTile + kTile - 1)), 0);
+            ring[h] = max(dy, dz);
+            rank[h] = atomicAdd(&s_ring[ring[h] + 1], 1);
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0)
+        for (int r = 1; r <= HALO + 1; ++r) s_ring[r] += s_ring[r - 1];
+    __syncthreads();
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        if (threadIdx.x + h * kTileThreads < nrows) {
+            const int pos = s_ring[ring[h]] + rank[h];
+            s_rb[pos] = rb[h];
+            s_cnt[pos] = rc[h];
+        }
     }
     __syncthreads();
     if (threadIdx.x == 0) {
         s_pref[0] = 0;
-        for (int r = 0; r < nrows; ++r) s_pref[r + 1] += s_pref[r];
+        for (int r = 0; r < nrows; ++r) s_pref[r + 1] = s_pref[r] + s_cnt[r];
     }
     __syncthreads();
     const int total = s_pref[nrows];
@@ -469,9 +545,8 @@ __global__ __launch_bounds__(kTileThreads) void knn_tile_kernel(
         const float qx = active ? q[3 * int64_t(qi)] : 0.f, qy = active ? q[3 * int64_t(qi) + 1] : 0.f,
                     qz = active ? q[3 * int64_t(qi) + 2] : 0.f;
         uint64_t best[K];
-#pragma unroll
-        for (int j = 0; j < K; ++j) best[j] = ~0ull;
-        uint64_t kth = ~0ull;
+        knn_reset<K>(best, k);
+        float kd2 = INFINITY;  // float d² of the k-th key: a cheap first test
         for (int cb = 0; cb < total; cb += kStage) {
             const int cnt = min(kStage, total - cb);
             __syncthreads();
@@ -491,13 +566,14 @@ __global__ __launch_bounds__(kTileThreads) void knn_tile_kernel(
             if (active) {
                 for (int j = 0; j < cnt; ++j) {
                     const float4 p = s_pts[j];
+                    const float d2 = dist2(qx - p.x, qy - p.y, qz - p.z);
+                    if (!(d2 <= kd2)) continue;
                     const int idx = __float_as_int(p.w);
                     if (idx < 0) continue;
-                    const float d2 = dist2(qx - p.x, qy - p.y, qz - p.z);
                     const uint64_t key = (uint64_t(__float_as_uint(d2)) << 32) | uint32_t(idx);
-                    if (key < kth) {
-                        knn_insert<K>(best, k, key);
-                        kth = kth_of<K>(best, k);
+                    if (key < best[0]) {
+                        knn_insert<K>(best, key);
+                        kd2 = best[0] == ~0ull ? INFINITY : __uint_as_float(uint32_t(best[0] >> 32));
                     }
                 }
             }
@@ -505,8 +581,7 @@ __global__ __launch_bounds__(kTileThreads) void knn_tile_kernel(
         if (!active) continue;
         const float dmin = fminf(fminf(fminf(qx - flx, fhx - qx), fminf(qy - fly, fhy - qy)), fminf(qz - flz, fhz - qz));
         bool exact = dmin == INFINITY;
-        if (!exact && kth != ~0ull) {
-            const float kd2 = __uint_as_float(uint32_t(kth >> 32));
+        if (!exact && best[0] != ~0ull) {
             const float safe = dmin * (1.0f - 1e-5f) - 1e-6f * G.cell;
             exact = safe > 0.f && kd2 < safe * safe;
         }
@@ -514,13 +589,220 @@ __global__ __launch_bounds__(kTileThreads) void knn_tile_kernel(
             redo[atomicAdd(n_redo, 1u)] = qi;
             continue;
         }
-        for (int j = 0; j < K; ++j) {
-            if (j >= k) break;
-            const uint64_t b = best[j];
-            out_idx[int64_t(qi) * k + j] = b == ~0ull ? -1 : int32_t(uint32_t(b));
-            if (out_d2) out_d2[int64_t(qi) * k + j] = b == ~0ull ? INFINITY : __uint_as_float(uint32_t(b >> 32));
+        knn_store<K>(best, k, qi, out_idx, out_d2);
+        if (SCATTER) {
+            long long v[3];
+#pragma unroll
+            for (int d = 0; d < 3; ++d) v[d] = __double2ll_rn(double(sc.val[3 * int64_t(qi) + d]) * kScatterFix);
+#pragma unroll
+            for (int j = 0; j < K; ++j) {
+                if (j >= k || best[j] == ~0ull) continue;
+                const int idx = int(uint32_t(best[j]));
+                int h = int((uint32_t(idx) * 0x9E3779B1u) >> 23);  // 9 bits
+                bool done = false;
+                for (int probe = 0; probe < 8 && !done; ++probe) {
+                    const int prev = atomicCAS(&s_hkey[h], -1, idx);
+                    if (prev == -1 || prev == idx) {
+#pragma unroll
+                        for (int d = 0; d < 3; ++d) atomicAdd(&s_hacc[h][d], (unsigned long long)v[d]);
+                        done = true;
+                    }
+                    h = (h + 1) & (kHash - 1);
+                }
+                if (!done)
+#pragma unroll
+                    for (int d = 0; d < 3; ++d) atomicAdd(&sc.acc[3 * int64_t(idx) + d], (unsigned long long)v[d]);
+            }
         }
     }
+    if (SCATTER) {
+        __syncthreads();
+        for (int h = threadIdx.x; h < kHash; h += kTileThreads) {
+            const int key = s_hkey[h];
+            if (key >= 0)
+#pragma unroll
+                for (int d = 0; d < 3; ++d) atomicAdd(&sc.acc[3 * int64_t(key) + d], s_hacc[h][d]);
+        }
+    }
+}
+
+// acc[neighbour] += value of the query, for the queries of qlist (or all)
+__global__ void scatter_pairs_kernel(const int32_t* __restrict__ knn, int k, const int* __restrict__ qlist, int64_t nq,
+                                     KnnScatter sc) {
+    const int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (t >= nq * k) return;
+    const int64_t qi = qlist ? int64_t(qlist[t / k]) : t / k;
+    const int32_t s = knn[qi * k + t % k];
+    if (s < 0) return;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        const long long v = __double2ll_rn(double(sc.val[3 * qi + d]) * kScatterFix);
+        atomicAdd(&sc.acc[3 * int64_t(s) + d], (unsigned long long)v);
+    }
+}
+
+__device__ __forceinline__ uint64_t shfl64(uint64_t v, int src) {
+    const uint32_t lo = uint32_t(__shfl(int(uint32_t(v)), src, 64));
+    const uint32_t hi = uint32_t(__shfl(int(uint32_t(v >> 32)), src, 64));
+    return (uint64_t(hi) << 32) | lo;
+}
+
+__device__ __forceinline__ uint64_t wave_min_u64(uint64_t v) {
+    for (int off = 32; off > 0; off >>= 1) {
+        const uint64_t o = shfl64(v, (threadIdx.x & 63) ^ off);
+        v = o < v ? o : v;
+    }
+    return v;
+}
+
+// One wave per query, for the queries the tiled passes could not close (far
+// from the points, or inside a cluster of skipped points).  On the coarse
+// level the cells of each Chebyshev shell are read 64 at a time, their
+// points flattened and scanned 64 per step (one candidate per lane), each
+// lane keeping its own k best keys.  The smallest lane-local k-th key bounds
+// the wave's k-th key from above, so the search stops once it is closer than
+// every cell not yet visited; k rounds of wave minima then merge the lists
+// into the exact k smallest (d², index) keys.
+// Cell offset of entry id of the Chebyshev shell of radius r, which has
+// 24r² + 2 cells (one for r = 0): the two z faces, then the two y faces
+// without their z rims, then the two x faces without either rim.
+__device__ __forceinline__ int3 shell_cell(int id, int r) {
+    if (r == 0) return make_int3(0, 0, 0);
+    const int s = 2 * r + 1, t = 2 * r - 1, nz = s * s, ny = s * t;
+    if (id < 2 * nz) {
+        const int f = id / nz, rem = id - f * nz;
+        return make_int3(rem % s - r, rem / s - r, f ? r : -r);
+    }
+    id -= 2 * nz;
+    if (id < 2 * ny) {
+        const int f = id / ny, rem = id - f * ny;
+        return make_int3(rem % s - r, f ? r : -r, rem / s - (r - 1));
+    }
+    id -= 2 * ny;
+    const int tt = t * t, f = id / tt, rem = id - f * tt;
+    return make_int3(f ? r : -r, rem % t - (r - 1), rem / t - (r - 1));
+}
+
+template <int K>
+__global__ __launch_bounds__(256) void knn_wave_kernel(KnnGridArgs G, const int64_t* __restrict__ cell_start,
+                                                       const float4* __restrict__ pts, const int* __restrict__ empty,
+                                                       const float* __restrict__ q, int64_t nq,
+                                                       const int* __restrict__ qlist, int k,
+                                                       const uint8_t* __restrict__ skip, int32_t* __restrict__ out_idx,
+                                                       float* __restrict__ out_d2) {
+    const int lane = threadIdx.x & 63;
+    const int64_t t = int64_t(blockIdx.x) * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    if (t >= nq) return;  // uniform per wave
+    const int64_t i = qlist ? int64_t(qlist[t]) : t;
+    const float qx = q[3 * i], qy = q[3 * i + 1], qz = q[3 * i + 2];
+    uint64_t best[K];
+    knn_reset<K>(best, k);
+    const int c0x = cell_coord(qx, G.lo[0], G.inv_cell, G.dim[0]);
+    const int c0y = cell_coord(qy, G.lo[1], G.inv_cell, G.dim[1]);
+    const int c0z = cell_coord(qz, G.lo[2], G.inv_cell, G.dim[2]);
+    const int maxr = max(G.dim[0], max(G.dim[1], G.dim[2]));
+    int r = empty[c0x + int64_t(G.dim[0]) * (c0y + int64_t(G.dim[1]) * c0z)];
+    for (; r <= maxr; ++r) {
+        // only the shell's own cells: O(r²) per shell
+        const int shell = r == 0 ? 1 : 24 * r * r + 2;
+        for (int base = 0; base < shell; base += 64) {
+            const int id = base + lane;
+            int64_t b = 0;
+            int cnt = 0;
+            if (id < shell) {
+                const int3 o = shell_cell(id, r);
+                const int x = c0x + o.x, y = c0y + o.y, z = c0z + o.z;
+                if (x >= 0 && y >= 0 && z >= 0 && x < G.dim[0] && y < G.dim[1] && z < G.dim[2]) {
+                    const int64_t c = x + int64_t(G.dim[0]) * (y + int64_t(G.dim[1]) * z);
+                    b = cell_start[c];
+                    cnt = int(cell_start[c + 1] - b);
+                }
+            }
+            // inclusive scan of the 64 cell sizes
+            int incl = cnt;
+            for (int off = 1; off < 64; off <<= 1) {
+                const int v = __shfl_up(incl, off, 64);
+                if (lane >= off) incl += v;
+            }
+            const int total = __shfl(incl, 63, 64);
+            for (int cb = 0; cb < total; cb += 64) {
+                const int j = cb + lane;
+                // owner lane: the number of lanes whose inclusive count is ≤ j
+                // (six uniform steps, every lane active for the shuffles)
+                int lo = 0;
+                for (int step = 32; step > 0; step >>= 1)
+                    if (__shfl(incl, lo + step - 1, 64) <= j) lo += step;
+                lo = min(lo, 63);
+                const int64_t ob = shfl64(uint64_t(b), lo);
+                const int oexcl = __shfl(incl - cnt, lo, 64);
+                if (j < total) {
+                    const float4 p = pts[ob + (j - oexcl)];
+                    const int idx = __float_as_int(p.w);
+                    if (!(skip && skip[idx])) {
+                        const float d2 = dist2(qx - p.x, qy - p.y, qz - p.z);
+                        const uint64_t key = (uint64_t(__float_as_uint(d2)) << 32) | uint32_t(idx);
+                        if (key < best[0]) knn_insert<K>(best, key);
+                    }
+                }
+            }
+        }
+        // no unvisited point is closer than the nearest face of the searched
+        // block that is not on the grid boundary
+        float dmin = INFINITY;
+        if (c0x - r > 0) dmin = fminf(dmin, qx - (G.lo[0] + float(c0x - r) * G.cell));
+        if (c0x + r + 1 < G.dim[0]) dmin = fminf(dmin, (G.lo[0] + float(c0x + r + 1) * G.cell) - qx);
+        if (c0y - r > 0) dmin = fminf(dmin, qy - (G.lo[1] + float(c0y - r) * G.cell));
+        if (c0y + r + 1 < G.dim[1]) dmin = fminf(dmin, (G.lo[1] + float(c0y + r + 1) * G.cell) - qy);
+        if (c0z - r > 0) dmin = fminf(dmin, qz - (G.lo[2] + float(c0z - r) * G.cell));
+        if (c0z + r + 1 < G.dim[2]) dmin = fminf(dmin, (G.lo[2] + float(c0z + r + 1) * G.cell) - qz);
+        if (dmin == INFINITY) break;
+        const uint64_t bound = wave_min_u64(best[0]);
+        if (bound != ~0ull) {
+            const float kd2 = __uint_as_float(uint32_t(bound >> 32));
+            const float safe = dmin * (1.0f - 1e-5f) - 1e-6f * G.cell;
+            if (safe > 0.f && kd2 < safe * safe) break;
+        }
+    }
+    // merge: reverse each list to ascending order, drop the K−k zero slots
+    // from its front, then the j-th smallest key is the wave minimum of the
+    // lanes' heads (keys are distinct: one lane pops)
+    uint64_t asc[K];
+#pragma unroll
+    for (int j = 0; j < K; ++j) asc[j] = best[K - 1 - j];
+    for (int s = 0; s < K - k; ++s) {
+#pragma unroll
+        for (int c = 0; c < K - 1; ++c) asc[c] = asc[c + 1];
+        asc[K - 1] = ~0ull;
+    }
+    for (int j = 0; j < k; ++j) {
+        const uint64_t m = wave_min_u64(asc[0]);
+        const uint64_t win = __ballot(asc[0] == m && m != ~0ull);
+        if (win && lane == __ffsll((unsigned long long)win) - 1) {
+#pragma unroll
+            for (int c = 0; c < K - 1; ++c) asc[c] = asc[c + 1];
+            asc[K - 1] = ~0ull;
+        }
+        if (lane == 0) {
+            out_idx[i * k + j] = m == ~0ull ? -1 : int32_t(uint32_t(m));
+            if (out_d2) out_d2[i * k + j] = m == ~0ull ? INFINITY : __uint_as_float(uint32_t(m >> 32));
+        }
+    }
+}
+
+static void launch_wave(dwg_ctx* ctx, const dwg_knn_index* I, const float* q, int64_t nq, const int* qlist, int k,
+                        const uint8_t* skip, int32_t* out_idx, float* out_d2) {
+    if (nq == 0) return;
+    dim3 grid(grid_for(nq, 4)), block(256);
+    const auto& L = I->lvl[1];
+#define DWG_KNNW(KK)                                                                                               \
+    hipLaunchKernelGGL(knn_wave_kernel<KK>, grid, block, 0, ctx->stream, L.G, L.cell_start, L.pts, L.empty, q, nq, \
+                       qlist, k, skip, out_idx, out_d2)
+    if (k <= 8) DWG_KNNW(8);
+    else if (k <= 10) DWG_KNNW(10);
+    else if (k <= 16) DWG_KNNW(16);
+    else DWG_KNNW(32);
+#undef DWG_KNNW
+    check_launch();
 }
 
 static void launch_per_lane(dwg_ctx* ctx, const dwg_knn_index* I, const float* q, int64_t nq, const int* qlist, int k,
@@ -540,18 +822,12 @@ static void launch_per_lane(dwg_ctx* ctx, const dwg_knn_index* I, const float* q
     check_launch();
 }
 
-void knn_query(dwg_ctx* ctx, const dwg_knn_index* I, const float* q, int64_t nq, int k, const uint8_t* skip,
-               const uint8_t* qmask, int32_t* out_idx, float* out_d2, const char* tag) {
-    DWG_REQUIRE(I, "null kNN index");
-    DWG_REQUIRE(k >= 1 && k <= 32, "k must be in [1, 32]");
-    DWG_REQUIRE(nq >= 0, "negative query count");
-    DWG_REQUIRE(nq < (int64_t(1) << 31) - 1, "too many queries");
-    if (nq == 0) return;
-    ProfileScope ps(ctx, tag ? tag : "knn_query");
-    if (nq < 4096) {  // too few to pay for the bucketing
-        launch_per_lane(ctx, I, q, nq, nullptr, k, skip, qmask, out_idx, out_d2);
-        return;
-    }
+// One tiled pass over the queries qlist[0..nq) (or 0..nq−1, masked by
+// qmask); the queries it cannot close are listed in redo.  Returns their count.
+template <int HALO>
+static int64_t tiled_pass(dwg_ctx* ctx, const dwg_knn_index* I, const float* q, int64_t nq, const int* qlist,
+                          const uint8_t* qmask, int k, const uint8_t* skip, int32_t* out_idx, float* out_d2,
+                          int* redo, const std::string& name, const KnnScatter* sc = nullptr) {
     const KnnGridArgs& G = I->lvl[0].G;
     TileArgs T;
     for (int d = 0; d < 3; ++d) T.tdim[d] = (G.dim[d] + kTile - 1) / kTile;
@@ -562,18 +838,17 @@ void knn_query(dwg_ctx* ctx, const dwg_knn_index* I, const float* q, int64_t nq,
     int* order = static_cast<int*>(ctx->buf("knnt_order", sizeof(int) * nq));
     int64_t* flags = static_cast<int64_t*>(ctx->buf("knnt_flags", sizeof(int64_t) * (ntiles + 1)));
     int* list = static_cast<int*>(ctx->buf("knnt_list", sizeof(int) * ntiles));
-    int* redo = static_cast<int*>(ctx->buf("knnt_redo", sizeof(int) * nq + 64));
     unsigned int* n_redo = static_cast<unsigned int*>(ctx->buf("knnt_nredo", 64));
     fill_u32(ctx, start, 0, 2 * (ntiles + 1));
     fill_u32(ctx, n_redo, 0, 1);
-    hipLaunchKernelGGL(tile_count_kernel, dim3(grid_for(nq, 256)), dim3(256), 0, ctx->stream, q, nq, qmask, G, T,
+    hipLaunchKernelGGL(tile_count_kernel, dim3(grid_for(nq, 256)), dim3(256), 0, ctx->stream, q, nq, qlist, qmask, G, T,
                        tile_of, reinterpret_cast<unsigned long long*>(start));
     check_launch();
     exclusive_scan<int64_t>(ctx, start, ntiles);
     hipLaunchKernelGGL(copy_i64_kernel, dim3(grid_for(ntiles, 256)), dim3(256), 0, ctx->stream, start, cursor, ntiles);
     check_launch();
-    hipLaunchKernelGGL(tile_scatter_kernel, dim3(grid_for(nq, 256)), dim3(256), 0, ctx->stream, tile_of, nq, cursor,
-                       order);
+    hipLaunchKernelGGL(tile_scatter_kernel, dim3(grid_for(nq, 256)), dim3(256), 0, ctx->stream, tile_of, nq, qlist,
+                       cursor, order);
     check_launch();
     hipLaunchKernelGGL(tile_flags_kernel, dim3(grid_for(ntiles + 1, 256)), dim3(256), 0, ctx->stream, start, ntiles,
                        flags);
@@ -582,15 +857,21 @@ void knn_query(dwg_ctx* ctx, const dwg_knn_index* I, const float* q, int64_t nq,
     hipLaunchKernelGGL(tile_list_kernel, dim3(grid_for(ntiles, 256)), dim3(256), 0, ctx->stream, flags, ntiles, list);
     check_launch();
     const int64_t nlist = read_scalar<int64_t>(ctx, flags + ntiles);
-    const std::string base = tag ? tag : "knn_query";
-    if (ctx->profile) ctx->count(base + ":queries", read_scalar<int64_t>(ctx, start + ntiles));
+    if (ctx->profile) ctx->count(name + ":queries", read_scalar<int64_t>(ctx, start + ntiles));
     if (nlist > 0) {
-        const std::string name = base + ":tiles";
         ProfileScope pt(ctx, name.c_str());
-#define DWG_KNNT(KK)                                                                                                  \
-    hipLaunchKernelGGL(knn_tile_kernel<KK>, dim3(unsigned(nlist)), dim3(kTileThreads), 0, ctx->stream, G, T,         \
-                       I->lvl[0].cell_start, I->lvl[0].pts, list, start, order, q, k, skip, out_idx, out_d2, redo, \
-                       n_redo)
+        const KnnScatter scv = sc ? *sc : KnnScatter{nullptr, nullptr};
+#define DWG_KNNT(KK)                                                                                                   \
+    do {                                                                                                               \
+        if (sc)                                                                                                        \
+            hipLaunchKernelGGL((knn_tile_kernel<KK, HALO, true>), dim3(unsigned(nlist)), dim3(kTileThreads), 0,        \
+                               ctx->stream, G, T, I->lvl[0].cell_start, I->lvl[0].pts, list, start, order, q, k, skip, \
+                               out_idx, out_d2, redo, n_redo, scv);                                                    \
+        else                                                                                                           \
+            hipLaunchKernelGGL((knn_tile_kernel<KK, HALO, false>), dim3(unsigned(nlist)), dim3(kTileThreads), 0,       \
+                               ctx->stream, G, T, I->lvl[0].cell_start, I->lvl[0].pts, list, start, order, q, k, skip, \
+                               out_idx, out_d2, redo, n_redo, scv);                                                    \
+    } while (0)
         if (k <= 8) DWG_KNNT(8);
         else if (k <= 10) DWG_KNNT(10);
         else if (k <= 16) DWG_KNNT(16);
@@ -599,10 +880,52 @@ void knn_query(dwg_ctx* ctx, const dwg_knn_index* I, const float* q, int64_t nq,
         check_launch();
     }
     const int64_t nredo = int64_t(read_scalar<unsigned int>(ctx, n_redo));
-    ctx->count(base + ":redo", nredo);
-    const std::string name = base + ":redo";
-    ProfileScope pr(ctx, name.c_str());
-    launch_per_lane(ctx, I, q, nredo, redo, k, skip, nullptr, out_idx, out_d2);
+    ctx->count(name + ":redo", nredo);
+    return nredo;
+}
+
+void knn_query(dwg_ctx* ctx, const dwg_knn_index* I, const float* q, int64_t nq, int k, const uint8_t* skip,
+               const uint8_t* qmask, int32_t* out_idx, float* out_d2, const char* tag, const KnnScatter* sc) {
+    DWG_REQUIRE(I, "null kNN index");
+    DWG_REQUIRE(k >= 1 && k <= 32, "k must be in [1, 32]");
+    DWG_REQUIRE(nq >= 0, "negative query count");
+    DWG_REQUIRE(nq < (int64_t(1) << 31) - 1, "too many queries");
+    DWG_REQUIRE(!sc || !qmask, "a scattering query covers every point");
+    if (nq == 0) return;
+    ProfileScope ps(ctx, tag ? tag : "knn_query");
+    const std::string base = tag ? tag : "knn_query";
+    if (nq < 4096) {  // too few to pay for the bucketing
+        launch_per_lane(ctx, I, q, nq, nullptr, k, skip, qmask, out_idx, out_d2);
+        if (sc) {
+            hipLaunchKernelGGL(scatter_pairs_kernel, dim3(grid_for(nq * k, 256)), dim3(256), 0, ctx->stream, out_idx, k,
+                               nullptr, nq, *sc);
+            check_launch();
+        }
+        return;
+    }
+    int* redo1 = static_cast<int*>(ctx->buf("knnt_redo1", sizeof(int) * nq + 64));
+    int* redo2 = static_cast<int*>(ctx->buf("knnt_redo2", sizeof(int) * nq + 64));
+    int* redo3 = static_cast<int*>(ctx->buf("knnt_redo3", sizeof(int) * nq + 64));
+    const int64_t n1 = tiled_pass<kHalo1>(ctx, I, q, nq, nullptr, qmask, k, skip, out_idx, out_d2, redo1,
+                                          base + ":tiles", sc);
+    const int64_t n2 = n1 > 0 ? tiled_pass<kHalo2>(ctx, I, q, n1, redo1, nullptr, k, skip, out_idx, out_d2, redo2,
+                                                   base + ":tiles2")
+                              : 0;
+    const int64_t n3 = n2 > 0 ? tiled_pass<kHalo3>(ctx, I, q, n2, redo2, nullptr, k, skip, out_idx, out_d2, redo3,
+                                                   base + ":tiles3")
+                              : 0;
+    {
+        const std::string name = base + ":redo";
+        ProfileScope pr(ctx, name.c_str());
+        launch_wave(ctx, I, q, n3, redo3, k, skip, out_idx, out_d2);
+    }
+    if (sc && n1 > 0) {  // the queries the first pass did not close scatter now
+        const std::string name = base + ":scatter";
+        ProfileScope pr(ctx, name.c_str());
+        hipLaunchKernelGGL(scatter_pairs_kernel, dim3(grid_for(n1 * k, 256)), dim3(256), 0, ctx->stream, out_idx, k,
+                           redo1, n1, *sc);
+        check_launch();
+    }
 }
 
 }  // namespace dwg

@@ -15,7 +15,7 @@ static void usage() {
                  "--in_name   The file name of the input point cloud (only supports .xyz currently)\n"
                  "--out_path   The absolute path of the output mesh/point cloud with normals\n"
                  "--output_mode   Ouput mode: 1 Mesh(.stl) (default)   2 Point cloud with normals(.xyz)   3 Both 1 and 2\n"
-                 "--in_normal   Initialization mode: 0 Random   1 Input   3 Gauss map (default)   4 Learned MLP (needs --mlp)\n"
+                 "--in_normal   Initialization mode: 0 Random   1 Input   2 Random (PCA areas with --com_As 1)   3 Gauss map (default)   4 Learned MLP (needs --mlp)   5 PCA (k=10)\n"
                  "--lambda   Screening Coefficient\n"
                  "--max_depth   The max depth of octree\n"
                  "--use_radii   Turn on screening mode or not (0 turn off   1 turn on)\n"

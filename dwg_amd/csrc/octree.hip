@@ -202,17 +202,27 @@ __global__ __launch_bounds__(256) void node_radius_kernel(const int4* __restrict
 
 // Float copies the traversal reads: (float(CM), float(R)) and the rounding
 // residual CM − float(CM).
-__global__ void node_pack_kernel(const double4* __restrict__ cm, const unsigned long long* __restrict__ rbits, int64_t m,
-                                 float4* __restrict__ cm_r, float4* __restrict__ cm_lo) {
+__global__ void node_pack_kernel(const double4* __restrict__ cm, const unsigned long long* __restrict__ rbits,
+                                 const int4* __restrict__ info, int64_t m, float4* __restrict__ cm_r,
+                                 float* __restrict__ radius, float4* __restrict__ cm_lo, float4* __restrict__ payload,
+                                 int2* __restrict__ kid) {
     const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i >= m) return;
+    // traversal record: (first child, 0) for an internal node (first child
+    // ≥ 1), (−count, begin) for a leaf (0 when empty)
+    const int4 nf = info[i];
+    kid[i] = nf.x >= 0 ? make_int2(nf.x, 0) : make_int2(nf.y - nf.z, nf.y);
     const double4 c = cm[i];
     const float cx = float(c.x), cy = float(c.y), cz = float(c.z);
-    cm_r[i] = make_float4(cx, cy, cz, float(__longlong_as_double((long long)rbits[i])));
+    const float r = float(__longlong_as_double((long long)rbits[i]));
+    radius[i] = r;
+    cm_r[i] = make_float4(cx, cy, cz, r);
     // the dipole of a node next to the query is (CM − q)·EC with CM − q
     // nearly tangent to EC; the residual makes CM − q accurate to one
     // rounding of the difference itself
     cm_lo[i] = make_float4(float(c.x - double(cx)), float(c.y - double(cy)), float(c.z - double(cz)), 0.f);
+    payload[2 * i] = make_float4(0.f, 0.f, 0.f, 0.f);
+    payload[2 * i + 1] = cm_lo[i];
 }
 
 __global__ void pack_positions_kernel(const double* __restrict__ P, const int* __restrict__ perm, int64_t n,
@@ -470,9 +480,12 @@ dwg_octree* octree_build(dwg_ctx* ctx, const double* P, const double* A, int64_t
                            code, T->perm, P, n, max_depth, rbits);
         check_launch();
         T->cm_r = dev_alloc<float4>(m);
+        T->radius = dev_alloc<float>(m);
         T->cm_lo = dev_alloc<float4>(m);
-        hipLaunchKernelGGL(node_pack_kernel, dim3(grid_for(m, 256)), dim3(256), 0, ctx->stream, T->cm64, rbits, m,
-                           T->cm_r, T->cm_lo);
+        T->payload = dev_alloc<float4>(2 * m);
+        T->kid = dev_alloc<int2>(m);
+        hipLaunchKernelGGL(node_pack_kernel, dim3(grid_for(m, 256)), dim3(256), 0, ctx->stream, T->cm64, rbits, T->info,
+                           m, T->cm_r, T->radius, T->cm_lo, T->payload, T->kid);
         check_launch();
 
         // 4. sources in tree order and the per-iteration buffers

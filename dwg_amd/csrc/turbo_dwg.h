@@ -6,6 +6,7 @@
 
 #include <chrono>
 #include <cstdint>
+#include <cstdlib>
 #include <string>
 #include <vector>
 
@@ -32,6 +33,22 @@ struct DevArray {
     void resize(int64_t n);
     void upload(const T* h, int64_t n, void* stream);
     void download(T* h, int64_t n, void* stream) const;
+};
+
+// Triangle soup on the host (ntri × 9 floats, std::malloc'd so that it can
+// be handed to a dwg_result and released by dwg_free).
+struct HostMesh {
+    float* data = nullptr;
+    int64_t ntri = 0;
+    HostMesh() = default;
+    HostMesh(const HostMesh&) = delete;
+    HostMesh& operator=(const HostMesh&) = delete;
+    ~HostMesh() { std::free(data); }
+    float* release() {
+        float* d = data;
+        data = nullptr;
+        return d;
+    }
 };
 
 struct Grid {
@@ -61,12 +78,16 @@ public:
     ~TurboDWG();
 
     // TurboDWG::TurboDWG (TurboDWG.cu:14-83): sources (device FP64, n×3),
-    // normals (device, n×3; nullptr → zeros), areas 2e-6 (DWGProcess.h:619),
-    // radii 0.0015, grid.  `prebuilt` (nullable, ownership taken): an octree
-    // already built over exactly these points with constant areas and leaf
-    // size 1.
+    // normals (device, n×3; nullptr → zeros), areas (device; nullptr → 2e-6,
+    // DWGProcess.h:619), radii 0.0015, grid.  `prebuilt` (nullable, ownership
+    // taken): an octree already built over exactly these points with constant
+    // areas and leaf size 1.
     void SetSources(const double* d_pts, int64_t n, const float* d_nrm, const Grid& grid,
-                    dwg_octree* prebuilt = nullptr);
+                    dwg_octree* prebuilt = nullptr, const float* d_area = nullptr);
+    // per-source areas (device, n floats) for the far-field moments EC = Σ A·n
+    // (TurboDWG::UpdateAs, TurboDWG.cu:693-701); CM and R keep the areas the
+    // octree was built with
+    void SetAreas(const float* d_area);
     // Radii of the sources from their k nearest neighbours (GetRadiiandAs /
     // TurboDWG::UpdateRadii).
     void UpdateRadii(int k);
@@ -80,9 +101,9 @@ public:
     // d_points: device FP64 (np×3); points_tree (nullable, ownership taken):
     // their octree, reused as in TurboDWG.cu:869-870.
     void GenerateMesh(const double* d_points, int64_t np, const Grid& final_grid, int64_t* tri_count,
-                      std::vector<float>* tris, std::vector<float>* point_normals, dwg_octree* points_tree = nullptr);
+                      HostMesh* tris, std::vector<float>* point_normals, dwg_octree* points_tree = nullptr);
     // GenerateMesh(grid) variant of --use_leaves_generate 1 (TurboDWG.cu:768-806).
-    void GenerateMeshFromSources(const Grid& final_grid, int64_t* tri_count, std::vector<float>* tris);
+    void GenerateMeshFromSources(const Grid& final_grid, int64_t* tri_count, HostMesh* tris);
 
     // exact Barnes-Hut term counts of one source and one grid evaluation
     void CountTerms(int64_t* src_terms, int64_t* grid_terms);
@@ -94,7 +115,7 @@ public:
 
 private:
     void check(int rc, const char* what) const;
-    void mesh_pass(const Grid& g, float iso_scale, int64_t* tri_count, std::vector<float>* tris);
+    void mesh_pass(const Grid& g, float iso_scale, int64_t* tri_count, HostMesh* tris);
 
     dwg_ctx* ctx_;
     dwg_options opt_;

@@ -16,6 +16,7 @@
 #include <stdexcept>
 
 #include "common.h"
+#include "diffuse.h"
 #include "gwn.h"
 #include "knn.h"
 #include "octree.h"
@@ -49,6 +50,9 @@ void DevArray<T>::upload(const T* h, int64_t n, void* stream) {
     resize(n);
     if (n > 0 && hipMemcpyAsync(ptr, h, sizeof(T) * n, hipMemcpyHostToDevice, static_cast<hipStream_t>(stream)) != hipSuccess)
         throw dwg::Error(DWG_ERR_HIP, "upload failed");
+    // the host buffer is pageable and often a temporary: wait for the copy
+    if (n > 0 && hipStreamSynchronize(static_cast<hipStream_t>(stream)) != hipSuccess)
+        throw dwg::Error(DWG_ERR_HIP, "sync failed");
 }
 
 template <class T>
@@ -181,7 +185,7 @@ void TurboDWG::check(int rc, const char* what) const {
 }
 
 void TurboDWG::SetSources(const double* d_pts, int64_t n, const float* d_nrm, const Grid& grid,
-                          dwg_octree* prebuilt) {
+                          dwg_octree* prebuilt, const float* d_area) {
     std::unique_ptr<dwg_octree, void (*)(dwg_octree*)> owned(prebuilt, dwg_octree_destroy);
     n_ = n;
     hipStream_t s = ctx_->stream;
@@ -209,12 +213,23 @@ void TurboDWG::SetSources(const double* d_pts, int64_t n, const float* d_nrm, co
     tree_ = nullptr;
     if (knn_) dwg_knn_destroy(knn_);
     knn_ = nullptr;
-    if (owned && opt_.leaf_size == 1 && owned->n_points == n_) {
+    if (d_area) {
+        // computed areas weight CM as well (fast_winding_number with As, DWGProcess.h:711-722)
+        SetAreas(d_area);
+        DevArray<double> a64(n_);
+        dwg::to_f64(ctx_, area_.ptr, n_, a64.ptr);
+        check(dwg_octree_build_device(ctx_, d_pts, a64.ptr, n_, opt_.max_depth, opt_.leaf_size, &tree_), "octree");
+    } else if (owned && opt_.leaf_size == 1 && owned->n_points == n_) {
         tree_ = owned.release();
     } else {
         check(dwg_octree_build_device(ctx_, d_pts, nullptr, n_, opt_.max_depth, opt_.leaf_size, &tree_), "octree");
     }
     check(dwg_knn_build(ctx_, pts_.ptr, n_, 15, &knn_), "knn index");
+}
+
+void TurboDWG::SetAreas(const float* d_area) {
+    if (hipMemcpyAsync(area_.ptr, d_area, sizeof(float) * n_, hipMemcpyDeviceToDevice, ctx_->stream) != hipSuccess)
+        throw dwg::Error(DWG_ERR_HIP, "area copy failed");
 }
 
 void TurboDWG::UpdateRadii(int k) {
@@ -276,7 +291,7 @@ double TurboDWG::TurboScreenedDWGLaunch(int total_iters, double* w_mean, int64_t
         const float* rad = opt_.use_radii ? radii_.ptr : nullptr;
         check(dwg_octree_update(ctx_, tree_, nrm_.ptr, area_.ptr, rad), "moments");
         if (opt_.beta > 0) {
-            check(dwg_gwn_tree(ctx_, tree_, pts_.ptr, n_, opt_.beta, sigma, flags, w_src_.ptr), "gwn sources");
+            check(dwg_gwn_tree_self(ctx_, tree_, opt_.beta, sigma, flags, w_src_.ptr), "gwn sources");
             check(dwg_gwn_tree_grid(ctx_, tree_, &g, opt_.beta, sigma, flags, w_grid_.ptr), "gwn grid");
         } else {
             check(dwg_gwn_direct(ctx_, pts_.ptr, nrm_.ptr, area_.ptr, rad, n_, pts_.ptr, n_, sigma, flags, w_src_.ptr,
@@ -332,7 +347,7 @@ void TurboDWG::CountTerms(int64_t* src_terms, int64_t* grid_terms) {
     check(dwg_gwn_tree_grid_terms(ctx_, tree_, &g, opt_.beta, grid_terms), "grid terms");
 }
 
-void TurboDWG::mesh_pass(const Grid& grid, float iso_scale, int64_t* tri_count, std::vector<float>* tris) {
+void TurboDWG::mesh_pass(const Grid& grid, float iso_scale, int64_t* tri_count, HostMesh* tris) {
     // one screened evaluation at the sources and on the grid with the
     // unscheduled σ, iso = mean source value, marching cubes
     struct Tag {  // kernels of the final pass are profiled as "<kernel>:final"
@@ -349,7 +364,7 @@ void TurboDWG::mesh_pass(const Grid& grid, float iso_scale, int64_t* tri_count, 
     check(dwg_octree_update(ctx_, tree_, nrm_.ptr, area_.ptr, rad), "moments");
     if (opt_.As_threshold > 0) check(dwg_outlier_filter(ctx_, area_.ptr, n_, opt_.As_threshold), "outlier filter");
     const float beta = opt_.beta > 0 ? opt_.beta : 1e30f;  // β → ∞ takes every point term
-    check(dwg_gwn_tree(ctx_, tree_, pts_.ptr, n_, beta, opt_.sigma, flags, w_src_.ptr), "gwn points");
+    check(dwg_gwn_tree_self(ctx_, tree_, beta, opt_.sigma, flags, w_src_.ptr), "gwn points");
     check(dwg_gwn_tree_grid(ctx_, tree_, &g, beta, opt_.sigma, flags, wg.ptr), "gwn final grid");
     double sum = 0;
     check(dwg_reduce_sum(ctx_, w_src_.ptr, n_, &sum), "reduce");
@@ -360,22 +375,36 @@ void TurboDWG::mesh_pass(const Grid& grid, float iso_scale, int64_t* tri_count, 
     if (tris) {
         DevArray<float> t(9 * std::max<int64_t>(ntri, 1));
         check(dwg_marching_cubes(ctx_, wg.ptr, &g, iso, ntri, t.ptr, nullptr, nullptr, &ntri), "marching cubes");
-        tris->resize(9 * ntri);
-        t.download(tris->data(), 9 * ntri, ctx_->stream);
+        // straight into the buffer the caller receives (no second host copy)
+        std::free(tris->data);
+        tris->data = static_cast<float*>(std::malloc(sizeof(float) * 9 * size_t(std::max<int64_t>(ntri, 1))));
+        if (!tris->data) throw dwg::Error(DWG_ERR_ARG, "host allocation of the mesh failed");
+        tris->ntri = ntri;
+        t.download(tris->data, 9 * ntri, ctx_->stream);
     }
     *tri_count = ntri;
 }
 
 void TurboDWG::GenerateMesh(const double* d_points, int64_t np, const Grid& final_grid, int64_t* tri_count,
-                            std::vector<float>* tris, std::vector<float>* point_normals, dwg_octree* points_tree) {
+                            HostMesh* tris, std::vector<float>* point_normals, dwg_octree* points_tree) {
     std::unique_ptr<dwg_octree, void (*)(dwg_octree*)> owned(points_tree, dwg_octree_destroy);
     // normals of the points from their normal_kw nearest sources (LoadNormals)
     DevArray<float> dp(3 * np), dn(3 * np);
     dwg::to_f32(ctx_, d_points, 3 * np, dp.ptr);
     check(dwg_transfer_normals(ctx_, knn_, nrm_.ptr, dp.ptr, np, opt_.normal_kw, dn.ptr), "transfer normals");
+    // --com_As 1: Voronoi areas of the points from their 15 nearest points and
+    // the transferred normals (PreGeneration + point_areas + UpdateAs,
+    // DWGProcess.h:771-792)
+    DevArray<float> pa;
+    if (opt_.com_As) {
+        pa.resize(np);
+        std::unique_ptr<dwg_knn_index> pidx(dwg::knn_build(ctx_, dp.ptr, np, 15));
+        dwg::point_areas(ctx_, pidx.get(), dp.ptr, np, 15, dn.ptr, pa.ptr);
+    }
     // the points become the sources (TurboDWG.cu:865-876), reusing the octree
-    // over the input points (TurboDWG.cu:869-870)
+    // over the input points (TurboDWG.cu:869-870), built with constant areas
     SetSources(d_points, np, dn.ptr, grid_, owned.release());
+    if (opt_.com_As) SetAreas(pa.ptr);
     UpdateRadii(opt_.final_kw);
     mesh_pass(final_grid, 1.0f, tri_count, tris);
     if (point_normals) {
@@ -384,7 +413,7 @@ void TurboDWG::GenerateMesh(const double* d_points, int64_t np, const Grid& fina
     }
 }
 
-void TurboDWG::GenerateMeshFromSources(const Grid& final_grid, int64_t* tri_count, std::vector<float>* tris) {
+void TurboDWG::GenerateMeshFromSources(const Grid& final_grid, int64_t* tri_count, HostMesh* tris) {
     UpdateRadii(opt_.final_kw);
     mesh_pass(final_grid, 1.0f, tri_count, tris);
 }
@@ -434,7 +463,6 @@ void GetRandomNs(int64_t n, uint64_t seed, std::vector<double>& N) {
 void DWGLaunch(dwg_ctx* ctx, const float* pts, const float* nrm, int64_t n, const dwg_options& opt,
                const char* out_base, dwg_result* res) {
     if (n <= 0 || !pts) throw dwg::Error(DWG_ERR_ARG, "no input points");
-    if (opt.com_As) throw dwg::Error(DWG_ERR_ARG, "--com_As 1 (Voronoi point areas) is not available in this build");
     if (opt.input_normal == 1 && !nrm) throw dwg::Error(DWG_ERR_ARG, "--in_normal 1 needs input normals");
     if (opt.input_normal == 4 && !opt.mlp) throw dwg::Error(DWG_ERR_ARG, "--in_normal 4 needs MLP weights");
     const auto t0 = Clock::now();
@@ -453,13 +481,27 @@ void DWGLaunch(dwg_ctx* ctx, const float* pts, const float* nrm, int64_t n, cons
             dwg::to_f64(ctx, P32.ptr, 3 * n, P.ptr);
         }
     }
-    // initial point normals (DWGProcess.h:488-510); the Gauss-map and learned
-    // initialisations overwrite the source normals, so they need none
+    // point normals (DWGProcess.h:152-210, 488-510).  Input normals for
+    // --in_normal 1; PCA normals (k = 10, turned towards the bounding-box
+    // centre) when --com_As 1 needs them for the area filter or for the PCA
+    // start (5, an extension); random otherwise.  The Gauss-map and learned
+    // starts overwrite the source normals later.
     DevArray<double> Ns;
-    if (opt.input_normal == 0 || opt.input_normal == 1) {
+    if (opt.input_normal == 1) {
+        std::vector<double> h(nrm, nrm + 3 * n);
+        Ns.upload(h.data(), 3 * n, stream);
+    } else if (opt.com_As || opt.input_normal == 5) {
+        float lo[3], hi[3];
+        dwg::bounds(ctx, P32.ptr, n, lo, hi);
+        const float vp[3] = {(lo[0] + hi[0]) / 2.0f, (lo[1] + hi[1]) / 2.0f, (lo[2] + hi[2]) / 2.0f};
+        std::unique_ptr<dwg_knn_index> index(dwg::knn_build(ctx, P32.ptr, n, 10));
+        DevArray<float> pn(3 * n);
+        dwg::pca_normals(ctx, index.get(), P32.ptr, n, 10, vp, pn.ptr);
+        Ns.resize(3 * n);
+        dwg::to_f64(ctx, pn.ptr, 3 * n, Ns.ptr);
+    } else if (opt.input_normal == 0 || opt.input_normal == 2) {
         std::vector<double> h;
-        if (opt.input_normal == 1) h.assign(nrm, nrm + 3 * n);
-        else GetRandomNs(n, opt.seed, h);
+        GetRandomNs(n, opt.seed, h);
         Ns.upload(h.data(), 3 * n, stream);
     }
     // sources
@@ -494,8 +536,26 @@ void DWGLaunch(dwg_ctx* ctx, const float* pts, const float* nrm, int64_t n, cons
         }
     }
     const Grid grid = voxel_grid_bounds(plo, phi, opt.offset_1, opt.grid_reso);
+    // --com_As 1: Voronoi areas of the sources from their 15 nearest sources
+    // and normals (GetRadiiandAs, DWGProcess.h:345-380)
+    DevArray<float> src_area;
+    if (opt.com_As) {
+        DevArray<float> S32(3 * ns);
+        dwg::to_f32(ctx, S, 3 * ns, S32.ptr);
+        src_area.resize(ns);
+        std::unique_ptr<dwg_knn_index> sidx(dwg::knn_build(ctx, S32.ptr, ns, 15));
+        dwg::point_areas(ctx, sidx.get(), S32.ptr, ns, 15, Sn.ptr, src_area.ptr);
+        if (opt.input_normal == 0 || opt.input_normal == 2) {
+            // the PCA normals only served the areas: the sources start from
+            // random normals (DWGProcess.h:643-656)
+            std::vector<double> h;
+            GetRandomNs(ns, opt.seed + 1, h);
+            std::vector<float> hf(h.begin(), h.end());
+            Sn.upload(hf.data(), 3 * ns, stream);
+        }
+    }
     TurboDWG dwg(ctx, opt);
-    dwg.SetSources(S, ns, Sn.ptr, grid);
+    dwg.SetSources(S, ns, Sn.ptr, grid, nullptr, opt.com_As ? src_area.ptr : nullptr);
     dwg.UpdateRadii(opt.kw);
     if (opt.input_normal == 3) dwg.GaussMapInit();
     if (opt.input_normal == 4) dwg.MlpInit(opt.mlp);
@@ -516,7 +576,8 @@ void DWGLaunch(dwg_ctx* ctx, const float* pts, const float* nrm, int64_t n, cons
                                                      res ? res->iter_tris : nullptr, res ? res->iter_zeros : nullptr);
     const auto t2 = Clock::now();
     int64_t ntri = 0;
-    std::vector<float> tris, point_nrm;
+    HostMesh tris;
+    std::vector<float> point_nrm;
     const bool want_tris = (res && res->want_mesh) || (out_base && (opt.output_mode == 1 || opt.output_mode == 3));
     if (opt.use_leaves_generate == 1) {
         double slo[3], shi[3];
@@ -535,7 +596,7 @@ void DWGLaunch(dwg_ctx* ctx, const float* pts, const float* nrm, int64_t n, cons
     }
     const double t_final = seconds_since(t2);
     if (out_base) {
-        if (opt.output_mode == 1 || opt.output_mode == 3) save_binary_stl(std::string(out_base) + ".stl", tris.data(), ntri);
+        if (opt.output_mode == 1 || opt.output_mode == 3) save_binary_stl(std::string(out_base) + ".stl", tris.data, ntri);
         if (opt.output_mode == 2 || opt.output_mode == 3) {
             const int64_t np = int64_t(point_nrm.size() / 3);
             DevArray<float> f(3 * np);
@@ -549,12 +610,7 @@ void DWGLaunch(dwg_ctx* ctx, const float* pts, const float* nrm, int64_t n, cons
         if (res->normals && opt.use_leaves_generate != 1)
             std::copy(point_nrm.begin(), point_nrm.begin() + std::min<size_t>(point_nrm.size(), size_t(3 * n)), res->normals);
         res->tri_count = ntri;
-        res->mesh = nullptr;
-        if (res->want_mesh && ntri > 0) {
-            res->mesh = static_cast<float*>(std::malloc(sizeof(float) * 9 * size_t(ntri)));
-            if (!res->mesh) throw dwg::Error(DWG_ERR_ARG, "host allocation of the mesh failed");
-            std::copy(tris.begin(), tris.begin() + 9 * ntri, res->mesh);
-        }
+        res->mesh = (res->want_mesh && ntri > 0) ? tris.release() : nullptr;
         res->n_sources = ns;
         for (int d = 0; d < 3; ++d) res->center[d] = center[d];
         res->scale = scale;

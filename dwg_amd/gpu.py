@@ -259,6 +259,20 @@ def radii(ctx: Context, index: KnnIndex, k=10, rmin=0.0015, rmax=0.015):
     return out
 
 
+def point_areas(ctx: Context, index: KnnIndex, k=15, nrm=None):
+    out = torch.empty(index.n, device=ctx.device, dtype=torch.float32)
+    nr = None if nrm is None else ctx.f32(nrm)
+    ctx.check(ctx.L.dwg_point_areas(ctx.h, index.h, _ptr(index.pts), index.n, int(k), _ptr(nr), _ptr(out)))
+    return out
+
+
+def pca_normals(ctx: Context, index: KnnIndex, k=10, viewpoint=(0.0, 0.0, 0.0)):
+    out = torch.empty((index.n, 3), device=ctx.device, dtype=torch.float32)
+    vp = (ctypes.c_float * 3)(*[float(v) for v in viewpoint])
+    ctx.check(ctx.L.dwg_pca_normals(ctx.h, index.h, _ptr(index.pts), index.n, int(k), vp, _ptr(out)))
+    return out
+
+
 def gauss_map(ctx: Context, pts, center):
     pts = ctx.f32(pts)
     out = torch.empty_like(pts)

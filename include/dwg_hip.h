@@ -104,6 +104,11 @@ int dwg_octree_update(dwg_ctx* ctx, dwg_octree* tree, const float* d_nrm, const 
  * TurboLibiGWN.cu:912-922; plain: TurboLibiGWNLaunch :900-910). */
 int dwg_gwn_tree(dwg_ctx* ctx, const dwg_octree* tree, const float* d_q, int64_t nq, float beta,
                  float sigma, uint32_t flags, float* d_out);
+/* The same sums at the octree's own points, evaluated in tree order (one
+ * wave per 64 consecutive tree slots) and written in the caller's point order
+ * (the source self-evaluation of TurboScreenedDWGLaunch, TurboDWG.cu:536). */
+int dwg_gwn_tree_self(dwg_ctx* ctx, const dwg_octree* tree, float beta, float sigma, uint32_t flags,
+                      float* d_out);
 /* Barnes-Hut sums at every point of an implicit grid (the reference
  * materialises the grid and calls the launch above, TurboDWG.cu:537). */
 int dwg_gwn_tree_grid(dwg_ctx* ctx, const dwg_octree* tree, const dwg_grid* grid, float beta,
@@ -150,6 +155,19 @@ int dwg_diffuse_normals(dwg_ctx* ctx, const dwg_knn_index* src_index, const floa
  * (DWGProcess.h:325-343; TurboDWG::UpdateRadii, TurboDWG.cu:674-691). */
 int dwg_radii(dwg_ctx* ctx, const dwg_knn_index* index, const float* d_pts, int64_t n, int k,
               float rmin, float rmax, float* d_out);
+/* PCA normals (PCL NormalEstimation, DWGProcess.h:152-200): smallest-
+ * eigenvalue direction of the covariance of each point's k nearest
+ * neighbours (itself included; `index` built over d_pts), turned towards
+ * h_viewpoint (the reference uses the bounding-box centre). */
+int dwg_pca_normals(dwg_ctx* ctx, const dwg_knn_index* index, const float* d_pts, int64_t n, int k,
+                    const float* h_viewpoint, float* d_out);
+/* Voronoi point areas (igl::copyleft::cgal::point_areas, point_areas.cpp:
+ * 41-205, which DWGProcess.h:345-380 and :790 call with k = 15) without
+ * CGAL: the k nearest neighbours of each point (itself first; `index` over
+ * d_pts), those whose normal faces the point's own (d_nrm, NULL = all kept),
+ * the PCA plane, the Delaunay triangles at the point, its Voronoi share. */
+int dwg_point_areas(dwg_ctx* ctx, const dwg_knn_index* index, const float* d_pts, int64_t n, int k,
+                    const float* d_nrm, float* d_out);
 /* n_i = (p_i − c)/(|p_i − c| + 1e-8) (TurboNormalInit.cu:3-27). */
 int dwg_gauss_map(dwg_ctx* ctx, const float* d_pts, int64_t n, const float* h_center,
                   float* d_nrm);
@@ -215,7 +233,7 @@ int dwg_gemm_bf16(dwg_ctx* ctx, const uint16_t* d_a, int64_t s_ai, int64_t s_ak,
  * via dwg_options_default().  Fields after output_mode have no reference
  * flag. */
 typedef struct dwg_options {
-    int32_t input_normal;        /* 0 random, 1 input normals, 3 Gauss map, 4 learned MLP */
+    int32_t input_normal;        /* 0 random, 1 input normals, 2 random (PCA areas with com_As), 3 Gauss map, 4 learned MLP, 5 PCA */
     int32_t grid_reso;           /* --grid_reso, 400 */
     int32_t final_reso;          /* --final_reso, 500 */
     int32_t kw;                  /* --kw, 10 */

@@ -247,6 +247,82 @@ def gauss_map(P):
     return N
 
 
+def pca_normals(P, k=10, viewpoint=None):
+    """PCA normals (PCL NormalEstimation with setKSearch(k) and
+    setViewPoint, DWGProcess.h:152-200): the smallest-eigenvalue eigenvector
+    of the covariance of each point's k nearest neighbours (the point itself
+    included, float32 (d², index) ranking), FP64 LAPACK eigen-solver, turned
+    so that n·(viewpoint − p) ≥ 0.  viewpoint defaults to the bounding-box
+    centre of the float32 points."""
+    P32 = _c(P, np.float32).reshape(-1, 3)
+    if viewpoint is None:
+        viewpoint = (P32.min(0) + P32.max(0)) / np.float32(2)
+    idx = knn_brute(P32, P32, k)
+    Q = P32.astype(np.float64)[idx]                      # n × k × 3
+    D = Q - Q.mean(1, keepdims=True)
+    C = np.einsum("nki,nkj->nij", D, D)
+    _, U = np.linalg.eigh(C)
+    N = U[:, :, 0]
+    t = np.asarray(viewpoint, np.float64) - P32.astype(np.float64)
+    N[(N * t).sum(1) < 0] *= -1
+    return N
+
+
+def point_areas(P, I, N=None):
+    """Voronoi point areas, igl::copyleft::cgal::point_areas
+    (deps/libigl/include/igl/copyleft/cgal/point_areas.cpp:41-205), FP64:
+    neighbours I[i] (the first is the point itself), kept where
+    n_first·n_j > 0; ≤ 2 kept → 0; PCA plane of the kept neighbours (SVD
+    scores, first two columns); 2-D Delaunay triangulation (here Qhull via
+    scipy instead of CGAL); every triangle at the first point adds its
+    Voronoi share (half the triangle if obtuse there, a quarter if obtuse
+    elsewhere, else the circumcentre split); non-finite → 0."""
+    from scipy.spatial import Delaunay
+    from scipy.spatial import QhullError
+    P = np.asarray(P, np.float64).reshape(-1, 3)
+    I = np.asarray(I)
+    A = np.zeros(len(I))
+    for i in range(len(I)):
+        nb = I[i][I[i] >= 0]
+        X = P[nb]
+        if N is not None and len(nb) > 1:
+            Nn = np.asarray(N, np.float64)[nb]
+            X = X[(Nn @ Nn[0]) > 0]
+        if len(X) <= 2:
+            continue
+        C = X - X.mean(0)
+        U, S, _ = np.linalg.svd(C, full_matrices=False)
+        plane = (U * S)[:, :2]
+        try:
+            tri = Delaunay(plane)
+        except QhullError:
+            continue
+        acc = 0.0
+        for f in tri.simplices:
+            if 0 not in f:
+                continue
+            f = list(f)
+            X0 = f.index(0)
+            Y, Z = f[(X0 + 1) % 3], f[(X0 + 2) % 3]
+            x = np.linalg.norm(plane[Y] - plane[Z])
+            y = np.linalg.norm(plane[0] - plane[Z])
+            z = np.linalg.norm(plane[Y] - plane[0])
+            cx = (z * z + y * y - x * x) / (2 * y * z)
+            cy = (z * z + x * x - y * y) / (2 * x * z)
+            cz = (x * x + y * y - z * z) / (2 * y * x)
+            bary = np.array([x * cx, y * cy, z * cz])
+            bary /= bary.sum()
+            full = 0.25 * np.sqrt((x + (y + z)) * (z - (x - y)) * (z + (x - y)) * (x + (y - z)))
+            if cx < 0:
+                acc += 0.5 * full
+            elif cy < 0 or cz < 0:
+                acc += 0.25 * full
+            else:
+                acc += (bary[1] + bary[2]) * full / 2
+        A[i] = acc if np.isfinite(acc) else 0.0
+    return A
+
+
 def sigma_schedule(it, sigma_in):
     return float(lib().orc_sigma_schedule(int(it), float(sigma_in)))
 

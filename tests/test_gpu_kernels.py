@@ -240,3 +240,63 @@ def test_small_reductions(ctx):
     c = (lo + hi) / 2
     g = gpu.gauss_map(ctx, P, c).cpu().numpy()
     assert np.allclose(g, O.gauss_map(P.astype(float)), atol=1e-6)
+
+
+def test_pca_normals_match_oracle(ctx):
+    rng = np.random.default_rng(63)
+    P, _ = shapes.torus(20_000, 64)
+    P = (P + rng.normal(0, 0.002, P.shape)).astype(np.float32)
+    lo, hi = P.min(0), P.max(0)
+    vp = (lo + hi) / 2
+    idx = gpu.KnnIndex(ctx, P, k_hint=10)
+    got = gpu.pca_normals(ctx, idx, 10, vp).cpu().numpy().astype(np.float64)
+    ref = O.pca_normals(P, 10, vp)
+    assert np.allclose(np.linalg.norm(got, axis=1), 1.0, atol=1e-6)
+    c = (got * ref).sum(1)
+    # same line (FP64 on both sides; only near-degenerate covariances may differ)
+    assert np.quantile(np.abs(c), 0.001) > 0.9999
+    # same side of the viewpoint wherever the side is well defined
+    t = vp - P
+    sure = np.abs((ref * t).sum(1)) > 1e-3 * np.linalg.norm(t, axis=1)
+    assert np.all(c[sure] > 0)
+
+
+def test_point_areas_match_oracle(ctx):
+    # noisy sphere with 10% of the normals flipped (the filter drops them),
+    # against the oracle's Qhull-triangulated restatement
+    rng = np.random.default_rng(71)
+    P, N = shapes.sphere(6000, 72)
+    P = (0.5 * P + rng.normal(0, 0.001, P.shape)).astype(np.float32)
+    N = N.astype(np.float32)
+    N[rng.uniform(size=len(N)) < 0.1] *= -1
+    idx = gpu.KnnIndex(ctx, P, k_hint=15)
+    got = gpu.point_areas(ctx, idx, 15, N).cpu().numpy().astype(np.float64)
+    nb = O.knn_brute(P, P, 15)
+    ref = O.point_areas(P.astype(np.float64), nb, N.astype(np.float64))
+    close = np.abs(got - ref) <= 1e-5 * np.abs(ref) + 1e-12
+    assert close.mean() > 0.999, (close.mean(), np.abs(got - ref).max())
+    assert np.all(got >= 0)
+    # without normals every neighbour is kept
+    a = gpu.point_areas(ctx, idx, 15).cpu().numpy().astype(np.float64)
+    ref = O.point_areas(P.astype(np.float64), nb, None)
+    assert (np.abs(a - ref) <= 1e-5 * ref + 1e-12).mean() > 0.999
+    # a quasi-uniform (Fibonacci) sphere has every Delaunay neighbour among the
+    # 15 nearest, so the cells tile the sphere: mean area = 4πr²/n
+    n = 6000
+    t = np.arange(n) + 0.5
+    phi, th = np.arccos(1 - 2 * t / n), np.pi * (1 + 5 ** 0.5) * t
+    F = (0.5 * np.stack([np.cos(th) * np.sin(phi), np.sin(th) * np.sin(phi), np.cos(phi)], 1)).astype(np.float32)
+    a = gpu.point_areas(ctx, gpu.KnnIndex(ctx, F, k_hint=15), 15).cpu().numpy()
+    assert abs(a.mean() / (np.pi / n) - 1) < 0.002
+
+
+def test_point_areas_hex_lattice(ctx):
+    h = 0.01
+    ij = np.array([(i, j) for i in range(-30, 31) for j in range(-30, 31)], np.float64)
+    xy = np.stack([h * (ij[:, 0] + 0.5 * ij[:, 1]), h * (np.sqrt(3) / 2) * ij[:, 1]], 1)
+    R, _ = np.linalg.qr(np.random.default_rng(73).normal(size=(3, 3)))
+    P = (np.concatenate([xy, np.zeros((len(xy), 1))], 1) @ R.T).astype(np.float32)
+    idx = gpu.KnnIndex(ctx, P, k_hint=15)
+    a = gpu.point_areas(ctx, idx, 15).cpu().numpy()
+    inner = np.abs(ij).max(1) <= 25
+    assert np.allclose(a[inner], np.sqrt(3) / 2 * h * h, rtol=1e-4)

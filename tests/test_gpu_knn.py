@@ -62,3 +62,20 @@ def test_knn_tiled_skip_region_and_mask(ctx):
     sel = qmask == 1
     assert np.array_equal(got[sel], ref[sel])
     assert np.all(got[~sel] == -1)
+
+
+def test_knn_far_queries(ctx):
+    # two small dense spheres far apart: queries in the empty space between
+    # them start many coarse cells away from any point and close only after
+    # wide shells (the wave fallback)
+    rng = np.random.default_rng(45)
+    S, _ = shapes.sphere(20_000, 46)
+    P = np.concatenate([0.05 * S[:10_000] + [-0.8, 0, 0], 0.05 * S[10_000:] + [0.8, 0.3, -0.2]]).astype(np.float32)
+    Q = np.concatenate([rng.uniform(-0.85, 0.85, (6000, 3)),
+                        rng.normal(0, 0.01, (2000, 3)) + [0.0, 0.1, 0.0]]).astype(np.float32)
+    for k in (10, 15):
+        idx = gpu.KnnIndex(ctx, P, k_hint=k)
+        got, d2 = idx.query(Q, k, d2=True)
+        ref, rd2 = O.knn_brute(P, Q, k, return_d2=True)
+        assert np.array_equal(got.cpu().numpy(), ref)
+        assert np.array_equal(d2.cpu().numpy(), rd2)
