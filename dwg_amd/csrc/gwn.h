@@ -19,7 +19,7 @@ struct dwg_octree {
     float far_beta = -1.f;     // β that cm_r.w was computed for
     float4* cm_lo = nullptr;   // (CM − float(CM), 0): residual of the rounded centre
     float4* ec = nullptr;      // (EC/4π, 0), refreshed per iteration
-    uint2* ec_bf16 = nullptr;  // bf16 copy of EC/4π
+
     float4* sa = nullptr;      // packed sources in tree order (x, y, z, s)
     float4* sb = nullptr;      // (A·n/4π, A)
     int* perm = nullptr;       // tree slot → caller index
@@ -30,6 +30,7 @@ struct dwg_octree {
     // in one vector load: internal node (EC/4π, 0, CM − float(CM), 0);
     // leaf with one point: that point's (A·n/4π, A, x, y, z, s)
     float4* payload = nullptr;
+    bool payload_bf16 = false;  // internal nodes' EC/4π in the payload rounded to BF16
     ~dwg_octree();
 };
 

@@ -27,10 +27,10 @@ namespace dwg {
 // per-pair terms
 // ---------------------------------------------------------------------------
 // term from the offset d = p − q; s = screening radius of the source
+// term from the offset d and a squared distance r2 (normally |d|²)
 template <bool SCREENED>
-__device__ __forceinline__ float gwn_term_d(float dx, float dy, float dz, float s, const float4 b, float sigma,
-                                            float sigma_l2e) {
-    const float r2 = dist2(dx, dy, dz);
+__device__ __forceinline__ float gwn_term_r2(float dx, float dy, float dz, float r2, float s, const float4 b,
+                                             float sigma, float sigma_l2e) {
     const float dot = __builtin_fmaf(dz, b.z, __builtin_fmaf(dy, b.y, dx * b.x));
     const float r = fast_sqrt(r2);
     if constexpr (!SCREENED) {
@@ -44,6 +44,12 @@ __device__ __forceinline__ float gwn_term_d(float dx, float dy, float dz, float 
         const float t = (__builtin_fmaf(sigma, rho, 1.0f) * dot) * fast_rcp(den);
         return r > 1e-10f ? t : 0.0f;
     }
+}
+
+template <bool SCREENED>
+__device__ __forceinline__ float gwn_term_d(float dx, float dy, float dz, float s, const float4 b, float sigma,
+                                            float sigma_l2e) {
+    return gwn_term_r2<SCREENED>(dx, dy, dz, dist2(dx, dy, dz), s, b, sigma, sigma_l2e);
 }
 
 template <bool SCREENED>
@@ -185,7 +191,7 @@ void gwn_direct(dwg_ctx* ctx, const float* pts, const float* nrm, const float* a
 // device octree
 // ---------------------------------------------------------------------------
 dwg_octree::~dwg_octree() {
-    for (void* p : {(void*)info, (void*)kid, (void*)cm_r, (void*)cm_lo, (void*)ec, (void*)ec_bf16, (void*)sa, (void*)sb, (void*)perm,
+    for (void* p : {(void*)info, (void*)kid, (void*)cm_r, (void*)cm_lo, (void*)ec, (void*)sa, (void*)sb, (void*)perm,
                     (void*)an64, (void*)cell, (void*)cm64, (void*)payload, (void*)radius})
         if (p) (void)hipFree(p);
 }
@@ -197,7 +203,7 @@ namespace dwg {
 // carries that point's (A·n/4π, A) and (x, y, z, s) instead.
 __global__ void node_ec_kernel(const int4* __restrict__ info, const Double3* __restrict__ scan, int64_t m,
                                const float4* __restrict__ sa, const float4* __restrict__ sb,
-                               float4* __restrict__ ec, uint2* __restrict__ ec_bf16, float4* __restrict__ payload) {
+                               float4* __restrict__ ec, float4* __restrict__ payload) {
     int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i >= m) return;
     const int4 nf = info[i];
@@ -209,15 +215,6 @@ __global__ void node_ec_kernel(const int4* __restrict__ info, const Double3* __r
     } else if (nf.z - nf.y == 1) {
         payload[2 * i] = sb[nf.y];
         payload[2 * i + 1] = sa[nf.y];
-    }
-    if (ec_bf16) {
-        // round-to-nearest-even bf16 of each component
-        auto bf = [](float f) -> uint32_t {
-            uint32_t u = __float_as_uint(f);
-            u += 0x7FFFu + ((u >> 16) & 1u);
-            return u >> 16;
-        };
-        ec_bf16[i] = make_uint2(bf(x) | (bf(y) << 16), bf(z));
     }
 }
 
@@ -244,8 +241,9 @@ void octree_update(dwg_ctx* ctx, dwg_octree* T, const float* nrm, const float* a
     }
     ProfileScope ps(ctx, "octree_ec");
     hipLaunchKernelGGL(node_ec_kernel, dim3(grid_for(m, 256)), dim3(256), 0, ctx->stream, T->info, T->an64, m, T->sa,
-                       T->sb, T->ec, T->ec_bf16, T->payload);
+                       T->sb, T->ec, T->payload);
     check_launch();
+    T->payload_bf16 = false;
     T->updated = true;
 }
 
@@ -417,18 +415,26 @@ __global__ __launch_bounds__(kTreeThreads) void gwn_tree_kernel(TreeArgs T, Quer
             // cm_r.w holds (β·R)², rounded as float(float(β·R)²), for this β
             const float4 cr = crs[c];
             const float dx = cr.x - qx, dy = cr.y - qy, dz = cr.z - qz;
-            const bool far = dist2(dx, dy, dz) > cr.w;
+            const float r2 = dist2(dx, dy, dz);
+            const bool far = r2 > cr.w;
             // lane masks straight from the compare: far for the lanes that
             // opened this node, near ones open the child
             const uint64_t farm = __builtin_amdgcn_ballot_w64(far);
             const uint64_t nm = mask & ~farm;
             if (mask & farm) {
-                float4 e4 = s_pay[wave][2 * c];
-                if (BF16) e4 = make_float4(bf16_round(e4.x), bf16_round(e4.y), bf16_round(e4.z), 0.f);
-                const float4 lo = s_pay[wave][2 * c + 1];
-                // d = (CM_f − q) + δ: the first difference is exact for nearby
-                // values, so d carries a single rounding
-                const float t = gwn_term_d<SCREENED>(dx + lo.x, dy + lo.y, dz + lo.z, kScreenDefault, e4, sigma, sl2e);
+                const float4 e4 = s_pay[wave][2 * c];
+                float t;
+                if (BF16) {
+                    // BF16 moments carry a relative error of 2^-9, far above
+                    // the rounding of the centre: d = CM_f − q and its |d|²
+                    // from the far test serve as they are
+                    t = gwn_term_r2<SCREENED>(dx, dy, dz, r2, kScreenDefault, e4, sigma, sl2e);
+                } else {
+                    // d = (CM_f − q) + δ: the first difference is exact for
+                    // nearby values, so d carries a single rounding
+                    const float4 lo = s_pay[wave][2 * c + 1];
+                    t = gwn_term_d<SCREENED>(dx + lo.x, dy + lo.y, dz + lo.z, kScreenDefault, e4, sigma, sl2e);
+                }
                 // the plain dipole has no r > 1e-10 guard in the reference either
                 part += (act && far) ? t : 0.f;
                 if (COUNT && act && far) nterms += 1;
@@ -490,6 +496,17 @@ __global__ void far_radius_kernel(const float* __restrict__ r, int64_t m, float 
     cm_r[i].w = br * br;
 }
 
+// Far-field moments in the payload of every internal node: EC/4π in FP32,
+// or rounded to BF16 (to nearest even, kept in FP32 words) for BF16
+// evaluations
+__global__ void payload_precision_kernel(const int2* __restrict__ kid, const float4* __restrict__ ec, int64_t m,
+                                         int bf16, float4* __restrict__ payload) {
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= m || kid[i].x <= 0) return;
+    const float4 e = ec[i];
+    payload[2 * i] = bf16 ? make_float4(bf16_round(e.x), bf16_round(e.y), bf16_round(e.z), 0.f) : e;
+}
+
 static void launch_tree(dwg_ctx* ctx, const dwg_octree* Tc, const QueryArgs& Q, float beta, float sigma,
                         uint32_t flags, float* out, int64_t* terms, bool count) {
     DWG_REQUIRE(Tc && Tc->updated, "octree has no moments yet: call dwg_octree_update first");
@@ -500,6 +517,13 @@ static void launch_tree(dwg_ctx* ctx, const dwg_octree* Tc, const QueryArgs& Q, 
                            T->n_nodes, beta, T->cm_r);
         check_launch();
         T->far_beta = beta;
+    }
+    const bool want_bf16 = (flags & DWG_FAR_BF16) && !count;
+    if (T->payload_bf16 != want_bf16) {
+        hipLaunchKernelGGL(payload_precision_kernel, dim3(grid_for(T->n_nodes, 256)), dim3(256), 0, ctx->stream, T->kid,
+                           T->ec, T->n_nodes, int(want_bf16), T->payload);
+        check_launch();
+        T->payload_bf16 = want_bf16;
     }
     const int64_t waves = Q.grid ? Q.n_bricks : (Q.nq + 63) / 64;
     if (waves == 0) return;

@@ -490,7 +490,6 @@ dwg_octree* octree_build(dwg_ctx* ctx, const double* P, const double* A, int64_t
 
         // 4. sources in tree order and the per-iteration buffers
         T->ec = dev_alloc<float4>(m);
-        T->ec_bf16 = dev_alloc<uint2>(m);
         T->sa = dev_alloc<float4>(n);
         T->sb = dev_alloc<float4>(n);
         T->an64 = dev_alloc<Double3>(n + 1);

@@ -37,13 +37,19 @@ struct DevArray {
 
 // Triangle soup on the host (ntri × 9 floats, std::malloc'd so that it can
 // be handed to a dwg_result and released by dwg_free).
+// Host buffers handed to the caller (released with dwg_free): pinned, so
+// the device-to-host copy runs at full rate into pages that are already
+// resident (a fresh pageable buffer costs ~25 ms of page faults per 100 MB).
+float* alloc_host_pinned(size_t bytes);
+void free_host(void* p);  // pinned buffers from alloc_host_pinned, else std::free
+
 struct HostMesh {
     float* data = nullptr;
     int64_t ntri = 0;
     HostMesh() = default;
     HostMesh(const HostMesh&) = delete;
     HostMesh& operator=(const HostMesh&) = delete;
-    ~HostMesh() { std::free(data); }
+    ~HostMesh() { free_host(data); }
     float* release() {
         float* d = data;
         data = nullptr;

@@ -11,9 +11,11 @@
 #include <cstring>
 #include <fstream>
 #include <memory>
+#include <mutex>
 #include <random>
 #include <sstream>
 #include <stdexcept>
+#include <unordered_set>
 
 #include "common.h"
 #include "diffuse.h"
@@ -27,6 +29,30 @@ using Clock = std::chrono::steady_clock;
 
 static double seconds_since(Clock::time_point t) {
     return std::chrono::duration<double>(Clock::now() - t).count();
+}
+
+static std::mutex g_pinned_mu;
+static std::unordered_set<void*> g_pinned;
+
+float* alloc_host_pinned(size_t bytes) {
+    void* p = nullptr;
+    if (hipHostMalloc(&p, bytes, hipHostMallocDefault) != hipSuccess || !p)
+        throw dwg::Error(DWG_ERR_HIP, "pinned host allocation failed");
+    std::lock_guard<std::mutex> lock(g_pinned_mu);
+    g_pinned.insert(p);
+    return static_cast<float*>(p);
+}
+
+void free_host(void* p) {
+    if (!p) return;
+    {
+        std::lock_guard<std::mutex> lock(g_pinned_mu);
+        if (g_pinned.erase(p)) {
+            (void)hipHostFree(p);
+            return;
+        }
+    }
+    std::free(p);
 }
 
 template <class T>
@@ -376,9 +402,9 @@ void TurboDWG::mesh_pass(const Grid& grid, float iso_scale, int64_t* tri_count, 
         DevArray<float> t(9 * std::max<int64_t>(ntri, 1));
         check(dwg_marching_cubes(ctx_, wg.ptr, &g, iso, ntri, t.ptr, nullptr, nullptr, &ntri), "marching cubes");
         // straight into the buffer the caller receives (no second host copy)
-        std::free(tris->data);
-        tris->data = static_cast<float*>(std::malloc(sizeof(float) * 9 * size_t(std::max<int64_t>(ntri, 1))));
-        if (!tris->data) throw dwg::Error(DWG_ERR_ARG, "host allocation of the mesh failed");
+        free_host(tris->data);
+        tris->data = nullptr;
+        tris->data = alloc_host_pinned(sizeof(float) * 9 * size_t(std::max<int64_t>(ntri, 1)));
         tris->ntri = ntri;
         t.download(tris->data, 9 * ntri, ctx_->stream);
     }
@@ -658,7 +684,7 @@ void dwg_options_default(dwg_options* o) {
     o->mlp = nullptr;
 }
 
-void dwg_free(void* p) { std::free(p); }
+void dwg_free(void* p) { dwg_host::free_host(p); }
 
 int dwg_launch(dwg_ctx* ctx, const float* h_pts, const float* h_nrm, int64_t n, const dwg_options* opt,
                const char* out_base, dwg_result* res) {

@@ -8,6 +8,7 @@ default; the MI355X-only options are ``source_mode``, ``leaf_size``,
 from __future__ import annotations
 
 import ctypes
+import weakref
 from dataclasses import dataclass, field
 
 import numpy as np
@@ -58,6 +59,18 @@ def options(**kw) -> DwgOptions:
     return o
 
 
+def _host_array(L, ptr, ntri):
+    """The library's mesh buffer as a (T, 3, 3) array without a copy; the
+    buffer goes back to dwg_free when the array is collected."""
+    if not ptr or ntri == 0:
+        if ptr:
+            L.dwg_free(ptr)
+        return np.zeros((0, 3, 3), np.float32)
+    raw = (ctypes.c_float * (9 * ntri)).from_address(ptr)
+    weakref.finalize(raw, L.dwg_free, ptr)
+    return np.frombuffer(raw, np.float32).reshape(ntri, 3, 3)
+
+
 def reconstruct(ctx, points, normals=None, want_mesh=True, out_base=None, mlp=None, **kw) -> Reconstruction:
     """Run DWG on host points (N×3); ``ctx`` is a dwg_amd.gpu.Context."""
     L = _lib.load()
@@ -80,12 +93,7 @@ def reconstruct(ctx, points, normals=None, want_mesh=True, out_base=None, mlp=No
                            None if out_base is None else out_base.encode(), ctypes.byref(r)))
     tris = None
     if want_mesh:
-        T = int(r.tri_count)
-        tris = np.zeros((T, 3, 3), np.float32)
-        if T and r.mesh:
-            ctypes.memmove(tris.ctypes.data, r.mesh, 36 * T)
-        if r.mesh:
-            L.dwg_free(r.mesh)
+        tris = _host_array(L, r.mesh, int(r.tri_count))
     return Reconstruction(out_n, tris, int(r.n_sources), np.array(r.center[:]), float(r.scale), r.time_setup_s,
                           r.time_iter_s, r.time_final_s, r.time_total_s, wm[: o.total_iter], it_t[: o.total_iter],
                           it_z[: o.total_iter], int(r.terms_src), int(r.terms_grid), int(r.grid_points))

@@ -267,8 +267,8 @@ void dwg_options_default(dwg_options* opt);
 typedef struct dwg_result {
     float* normals;        /* in: n×3 buffer or NULL; out: final point normals (normalised frame) */
     int32_t want_mesh;     /* in: 1 to receive the final mesh in `mesh` */
-    float* mesh;           /* out: tri_count×9 floats (normalised frame), allocated by the
-                              library; release with dwg_free() */
+    float* mesh;           /* out: tri_count×9 floats (normalised frame) in pinned host
+                              memory allocated by the library; release with dwg_free() */
     int64_t tri_count;     /* out: triangles of the final mesh */
     int64_t n_sources;     /* out: sources of the iterations */
     double center[3];      /* out: input normalisation, p_norm = (p − center)/scale */

@@ -12,7 +12,7 @@ import numpy as np
 import pytest
 import torch
 
-from dwg_amd import gpu, pipeline, shapes
+from dwg_amd import gpu, metrics, pipeline, shapes
 from oracle import mlp as M
 
 pytestmark = pytest.mark.gpu
@@ -198,3 +198,24 @@ def test_learned_init_orientation(ctx):
     n = net.init_normals(P, c).cpu().numpy()
     assert np.allclose(np.linalg.norm(n, axis=1), 1.0, atol=1e-5)
     assert np.all((n * (P - c)).sum(1) >= -1e-7)
+
+
+def test_trained_network_gain(ctx):
+    # What the shipped network gains over the start without it, on a
+    # held-out union shape (training used seeds 1000-1511): its normal lines
+    # are far closer to the true ones than the Gauss map's, and after the
+    # same few DWG iterations the learned start is ahead
+    from bench import WEIGHTS
+    P, N = shapes.union(200_000, 32, 2001)
+    net = gpu.Mlp(ctx, path=WEIGHTS)
+    c = (P.min(0) + P.max(0)) / 2
+    nl = net.init_normals(P, c).cpu().numpy()
+    ng = gpu.gauss_map(ctx, P, c).cpu().numpy()
+    line = lambda n: np.median(metrics.normal_angle_deg(n, N, oriented=False))
+    assert line(nl) < 0.5 * line(ng), (line(nl), line(ng))
+    base = dict(grid_reso=192, final_reso=192, total_iter=6, source_mode=0)
+    rl = pipeline.reconstruct(ctx, P, mlp=net, input_normal=4, **base)
+    rg = pipeline.reconstruct(ctx, P, input_normal=3, **base)
+    al = metrics.normal_angle_deg(rl.normals, N).mean()
+    ag = metrics.normal_angle_deg(rg.normals, N).mean()
+    assert al < ag, (al, ag)
