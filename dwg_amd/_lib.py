@@ -50,7 +50,8 @@ class DwgOptions(ctypes.Structure):
                 ("total_iter", ctypes.c_int32), ("test_iter", ctypes.c_int32), ("output_mode", ctypes.c_int32),
                 ("source_mode", ctypes.c_int32), ("leaf_size", ctypes.c_int32), ("beta", ctypes.c_float),
                 ("precision", ctypes.c_uint32), ("seed", ctypes.c_uint64), ("normalize_input", ctypes.c_int32),
-                ("verbose", ctypes.c_int32), ("count_terms", ctypes.c_int32), ("mlp", ctypes.c_void_p)]
+                ("verbose", ctypes.c_int32), ("count_terms", ctypes.c_int32), ("mlp", ctypes.c_void_p),
+                ("sigma_warm_iters", ctypes.c_int32)]
 
 
 class DwgResult(ctypes.Structure):

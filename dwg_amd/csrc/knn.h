@@ -15,6 +15,9 @@ struct KnnGridArgs {
     float cell, inv_cell;
     int dim[3];
 };
+
+// levels of the fine level's occupancy pyramid (up to 2^15 cells per axis)
+constexpr int kMaxPyr = 16;
 }  // namespace dwg
 
 // Two cell grids over the same points: a fine one sized from the measured
@@ -28,6 +31,16 @@ struct dwg_knn_index {
         int64_t* cell_start = nullptr;  // ncells + 1
         float4* pts = nullptr;          // (x, y, z, index bits) bucketed by cell
         int* empty = nullptr;           // coarse level: Chebyshev distance (cells) to the nearest occupied cell
+        // fine level: occupancy pyramid.  Level l ≥ 1 has one byte per
+        // block of 2^l cells per axis, bit c set when child block c (octant
+        // bits x, y, z) of level l − 1 holds a point; level 0 are the cells.
+        uint8_t* pyr = nullptr;
+        int4* pyr_dim = nullptr;        // (blocks per axis, 0) of every level
+        int64_t* pyr_off = nullptr;     // byte offset of every level in pyr
+        int4 pyr_dim_h[dwg::kMaxPyr];   // host copies (source of the uploads)
+        int64_t pyr_off_h[dwg::kMaxPyr];
+        int pyr_levels = 0;             // 0: no pyramid (grid too elongated)
+        float pyr_margin = 0.f;         // rounding allowance of cell boundaries
     } lvl[2];
     ~dwg_knn_index();
 };

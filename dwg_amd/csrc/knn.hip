@@ -20,6 +20,9 @@ dwg_knn_index::~dwg_knn_index() {
         if (L.cell_start) (void)hipFree(L.cell_start);
         if (L.pts) (void)hipFree(L.pts);
         if (L.empty) (void)hipFree(L.empty);
+        if (L.pyr) (void)hipFree(L.pyr);
+        if (L.pyr_dim) (void)hipFree(L.pyr_dim);
+        if (L.pyr_off) (void)hipFree(L.pyr_off);
     }
 }
 
@@ -195,6 +198,80 @@ static void empty_distance(dwg_ctx* ctx, dwg_knn_index::Level& L) {
     check_launch();
 }
 
+// Occupancy pyramid of a level: byte b of level l ≥ 1 has bit c set when
+// child c (x + 2y + 4z) of block b, at level l − 1, holds a point.
+__global__ void pyr_first_kernel(const int64_t* __restrict__ cell_start, int dx, int dy, int dz, int ux, int uy, int uz,
+                                 uint8_t* __restrict__ out) {
+    const int64_t b = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (b >= int64_t(ux) * uy * uz) return;
+    const int x = int(b % ux), y = int((b / ux) % uy), z = int(b / (int64_t(ux) * uy));
+    uint32_t m = 0;
+    for (int c = 0; c < 8; ++c) {
+        const int X = 2 * x + (c & 1), Y = 2 * y + ((c >> 1) & 1), Z = 2 * z + (c >> 2);
+        if (X < dx && Y < dy && Z < dz) {
+            const int64_t cell = X + int64_t(dx) * (Y + int64_t(dy) * Z);
+            if (cell_start[cell + 1] > cell_start[cell]) m |= 1u << c;
+        }
+    }
+    out[b] = uint8_t(m);
+}
+
+__global__ void pyr_up_kernel(const uint8_t* __restrict__ lower, int dx, int dy, int dz, int ux, int uy, int uz,
+                              uint8_t* __restrict__ out) {
+    const int64_t b = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (b >= int64_t(ux) * uy * uz) return;
+    const int x = int(b % ux), y = int((b / ux) % uy), z = int(b / (int64_t(ux) * uy));
+    uint32_t m = 0;
+    for (int c = 0; c < 8; ++c) {
+        const int X = 2 * x + (c & 1), Y = 2 * y + ((c >> 1) & 1), Z = 2 * z + (c >> 2);
+        if (X < dx && Y < dy && Z < dz && lower[X + int64_t(dx) * (Y + int64_t(dy) * Z)]) m |= 1u << c;
+    }
+    out[b] = uint8_t(m);
+}
+
+static void pyramid(dwg_ctx* ctx, dwg_knn_index::Level& L) {
+    int4* dims = L.pyr_dim_h;
+    int64_t* off = L.pyr_off_h;
+    int d[3] = {L.G.dim[0], L.G.dim[1], L.G.dim[2]};
+    dims[0] = make_int4(d[0], d[1], d[2], 0);
+    off[0] = 0;
+    int levels = 1;
+    int64_t total = 0;
+    while (d[0] > 1 || d[1] > 1 || d[2] > 1) {
+        if (levels == kMaxPyr) return;  // no pyramid: queries take the shell search
+        for (int a = 0; a < 3; ++a) d[a] = (d[a] + 1) / 2;
+        dims[levels] = make_int4(d[0], d[1], d[2], 0);
+        off[levels] = total;
+        total += int64_t(d[0]) * d[1] * d[2];
+        ++levels;
+    }
+    if (levels == 1) return;  // one cell: the shell search is already optimal
+    DWG_HIP_CHECK(hipMalloc(&L.pyr, size_t(total)));
+    DWG_HIP_CHECK(hipMalloc(&L.pyr_dim, sizeof(int4) * kMaxPyr));
+    DWG_HIP_CHECK(hipMalloc(&L.pyr_off, sizeof(int64_t) * kMaxPyr));
+    DWG_HIP_CHECK(hipMemcpyAsync(L.pyr_dim, dims, sizeof(int4) * levels, hipMemcpyHostToDevice, ctx->stream));
+    DWG_HIP_CHECK(hipMemcpyAsync(L.pyr_off, off, sizeof(int64_t) * levels, hipMemcpyHostToDevice, ctx->stream));
+    for (int l = 1; l < levels; ++l) {
+        const int4 lo = dims[l - 1], up = dims[l];
+        const int64_t nb = int64_t(up.x) * up.y * up.z;
+        if (l == 1)
+            hipLaunchKernelGGL(pyr_first_kernel, dim3(grid_for(nb, 256)), dim3(256), 0, ctx->stream, L.cell_start, lo.x,
+                               lo.y, lo.z, up.x, up.y, up.z, L.pyr + off[1]);
+        else
+            hipLaunchKernelGGL(pyr_up_kernel, dim3(grid_for(nb, 256)), dim3(256), 0, ctx->stream, L.pyr + off[l - 1],
+                               lo.x, lo.y, lo.z, up.x, up.y, up.z, L.pyr + off[l]);
+        check_launch();
+    }
+    float ext = 0.f;
+    for (int a = 0; a < 3; ++a)
+        ext = std::max({ext, std::fabs(L.G.lo[a]), std::fabs(L.G.lo[a] + L.G.cell * float(L.G.dim[a]))});
+    // cell boundaries are computed as lo + b·w in float and points are
+    // assigned by floor((v − lo)/cell): both are within a few ulp of the
+    // exact boundary; 16 ulp of the largest coordinate plus 1e-5 cell
+    L.pyr_margin = 16.f * 1.1920929e-7f * ext + 1e-5f * L.G.cell;
+    L.pyr_levels = levels;
+}
+
 dwg_knn_index* knn_build(dwg_ctx* ctx, const float* p, int64_t n, int k_hint) {
     DWG_REQUIRE(n > 0 && n < (int64_t(1) << 31) - 1, "kNN index needs 1 ≤ n < 2^31 points");
     float lo[3], hi[3];
@@ -233,6 +310,7 @@ dwg_knn_index* knn_build(dwg_ctx* ctx, const float* p, int64_t n, int k_hint) {
         I->lvl[1].G = grid_args(lo, hi, cell * 8.0);
         bucket(ctx, p, n, I->lvl[1], true);
         empty_distance(ctx, I->lvl[1]);
+        pyramid(ctx, I->lvl[0]);
     } catch (...) {
         delete I;
         throw;
@@ -251,6 +329,7 @@ dwg_knn_index* knn_build_like(dwg_ctx* ctx, const dwg_knn_index* like, const flo
             bucket(ctx, p, n, I->lvl[l], true);
         }
         empty_distance(ctx, I->lvl[1]);
+        pyramid(ctx, I->lvl[0]);
     } catch (...) {
         delete I;
         throw;
@@ -805,6 +884,137 @@ static void launch_wave(dwg_ctx* ctx, const dwg_knn_index* I, const float* q, in
     check_launch();
 }
 
+// Exact search over the occupancy pyramid of the fine level, one query per
+// lane: a depth-first walk from the top block that always descends into the
+// nearest remaining child block and drops every block whose box is farther
+// than the current k-th key.  A query far from every point reaches its
+// neighbours in a few hundred short steps, where a shell search visits
+// every cell of the shells in between.  The stack holds one entry per level
+// in LDS: block coordinates and the mask of children not yet visited.
+struct PyrArgs {
+    const uint8_t* mask;
+    const int4* dim;
+    const int64_t* off;
+    int levels;
+    float margin;
+};
+
+__device__ __forceinline__ uint64_t pyr_entry(int x, int y, int z, uint32_t m) {
+    return uint64_t(uint32_t(x)) | (uint64_t(uint32_t(y)) << 16) | (uint64_t(uint32_t(z)) << 32) | (uint64_t(m) << 48);
+}
+
+// squared distance from q to the box of cells [b·2^l, (b+1)·2^l) along each
+// axis, each axis gap shrunk by the rounding margin
+__device__ __forceinline__ float pyr_box_d2(const KnnGridArgs& G, float w, float margin, float qx, float qy, float qz,
+                                            int x, int y, int z) {
+    auto gap = [&](float v, float lo, int b) {
+        const float a = __builtin_fmaf(float(b), w, lo);
+        return fmaxf(fmaxf(a - v, v - (a + w)) - margin, 0.f);
+    };
+    const float gx = gap(qx, G.lo[0], x), gy = gap(qy, G.lo[1], y), gz = gap(qz, G.lo[2], z);
+    return __builtin_fmaf(gz, gz, __builtin_fmaf(gy, gy, gx * gx));
+}
+
+constexpr int kPyrThreads = 256;
+
+template <int K>
+__global__ __launch_bounds__(kPyrThreads) void knn_pyramid_kernel(KnnGridArgs G, PyrArgs Y,
+                                                                  const int64_t* __restrict__ cell_start,
+                                                                  const float4* __restrict__ pts,
+                                                                  const float* __restrict__ q, int64_t nq,
+                                                                  const int* __restrict__ qlist, int k,
+                                                                  const uint8_t* __restrict__ skip,
+                                                                  int32_t* __restrict__ out_idx,
+                                                                  float* __restrict__ out_d2) {
+    extern __shared__ uint64_t stk[];  // Y.levels × kPyrThreads entries
+    __shared__ int4 s_dim[kMaxPyr];
+    __shared__ int64_t s_off[kMaxPyr];
+    if (threadIdx.x < Y.levels) {
+        s_dim[threadIdx.x] = Y.dim[threadIdx.x];
+        s_off[threadIdx.x] = Y.off[threadIdx.x];
+    }
+    __syncthreads();
+    const int64_t t = int64_t(blockIdx.x) * kPyrThreads + threadIdx.x;
+    if (t >= nq) return;
+    const int64_t i = qlist ? int64_t(qlist[t]) : t;
+    const float qx = q[3 * i], qy = q[3 * i + 1], qz = q[3 * i + 2];
+    uint64_t best[K];
+    knn_reset<K>(best, k);
+    uint64_t* st = stk + threadIdx.x;
+    const int top = Y.levels - 1;
+    st[top * kPyrThreads] = pyr_entry(0, 0, 0, Y.mask[s_off[top]]);
+    int l = top;
+    while (l <= top) {
+        const uint64_t e = st[l * kPyrThreads];
+        const int bx = int(e & 0xFFFF), by = int((e >> 16) & 0xFFFF), bz = int((e >> 32) & 0xFFFF);
+        uint32_t m = uint32_t(e >> 48);
+        // the nearest child block not yet visited
+        const float w = G.cell * float(1 << (l - 1));
+        float bd = INFINITY;
+        int bc = -1;
+        for (uint32_t r = m; r; r &= r - 1) {
+            const int c = __builtin_ctz(r);
+            const float d = pyr_box_d2(G, w, Y.margin, qx, qy, qz, 2 * bx + (c & 1), 2 * by + ((c >> 1) & 1),
+                                       2 * bz + (c >> 2));
+            if (d < bd) {
+                bd = d;
+                bc = c;
+            }
+        }
+        const float kth = best[0] == ~0ull ? INFINITY : __uint_as_float(uint32_t(best[0] >> 32));
+        if (bc < 0 || !(bd <= kth)) {  // nothing left here can hold a closer point
+            ++l;
+            continue;
+        }
+        m &= ~(1u << bc);
+        st[l * kPyrThreads] = pyr_entry(bx, by, bz, m);
+        const int x = 2 * bx + (bc & 1), y = 2 * by + ((bc >> 1) & 1), z = 2 * bz + (bc >> 2);
+        if (l == 1) {  // a cell: its points
+            const int64_t c = x + int64_t(G.dim[0]) * (y + int64_t(G.dim[1]) * z);
+            const int64_t se = cell_start[c + 1];
+            for (int64_t s = cell_start[c]; s < se; ++s) {
+                const float4 p = pts[s];
+                const int idx = __float_as_int(p.w);
+                if (skip && skip[idx]) continue;
+                const float d2 = dist2(qx - p.x, qy - p.y, qz - p.z);
+                const uint64_t key = (uint64_t(__float_as_uint(d2)) << 32) | uint32_t(idx);
+                if (key < best[0]) knn_insert<K>(best, key);
+            }
+        } else {
+            const int4 cd = s_dim[l - 1];
+            --l;
+            st[l * kPyrThreads] = pyr_entry(x, y, z, Y.mask[s_off[l] + x + int64_t(cd.x) * (y + int64_t(cd.y) * z)]);
+        }
+    }
+    knn_store<K>(best, k, i, out_idx, out_d2);
+}
+
+// far queries: the pyramid walk when there are enough of them to fill the
+// lanes, else (or without a pyramid) one wave per query on the coarse level
+constexpr int64_t kPyrMinQueries = 2048;
+
+static void launch_far(dwg_ctx* ctx, const dwg_knn_index* I, const float* q, int64_t nq, const int* qlist, int k,
+                       const uint8_t* skip, int32_t* out_idx, float* out_d2) {
+    if (nq == 0) return;
+    const auto& L = I->lvl[0];
+    if (L.pyr_levels < 2 || nq < kPyrMinQueries) {
+        launch_wave(ctx, I, q, nq, qlist, k, skip, out_idx, out_d2);
+        return;
+    }
+    const PyrArgs Y{L.pyr, L.pyr_dim, L.pyr_off, L.pyr_levels, L.pyr_margin};
+    dim3 grid(grid_for(nq, kPyrThreads)), block(kPyrThreads);
+    const size_t lds = sizeof(uint64_t) * size_t(L.pyr_levels) * kPyrThreads;
+#define DWG_KNNP(KK)                                                                                             \
+    hipLaunchKernelGGL(knn_pyramid_kernel<KK>, grid, block, lds, ctx->stream, L.G, Y, L.cell_start, L.pts, q, nq, \
+                       qlist, k, skip, out_idx, out_d2)
+    if (k <= 8) DWG_KNNP(8);
+    else if (k <= 10) DWG_KNNP(10);
+    else if (k <= 16) DWG_KNNP(16);
+    else DWG_KNNP(32);
+#undef DWG_KNNP
+    check_launch();
+}
+
 static void launch_per_lane(dwg_ctx* ctx, const dwg_knn_index* I, const float* q, int64_t nq, const int* qlist, int k,
                             const uint8_t* skip, const uint8_t* qmask, int32_t* out_idx, float* out_d2) {
     if (nq == 0) return;
@@ -917,7 +1127,7 @@ void knn_query(dwg_ctx* ctx, const dwg_knn_index* I, const float* q, int64_t nq,
     {
         const std::string name = base + ":redo";
         ProfileScope pr(ctx, name.c_str());
-        launch_wave(ctx, I, q, n3, redo3, k, skip, out_idx, out_d2);
+        launch_far(ctx, I, q, n3, redo3, k, skip, out_idx, out_d2);
     }
     if (sc && n1 > 0) {  // the queries the first pass did not close scatter now
         const std::string name = base + ":scatter";

@@ -285,10 +285,10 @@ void TurboDWG::download_normals(std::vector<float>& out) const {
 
 void TurboDWG::set_normals(const std::vector<float>& n) { nrm_.upload(n.data(), 3 * n_, ctx_->stream); }
 
-static float sigma_schedule(int it, float sigma_in) {
-    // TurboDWG.cu:526-533
+static float sigma_schedule(int it, float sigma_in, int warm_iters) {
+    // TurboDWG.cu:526-533 (warm_iters = 4 there)
     float s = sigma_in;
-    if (it < 4) s = 20.0f;
+    if (it < warm_iters) s = 20.0f;
     if (it > 0 && it % 7 == 0) s = 10.0f;
     return s;
 }
@@ -313,7 +313,7 @@ double TurboDWG::TurboScreenedDWGLaunch(int total_iters, double* w_mean, int64_t
     }
     int64_t capacity = std::max<int64_t>(cent_.count / 3, 1 << 20);
     for (int it = 0; it < total_iters; ++it) {
-        const float sigma = sigma_schedule(it, opt_.sigma);
+        const float sigma = sigma_schedule(it, opt_.sigma, opt_.sigma_warm_iters);
         const float* rad = opt_.use_radii ? radii_.ptr : nullptr;
         check(dwg_octree_update(ctx_, tree_, nrm_.ptr, area_.ptr, rad), "moments");
         if (opt_.beta > 0) {
@@ -682,6 +682,7 @@ void dwg_options_default(dwg_options* o) {
     o->verbose = 0;
     o->count_terms = 0;
     o->mlp = nullptr;
+    o->sigma_warm_iters = 4;
 }
 
 void dwg_free(void* p) { dwg_host::free_host(p); }

@@ -260,6 +260,7 @@ typedef struct dwg_options {
     int32_t verbose;             /* progress lines on stdout */
     int32_t count_terms;         /* 1: count the Barnes-Hut terms of one iteration (untimed) */
     const dwg_mlp* mlp;          /* weights for input_normal 4 */
+    int32_t sigma_warm_iters;    /* first iterations at σ = 20 (TurboDWG.cu:526-528), 4 */
 } dwg_options;
 
 void dwg_options_default(dwg_options* opt);

@@ -75,7 +75,12 @@ def test_knn_far_queries(ctx):
                         rng.normal(0, 0.01, (2000, 3)) + [0.0, 0.1, 0.0]]).astype(np.float32)
     for k in (10, 15):
         idx = gpu.KnnIndex(ctx, P, k_hint=k)
+        ctx.profile_reset()
+        ctx.profile(True)
         got, d2 = idx.query(Q, k, d2=True)
+        ctx.profile(False)
+        # enough far queries to take the pyramid walk (≥ 2048 after the tiles)
+        assert ctx.counter("knn_query:tiles3:redo") >= 2048
         ref, rd2 = O.knn_brute(P, Q, k, return_d2=True)
         assert np.array_equal(got.cpu().numpy(), ref)
         assert np.array_equal(d2.cpu().numpy(), rd2)
