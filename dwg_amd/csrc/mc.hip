@@ -3,13 +3,14 @@
 // The reference launches one 12-thread block per cube and appends triangles
 // through a global atomic counter into a buffer of max(R³/2, 1e6) triangles
 // (TurboMC.cu:383-478, 572-638), so only 12 of 64 lanes work and the output
-// order is unordered.  Here one lane owns one cube (x fastest, coalesced
-// field reads) and the launch runs twice: a count pass writes one triangle
-// count per 256-cube tile, a scan turns tile counts into offsets, and an
-// emit pass recomputes the cube case, takes its in-tile offset from a block
-// prefix sum and writes triangles in cube order.  The result is exact-size
-// and deterministic.  Corner convention, edge interpolation, table and the
-// negated face normal follow TurboMC.cu:399-470.
+// order is unordered.  Here one lane owns one cube, a block a brick of
+// 8 × 8 × 4 cubes, and the launch runs twice: a count pass writes one
+// triangle count per brick, a scan turns brick counts into offsets, and an
+// emit pass skips the bricks without triangles, recomputes the cube case of
+// the others, takes its in-brick offset from a block prefix sum (wavefront
+// scans) and writes triangles in brick order, cube order inside a brick.
+// The result is exact-size and deterministic.  Corner convention, edge
+// interpolation, table and the negated face normal follow TurboMC.cu:399-470.
 #include "mc.h"
 #include "mc_table.h"
 #include "scan.h"
@@ -46,45 +47,54 @@ __device__ __forceinline__ int cube_case(const McArgs& A, int x, int y, int z, f
     return c;
 }
 
-__device__ __forceinline__ void cube_of(const McArgs& A, int64_t id, int& x, int& y, int& z) {
-    const int64_t nx = A.rx - 1, ny = A.ry - 1;
-    x = int(id % nx);
-    const int64_t t = id / nx;
-    y = int(t % ny);
-    z = int(t / ny);
+// One block per brick of 8 × 8 × 4 cubes (x fastest inside the brick,
+// bricks in x, y, z order): a surface crosses few bricks, so the emit pass
+// returns at once from the many bricks without triangles.
+constexpr int kBx = 8, kBy = 8, kBz = 4;
+static_assert(kBx * kBy * kBz == kMcThreads, "one cube per thread");
+
+struct McBricks {
+    int nbx, nby, nbz;
+};
+
+__device__ __forceinline__ bool cube_of(const McArgs& A, const McBricks& B, int& x, int& y, int& z) {
+    const int64_t b = blockIdx.x;
+    x = int(b % B.nbx) * kBx + (threadIdx.x & (kBx - 1));
+    y = int((b / B.nbx) % B.nby) * kBy + ((threadIdx.x / kBx) & (kBy - 1));
+    z = int(b / (int64_t(B.nbx) * B.nby)) * kBz + threadIdx.x / (kBx * kBy);
+    return x < A.rx - 1 && y < A.ry - 1 && z < A.rz - 1;
 }
 
+__device__ __forceinline__ int wave_inclusive_sum(int v) {
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const int y = __shfl_up(v, off, 64);
+        if (lane >= off) v += y;
+    }
+    return v;
+}
+
+// exclusive prefix of v over the block and the block total
 __device__ int block_exclusive_sum(int v, int& total) {
-    __shared__ int lds[kMcThreads];
     __shared__ int wsum[kMcThreads / 64];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    int x = v;
-    lds[threadIdx.x] = x;
-    __syncthreads();
-    for (int off = 1; off < 64; off <<= 1) {
-        const int y = lane >= off ? lds[threadIdx.x - off] : 0;
-        __syncthreads();
-        x += y;
-        lds[threadIdx.x] = x;
-        __syncthreads();
-    }
-    if (lane == 63) wsum[wave] = x;
+    const int incl = wave_inclusive_sum(v);
+    if (lane == 63) wsum[wave] = incl;
     __syncthreads();
     int before = 0;
     total = 0;
+#pragma unroll
     for (int w = 0; w < kMcThreads / 64; ++w) {
-        if (w < wave) before += wsum[w];
+        before += w < wave ? wsum[w] : 0;
         total += wsum[w];
     }
-    return before + x - v;
+    return before + incl - v;
 }
 
-__global__ __launch_bounds__(kMcThreads) void mc_count_kernel(McArgs A, int64_t* __restrict__ tile_counts) {
-    const int64_t id = int64_t(blockIdx.x) * kMcThreads + threadIdx.x;
-    int cnt = 0;
-    if (id < A.ncubes) {
-        int x, y, z;
-        cube_of(A, id, x, y, z);
+__global__ __launch_bounds__(kMcThreads) void mc_count_kernel(McArgs A, McBricks B, int64_t* __restrict__ tile_counts) {
+    int x, y, z, cnt = 0;
+    if (cube_of(A, B, x, y, z)) {
         float v[8];
         cnt = int(c_mc_rows[cube_case(A, x, y, z, v)] >> 60);
     }
@@ -93,14 +103,13 @@ __global__ __launch_bounds__(kMcThreads) void mc_count_kernel(McArgs A, int64_t*
     if (threadIdx.x == 0) tile_counts[blockIdx.x] = total;
 }
 
-__global__ __launch_bounds__(kMcThreads) void mc_emit_kernel(McArgs A, const int64_t* __restrict__ tile_offsets,
+__global__ __launch_bounds__(kMcThreads) void mc_emit_kernel(McArgs A, McBricks B, const int64_t* __restrict__ tile_offsets,
                                                              float* __restrict__ tris, float* __restrict__ cent,
                                                              float* __restrict__ nrm) {
-    const int64_t id = int64_t(blockIdx.x) * kMcThreads + threadIdx.x;
-    int cnt = 0, x = 0, y = 0, z = 0, cs = 0;
+    if (tile_offsets[blockIdx.x + 1] == tile_offsets[blockIdx.x]) return;  // no triangle in this brick
+    int x, y, z, cnt = 0, cs = 0;
     float v[8];
-    if (id < A.ncubes) {
-        cube_of(A, id, x, y, z);
+    if (cube_of(A, B, x, y, z)) {
         cs = cube_case(A, x, y, z, v);
         cnt = int(c_mc_rows[cs] >> 60);
     }
@@ -153,12 +162,13 @@ int64_t marching_cubes(dwg_ctx* ctx, const float* field, const dwg_grid* g, floa
     ensure_table(ctx);
     McArgs A{field, g->res[0], g->res[1], g->res[2], 0, g->origin[0], g->origin[1], g->origin[2], g->h, iso};
     A.ncubes = int64_t(A.rx - 1) * (A.ry - 1) * (A.rz - 1);
-    const int64_t tiles = (A.ncubes + kMcThreads - 1) / kMcThreads;
+    const McBricks B{(A.rx - 1 + kBx - 1) / kBx, (A.ry - 1 + kBy - 1) / kBy, (A.rz - 1 + kBz - 1) / kBz};
+    const int64_t tiles = int64_t(B.nbx) * B.nby * B.nbz;
+    DWG_REQUIRE(tiles < (int64_t(1) << 31), "grid too large for the marching-cubes launch");
     int64_t* offs = static_cast<int64_t*>(ctx->buf("mc_tiles", sizeof(int64_t) * (tiles + 1)));
     {
         ProfileScope ps(ctx, "mc_count");
-        hipLaunchKernelGGL(mc_count_kernel, dim3(grid_for(A.ncubes, kMcThreads)), dim3(kMcThreads), 0, ctx->stream, A,
-                           offs);
+        hipLaunchKernelGGL(mc_count_kernel, dim3(unsigned(tiles)), dim3(kMcThreads), 0, ctx->stream, A, B, offs);
         check_launch();
         exclusive_scan<int64_t>(ctx, offs, tiles);
     }
@@ -166,8 +176,8 @@ int64_t marching_cubes(dwg_ctx* ctx, const float* field, const dwg_grid* g, floa
     if (total > capacity) throw Error(DWG_ERR_CAPACITY, "marching cubes output exceeds capacity", total);
     if (total > 0 && (tris || cent || nrm)) {
         ProfileScope ps(ctx, "mc_emit");
-        hipLaunchKernelGGL(mc_emit_kernel, dim3(grid_for(A.ncubes, kMcThreads)), dim3(kMcThreads), 0, ctx->stream, A,
-                           offs, tris, cent, nrm);
+        hipLaunchKernelGGL(mc_emit_kernel, dim3(unsigned(tiles)), dim3(kMcThreads), 0, ctx->stream, A, B, offs, tris,
+                           cent, nrm);
         check_launch();
     }
     return total;

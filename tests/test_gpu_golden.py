@@ -11,6 +11,7 @@ import pytest
 import torch
 
 from dwg_amd import gpu, pipeline
+from mc_order import to_brick_order
 
 pytestmark = pytest.mark.gpu
 
@@ -54,10 +55,11 @@ def test_knn_and_radii(ctx):
 def test_marching_cubes(ctx):
     grid = gpu.make_grid(G["mc_origin"], float(G["mc_h"]), G["mc_res"])
     tris, cent, nrm = gpu.marching_cubes(ctx, torch.from_numpy(G["mc_field"]), grid, float(G["mc_iso"]), tris=True)
+    order = to_brick_order(G["mc_field"], G["mc_res"], np.float32(G["mc_iso"]))
     assert tris.shape[0] == len(G["mc_tris"])
-    assert np.abs(tris.cpu().numpy() - G["mc_tris"]).max() < 2e-5
-    assert np.abs(cent.cpu().numpy() - G["mc_centroids"]).max() < 2e-5
-    assert np.abs(nrm.cpu().numpy() - G["mc_normals"]).max() < 2e-3
+    assert np.abs(tris.cpu().numpy() - G["mc_tris"][order]).max() < 2e-5
+    assert np.abs(cent.cpu().numpy() - G["mc_centroids"][order]).max() < 2e-5
+    assert np.abs(nrm.cpu().numpy() - G["mc_normals"][order]).max() < 2e-3
 
 
 def test_gauss_map(ctx):

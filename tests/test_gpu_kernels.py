@@ -10,6 +10,7 @@ import pytest
 import torch
 
 import oracle as O
+from mc_order import to_brick_order
 from dwg_amd import gpu, shapes
 
 pytestmark = pytest.mark.gpu
@@ -177,6 +178,8 @@ def test_marching_cubes_matches_oracle(ctx):
         tris, cent, nrm = gpu.marching_cubes(ctx, torch.from_numpy(f), grid, iso, tris=True)
         G32 = _grid_points_f32(grid.origin, grid.h, res).astype(np.float64)
         rt, rc, rn = O.marching_cubes(f.astype(np.float64), res, np.asarray(grid.origin, np.float64), float(np.float32(h)), iso)
+        order = to_brick_order(f, res, np.float32(iso))
+        rt, rc, rn = rt[order], rc[order], rn[order]
         assert len(rt) == tris.shape[0] > 0
         assert np.abs(tris.cpu().numpy() - rt).max() < 2e-5
         assert np.abs(cent.cpu().numpy() - rc).max() < 2e-5
