@@ -77,6 +77,15 @@ class _Prim:
             return 4 * np.pi ** 2 * p[0] * p[1]
         return 4 * np.pi * (p[0] ** 2 + (p[0] - p[1]) ** 2)
 
+    def bound(self):
+        """Radius of a ball about the centre that holds the solid."""
+        p = self.params
+        if self.kind == 1:
+            return float(max(p))
+        if self.kind == 2:
+            return float(p[0] + p[1])
+        return float(p[0])
+
     def to_local(self, x):
         return (x - self.ctr) @ self.rot  # rot columns are the local axes
 
@@ -177,6 +186,12 @@ def union_of(prims, n, rng, noise):
     rejected; then Gaussian noise of `noise`·bbox-diagonal."""
     areas = np.array([p.area() for p in prims])
     probs = areas / areas.sum()
+    # a sample of primitive i can only be inside primitive j when their
+    # bounding balls overlap
+    ctr = np.array([p.ctr for p in prims])
+    rad = np.array([p.bound() for p in prims])
+    gap = np.linalg.norm(ctr[:, None] - ctr[None], axis=2) - (rad[:, None] + rad[None])
+    near = [[j for j in np.nonzero(gap[i] < 1e-9)[0] if j != i] for i in range(len(prims))]
     pts, nrms, have = [], [], 0
     while have < n:
         m = int(1.5 * (n - have)) + 4096
@@ -186,9 +201,8 @@ def union_of(prims, n, rng, noise):
                 continue
             x, nn = pr.sample(rng, int(c))
             keep = np.ones(len(x), bool)
-            for j, other in enumerate(prims):
-                if j != i:
-                    keep &= ~other.inside(x)
+            for j in near[i]:
+                keep &= ~prims[j].inside(x)
             pts.append(x[keep])
             nrms.append(nn[keep])
             have += int(keep.sum())
