@@ -180,10 +180,13 @@ def random_primitives(rng, m, shells=0):
     return prims
 
 
-def union_of(prims, n, rng, noise):
+def union_of(prims, n, rng, noise, adaptive=False):
     """Area-uniform samples of the union's boundary: each primitive is sampled
     in proportion to its area and samples inside any other primitive are
-    rejected; then Gaussian noise of `noise`·bbox-diagonal."""
+    rejected; then Gaussian noise of `noise`·bbox-diagonal.  Each pass draws
+    1.5× the missing count, or (adaptive) the missing count over the
+    acceptance rate seen so far, which needs few passes when the primitives
+    overlap a lot."""
     areas = np.array([p.area() for p in prims])
     probs = areas / areas.sum()
     # a sample of primitive i can only be inside primitive j when their
@@ -192,9 +195,13 @@ def union_of(prims, n, rng, noise):
     rad = np.array([p.bound() for p in prims])
     gap = np.linalg.norm(ctr[:, None] - ctr[None], axis=2) - (rad[:, None] + rad[None])
     near = [[j for j in np.nonzero(gap[i] < 1e-9)[0] if j != i] for i in range(len(prims))]
-    pts, nrms, have = [], [], 0
+    pts, nrms, have, drawn = [], [], 0, 0
     while have < n:
-        m = int(1.5 * (n - have)) + 4096
+        if adaptive and have > 0:
+            m = int(1.1 * (n - have) * drawn / have) + 4096
+        else:
+            m = int(1.5 * (n - have)) + 4096
+        drawn += m
         counts = rng.multinomial(m, probs)
         for i, (pr, c) in enumerate(zip(prims, counts)):
             if c == 0:
@@ -227,7 +234,7 @@ def large(n=5_000_000, seed=5, m=128, shells=16, hole_frac=0.05, hole_radius=0.0
     rng = np.random.default_rng(seed)
     prims = random_primitives(rng, m, shells)
     n0 = int(np.ceil(n / (1.0 - hole_frac)))
-    p, nn = union_of(prims, n0, rng, noise)
+    p, nn = union_of(prims, n0, rng, noise, adaptive=True)
     keep = np.ones(len(p), bool)
     while keep.sum() > n:
         c = p[rng.integers(0, len(p))]
