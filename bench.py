@@ -93,6 +93,70 @@ KERNELS = ["gwn_tree_grid_screened", "gwn_tree_screened", "octree_pack", "octree
            "gauss_map", "mlp_init", "mlp_forward"]
 
 
+def large_config(args):
+    """The large configuration, one step of each method after one warm-up
+    run of each: 5M points from a seeded union of 128 primitives and 16
+    sphere shells 0.01 thick, about 5% removed in 0.05-radius holes, noise
+    0.001·diag (seed 5); 1024^3 grids; learned+mixed (learned start,
+    LEARNED_ITERS iterations, BF16 far field + Kahan) against plain FP32 (40
+    iterations); quality against the same points before the noise."""
+    from dwg_amd import shapes
+    t0 = time.perf_counter()
+    P, Ntrue, S = shapes.large(5_000_000, 5, clean=True,
+                               progress=lambda have, n: log(f"sampling: {have} of {n} ({time.perf_counter() - t0:.0f} s)"))
+    log(f"large workload: {len(P)} points in {time.perf_counter() - t0:.0f} s")
+    import torch
+    from dwg_amd import gpu
+    ctx = gpu.Context(0)
+    net = gpu.Mlp(ctx, path=args.weights)
+    a = argparse.Namespace(**dict(vars(args), grid=1024))
+    plain, learned = plain_options(a), learned_options(a)
+    run_once(ctx, P, dict(plain, total_iter=2))
+    run_once(ctx, P, dict(learned, total_iter=2), mlp=net)
+    plain_t, plain_r = timed(ctx, P, plain, 1, "plain")
+    learned_t, learned_r = timed(ctx, P, learned, 1, "learned+mixed", mlp=net)
+    q_plain, q_learned = quality(ctx, plain_r, Ntrue, S), quality(ctx, learned_r, Ntrue, S)
+    gate = bool(q_learned["mean_angle_deg"] <= q_plain["mean_angle_deg"] + 0.5
+                and q_learned["chamfer"] <= 1.01 * q_plain["chamfer"])
+    print(json.dumps({"config": "large: 5M points, 128 primitives + 16 shells, 5% holes, 1024^3 grids",
+                      "n_points": int(len(P)), "n_sources": int(learned_r.n_sources),
+                      "learned_mixed_s": round(learned_t[0], 3), "plain_fp32_s": round(plain_t[0], 3),
+                      "speedup_vs_plain": round(plain_t[0] / learned_t[0], 3), "equal_error": gate,
+                      "learned_iterations": args.learned_iters, "quality": q_learned, "plain_quality": q_plain}),
+          flush=True)
+
+
+def direct_sums(reps=5):
+    """FLOP rate of the direct all-pairs sums (plain and screened) for the
+    small sphere configuration: 10,000 area-uniform sphere points (seed 1),
+    queries at the 128^3 points of a grid over the bounding box padded by
+    10%.  One JSON line per kernel."""
+    import torch
+    from dwg_amd import gpu, shapes
+    P, N = shapes.sphere(10_000, 1)
+    A = np.full(len(P), 4 * np.pi / len(P), np.float32)
+    lo, hi = P.min(0), P.max(0)
+    pad = 0.1 * (hi - lo)
+    axes = [np.linspace(lo[d] - pad[d], hi[d] + pad[d], 128, dtype=np.float32) for d in range(3)]
+    Q = np.stack(np.meshgrid(*axes, indexing="ij"), -1).reshape(-1, 3)
+    ctx = gpu.Context(0)
+    Pt, Nt, At, Qt = (torch.from_numpy(np.ascontiguousarray(a)).cuda() for a in (P, N, A, Q))
+    for screened, flops in ((False, FLOPS_PER_TERM["plain"]), (True, FLOPS_PER_TERM["screened"])):
+        ctx.gwn_direct(Pt, Nt, At, Qt, screened=screened, kahan=False)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            ctx.gwn_direct(Pt, Nt, At, Qt, screened=screened, kahan=False)
+        torch.cuda.synchronize()
+        ms = (time.perf_counter() - t0) / reps * 1e3
+        pairs = len(P) * len(Q)
+        tf = pairs * flops / (ms * 1e-3) / 1e12
+        print(json.dumps({"kernel": "gwn_direct_kernel", "screened": screened, "pairs": pairs, "ms": round(ms, 3),
+                          "tpairs_per_s": round(pairs / (ms * 1e-3) / 1e12, 3), "tflops": round(tf, 2),
+                          "frac_of_157.3": round(tf / PEAK_FP32, 4), "frac_of_78.6_non_packed": round(tf / 78.6, 4)}),
+              flush=True)
+
+
 def measured_traffic():
     """`traffic_bytes_per_launch` from the committed counter summary, or None."""
     try:
@@ -185,6 +249,12 @@ def main():
     ap.add_argument("--seed", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-quality", action="store_true")
+    ap.add_argument("--large", action="store_true",
+                    help="only run the large configuration: 5M points (128 primitives + 16 thin shells, 5%% in "
+                         "holes), 1024^3 grids, learned+mixed vs plain FP32; prints its own JSON line")
+    ap.add_argument("--direct-sums", action="store_true",
+                    help="only time the direct (all-pairs) sums of the small sphere configuration: 10,000 "
+                         "sources, 128^3 grid queries")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a device-to-device copy of MIB MiB (HBM counter calibration for rocprofv3 --pmc)")
     args = ap.parse_args()
@@ -197,6 +267,12 @@ def main():
         b.copy_(a)
         torch.cuda.synchronize()
         print(json.dumps({"copy_bytes_read": a.numel() * 4, "copy_bytes_written": b.numel() * 4}))
+        return
+    if args.direct_sums:
+        direct_sums()
+        return
+    if args.large:
+        large_config(args)
         return
     if args.gpus != 1:
         print(f"bench.py measures one GPU only: --gpus {args.gpus} is not supported in this build "

@@ -1118,22 +1118,24 @@ void knn_query(dwg_ctx* ctx, const dwg_knn_index* I, const float* q, int64_t nq,
     int* redo3 = static_cast<int*>(ctx->buf("knnt_redo3", sizeof(int) * nq + 64));
     const int64_t n1 = tiled_pass<kHalo1>(ctx, I, q, nq, nullptr, qmask, k, skip, out_idx, out_d2, redo1,
                                           base + ":tiles", sc);
+    // every tiled pass scatters the queries it closes (summed per tile in
+    // LDS first); the far queries scatter after the last search
     const int64_t n2 = n1 > 0 ? tiled_pass<kHalo2>(ctx, I, q, n1, redo1, nullptr, k, skip, out_idx, out_d2, redo2,
-                                                   base + ":tiles2")
+                                                   base + ":tiles2", sc)
                               : 0;
     const int64_t n3 = n2 > 0 ? tiled_pass<kHalo3>(ctx, I, q, n2, redo2, nullptr, k, skip, out_idx, out_d2, redo3,
-                                                   base + ":tiles3")
+                                                   base + ":tiles3", sc)
                               : 0;
     {
         const std::string name = base + ":redo";
         ProfileScope pr(ctx, name.c_str());
         launch_far(ctx, I, q, n3, redo3, k, skip, out_idx, out_d2);
     }
-    if (sc && n1 > 0) {  // the queries the first pass did not close scatter now
+    if (sc && n3 > 0) {
         const std::string name = base + ":scatter";
         ProfileScope pr(ctx, name.c_str());
-        hipLaunchKernelGGL(scatter_pairs_kernel, dim3(grid_for(n1 * k, 256)), dim3(256), 0, ctx->stream, out_idx, k,
-                           redo1, n1, *sc);
+        hipLaunchKernelGGL(scatter_pairs_kernel, dim3(grid_for(n3 * k, 256)), dim3(256), 0, ctx->stream, out_idx, k,
+                           redo3, n3, *sc);
         check_launch();
     }
 }

@@ -451,18 +451,117 @@ static void forward(dwg_ctx* ctx, const dwg_mlp* m, const uint16_t* x, int64_t r
     gemm(ctx, {a.h2, kHid, 1, W + kOffW3, 1, kHid, a.y, kOut, 1, rows, kOut, kHid, 0});
 }
 
-void mlp_forward(dwg_ctx* ctx, const dwg_mlp* m, const uint16_t* x, int64_t n, float* out, int normalize) {
-    const int64_t chunk = 1 << 18;
-    ProfileScope ps(ctx, "mlp_forward");
-    for (int64_t s = 0; s < n; s += chunk) {
-        const int64_t rows = std::min(chunk, n - s);
-        Acts a = acts(ctx, rows);
-        forward(ctx, m, x + s * kIn, rows, a);
-        hipLaunchKernelGGL(add_bias_normalize_kernel, dim3(grid_for(rows, 256)), dim3(256), 0, ctx->stream, a.y,
-                           m->params + kOffB3, rows, normalize);
-        check_launch();
-        DWG_HIP_CHECK(hipMemcpyAsync(out + 3 * s, a.y, sizeof(float) * 3 * rows, hipMemcpyDeviceToDevice, ctx->stream));
+// Inference of the whole network, 32 patches per wave: the three layers as
+// the same sequence of v_mfma_f32_32x32x8_bf16 steps as the layer-by-layer
+// GEMMs (so the same sums, bit for bit), bias and GELU applied to the
+// accumulators, the BF16 activations staged in LDS between layers, and the
+// output bias and normalisation at the end: one launch, no activations in
+// HBM.
+constexpr int kInferWaves = 4;
+constexpr int kHPad = kHid + 8;  // LDS row of 136 BF16 values: 2-way bank conflicts at most
+
+__device__ __forceinline__ void infer_layer(const f32x16 (&acc)[4], const float* __restrict__ bias, int lane,
+                                            uint16_t (*h)[kHPad]) {
+#pragma unroll
+    for (int t = 0; t < 4; ++t)
+#pragma unroll
+        for (int q = 0; q < 16; ++q) {
+            const int rr = 8 * (q / 4) + 4 * (lane >> 5) + (q % 4), cc = t * 32 + (lane & 31);
+            h[rr][cc] = to_bf16(gelu(acc[t][q] + bias[cc], nullptr));
+        }
+}
+
+__global__ __launch_bounds__(64 * kInferWaves) void mlp_infer_kernel(const uint16_t* __restrict__ x, int64_t n,
+                                                                     const uint16_t* __restrict__ W,
+                                                                     const float* __restrict__ P, int normalize,
+                                                                     float* __restrict__ out) {
+    __shared__ uint16_t hs[kInferWaves][32][kHPad];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    uint16_t(*h)[kHPad] = hs[wave];
+    const int64_t row0 = (int64_t(blockIdx.x) * kInferWaves + wave) * 32;
+    const int r = lane & 31, kq = 4 * (lane >> 5);
+    const int64_t row = row0 + r;
+    const bool ok = row < n;
+    f32x16 acc[4];
+    // layer 1: Z1 = X·W1ᵀ (K = 48)
+#pragma unroll
+    for (int t = 0; t < 4; ++t)
+#pragma unroll
+        for (int q = 0; q < 16; ++q) acc[t][q] = 0.f;
+    for (int k0 = 0; k0 < kIn; k0 += 8) {
+        bf16x4 a;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) a[j] = ok ? short(x[row * kIn + k0 + kq + j]) : short(0);
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+            bf16x4 b;
+#pragma unroll
+            for (int j = 0; j < 4; ++j) b[j] = short(W[kOffW1 + int64_t(t * 32 + r) * kIn + k0 + kq + j]);
+            acc[t] = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, b, acc[t], 0, 0, 0);
+        }
     }
+    infer_layer(acc, P + kOffB1, lane, h);
+    __syncthreads();
+    // layer 2: Z2 = H1·W2ᵀ (K = 128)
+#pragma unroll
+    for (int t = 0; t < 4; ++t)
+#pragma unroll
+        for (int q = 0; q < 16; ++q) acc[t][q] = 0.f;
+    for (int k0 = 0; k0 < kHid; k0 += 8) {
+        bf16x4 a;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) a[j] = short(h[r][k0 + kq + j]);
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+            bf16x4 b;
+#pragma unroll
+            for (int j = 0; j < 4; ++j) b[j] = short(W[kOffW2 + int64_t(t * 32 + r) * kHid + k0 + kq + j]);
+            acc[t] = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, b, acc[t], 0, 0, 0);
+        }
+    }
+    __syncthreads();
+    infer_layer(acc, P + kOffB2, lane, h);
+    __syncthreads();
+    // layer 3: Y = H2·W3ᵀ (3 columns of a 32-wide tile)
+    f32x16 y;
+#pragma unroll
+    for (int q = 0; q < 16; ++q) y[q] = 0.f;
+    for (int k0 = 0; k0 < kHid; k0 += 8) {
+        bf16x4 a, b;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            a[j] = short(h[r][k0 + kq + j]);
+            b[j] = r < kOut ? short(W[kOffW3 + int64_t(r) * kHid + k0 + kq + j]) : short(0);
+        }
+        y = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, b, y, 0, 0, 0);
+    }
+    __syncthreads();
+    // column c of row i sits in lane c (+32), register 8(i/8)... stage through LDS as floats
+    float* ys = reinterpret_cast<float*>(&h[0][0]);  // 32 × 3 floats
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+        const int rr = 8 * (q / 4) + 4 * (lane >> 5) + (q % 4);
+        if (r < kOut) ys[rr * kOut + r] = y[q];
+    }
+    __syncthreads();
+    if (lane < 32 && ok) {
+        float v[3];
+        for (int c = 0; c < 3; ++c) v[c] = ys[lane * kOut + c] + P[kOffB3 + c];
+        if (normalize) {
+            const float s = 1.0f / (sqrtf(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]) + 1e-12f);
+            for (int c = 0; c < 3; ++c) v[c] *= s;
+        }
+        for (int c = 0; c < 3; ++c) out[row * 3 + c] = v[c];
+    }
+}
+
+void mlp_forward(dwg_ctx* ctx, const dwg_mlp* m, const uint16_t* x, int64_t n, float* out, int normalize) {
+    ProfileScope ps(ctx, "mlp_forward");
+    if (n <= 0) return;
+    const int64_t waves = (n + 31) / 32;
+    hipLaunchKernelGGL(mlp_infer_kernel, dim3(grid_for(waves, kInferWaves)), dim3(64 * kInferWaves), 0, ctx->stream,
+                       x, n, m->bf16, m->params, normalize, out);
+    check_launch();
 }
 
 double mlp_loss_grad(dwg_ctx* ctx, dwg_mlp* m, const uint16_t* x, const float* target, int64_t B, int mode) {

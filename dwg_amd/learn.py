@@ -138,8 +138,19 @@ def main(argv=None):
     ap.add_argument("--eval", type=int, default=64)
     ap.add_argument("--out", default="dwg_mlp.bin")
     ap.add_argument("--report", default=None)
+    ap.add_argument("--eval-only", default=None, metavar="WEIGHTS",
+                    help="skip training: evaluate these weights on the held-out shapes")
     args = ap.parse_args(argv)
     ctx = gpu.Context(0)
+    if args.eval_only:
+        net = gpu.Mlp(ctx, path=args.eval_only)
+        rows = evaluate(ctx, net, range(EVAL_SEED0, EVAL_SEED0 + args.eval), args.points)
+        rep = {"weights": args.eval_only, "eval": summary(rows), "rows": rows}
+        log(json.dumps(rep["eval"]))
+        if args.report:
+            with open(args.report, "w") as f:
+                json.dump(rep, f, indent=1)
+        return rep
     net = gpu.Mlp(ctx, 1)
     X, T = training_set(ctx, net, range(TRAIN_SEED0, TRAIN_SEED0 + args.shapes), args.points)
     log(f"training set: {X.shape[0]} patches")

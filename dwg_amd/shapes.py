@@ -20,6 +20,9 @@ Workloads:
 """
 from __future__ import annotations
 
+import os
+from concurrent.futures import ThreadPoolExecutor
+
 import numpy as np
 
 
@@ -91,19 +94,21 @@ class _Prim:
 
     def inside(self, x):
         """Strictly inside the solid (points on or near the boundary are not)."""
-        y = self.to_local(x)
         p = self.params
-        if self.kind == 0:
-            return np.einsum("ij,ij->i", y, y) < (p[0] * (1 - 1e-6)) ** 2
+        if self.kind in (0, 3):  # radial: no rotation needed
+            d = x - self.ctr
+            d2 = np.einsum("ij,ij->i", d, d)
+            if self.kind == 0:
+                return d2 < (p[0] * (1 - 1e-6)) ** 2
+            ro, t = p
+            return (d2 < (ro * (1 - 1e-6)) ** 2) & (d2 > ((ro - t) * (1 + 1e-6)) ** 2)
+        y = self.to_local(x)
         if self.kind == 1:
             return ((y / p) ** 2).sum(1) < 1 - 1e-6
         if self.kind == 2:
             R, r = p
             q = np.sqrt(y[:, 0] ** 2 + y[:, 1] ** 2) - R
             return q * q + y[:, 2] ** 2 < (r * (1 - 1e-6)) ** 2
-        ro, t = p
-        d = np.sqrt(np.einsum("ij,ij->i", y, y))
-        return (d < ro * (1 - 1e-6)) & (d > (ro - t) * (1 + 1e-6))
 
     def sample(self, rng, m):
         """m area-uniform surface samples (points, outward normals) in world frame."""
@@ -180,13 +185,14 @@ def random_primitives(rng, m, shells=0):
     return prims
 
 
-def union_of(prims, n, rng, noise, adaptive=False):
+def union_of(prims, n, rng, noise, adaptive=False, progress=None, clean=False):
     """Area-uniform samples of the union's boundary: each primitive is sampled
     in proportion to its area and samples inside any other primitive are
     rejected; then Gaussian noise of `noise`·bbox-diagonal.  Each pass draws
     1.5× the missing count, or (adaptive) the missing count over the
     acceptance rate seen so far, which needs few passes when the primitives
-    overlap a lot."""
+    overlap a lot.  progress(have, n) is called after every pass; clean=True
+    also returns the points before the noise."""
     areas = np.array([p.area() for p in prims])
     probs = areas / areas.sum()
     # a sample of primitive i can only be inside primitive j when their
@@ -203,22 +209,38 @@ def union_of(prims, n, rng, noise, adaptive=False):
             m = int(1.5 * (n - have)) + 4096
         drawn += m
         counts = rng.multinomial(m, probs)
-        for i, (pr, c) in enumerate(zip(prims, counts)):
-            if c == 0:
-                continue
-            x, nn = pr.sample(rng, int(c))
+        drawn_i = [(i, *prims[i].sample(rng, int(c))) for i, c in enumerate(counts) if c > 0]
+
+        def kept(item):
+            i, x, _ = item
             keep = np.ones(len(x), bool)
             for j in near[i]:
-                keep &= ~prims[j].inside(x)
+                # only the samples inside j's bounding box can be inside j
+                cand = np.nonzero(keep & (np.abs(x - ctr[j]) < rad[j]).all(1))[0]
+                if cand.size:
+                    keep[cand] = ~prims[j].inside(x[cand])
+            return keep
+
+        # the tests release the GIL inside numpy, so threads share the work;
+        # the samples were all drawn first, in order, so the result does not
+        # depend on the thread count
+        with ThreadPoolExecutor(max_workers=min(16, os.cpu_count() or 1)) as pool:
+            keeps = list(pool.map(kept, drawn_i))
+        for (i, x, nn), keep in zip(drawn_i, keeps):
             pts.append(x[keep])
             nrms.append(nn[keep])
             have += int(keep.sum())
+        if progress:
+            progress(have, n)
     p = np.concatenate(pts)
     nn = np.concatenate(nrms)
     perm = rng.permutation(len(p))[:n]
     p, nn = p[perm], nn[perm]
     diag = np.linalg.norm(p.max(0) - p.min(0))
+    p0 = p
     p = p + rng.normal(0.0, noise * diag, p.shape)
+    if clean:
+        return p.astype(np.float32), nn.astype(np.float32), p0.astype(np.float32)
     return p.astype(np.float32), nn.astype(np.float32)
 
 
@@ -228,18 +250,22 @@ def union(n=1_000_000, m=32, seed=3, noise=0.001):
     return union_of(prims, n, rng, noise)
 
 
-def large(n=5_000_000, seed=5, m=128, shells=16, hole_frac=0.05, hole_radius=0.05, noise=0.001):
+def large(n=5_000_000, seed=5, m=128, shells=16, hole_frac=0.05, hole_radius=0.05, noise=0.001, progress=None,
+          clean=False):
     """Large workload: primitives plus 0.01-thick sphere shells, with seeded
-    hole patches of radius 0.05 removing about 5% of the points."""
+    hole patches of radius 0.05 removing about 5% of the points.  clean=True
+    also returns the same points before the noise (for Chamfer distances)."""
     rng = np.random.default_rng(seed)
     prims = random_primitives(rng, m, shells)
     n0 = int(np.ceil(n / (1.0 - hole_frac)))
-    p, nn = union_of(prims, n0, rng, noise, adaptive=True)
+    p, nn, p0 = union_of(prims, n0, rng, noise, adaptive=True, progress=progress, clean=True)
     keep = np.ones(len(p), bool)
     while keep.sum() > n:
         c = p[rng.integers(0, len(p))]
         keep &= np.einsum("ij,ij->i", p - c, p - c) > hole_radius ** 2
     idx = np.nonzero(keep)[0][:n]
+    if clean:
+        return p[idx], nn[idx], p0[idx]
     return p[idx], nn[idx]
 
 

@@ -1,0 +1,33 @@
+"""The synthetic workloads: sizes, determinism, points on the stated
+surfaces, unit normals, noise of the stated size."""
+import numpy as np
+
+from dwg_amd import shapes
+
+
+def test_sphere_and_torus():
+    P, N = shapes.sphere(3000, 1)
+    assert P.shape == N.shape == (3000, 3)
+    assert np.allclose(np.linalg.norm(P, axis=1), 1.0, atol=1e-6)
+    assert np.allclose(P, N, atol=1e-6)
+    P, N = shapes.torus(4000, 2, noise=0.0)
+    q = np.sqrt(P[:, 0] ** 2 + P[:, 1] ** 2) - 1.0
+    assert np.allclose(q ** 2 + P[:, 2] ** 2, 0.35 ** 2, atol=1e-5)
+    assert np.allclose(np.linalg.norm(N, axis=1), 1.0, atol=1e-6)
+    # density ratio 1..8 along the major angle: the far side (φ = π) holds more
+    phi = np.arctan2(P[:, 1], P[:, 0])
+    assert np.mean(np.abs(phi) > 2.5) > 3 * np.mean(np.abs(phi) < 0.64)
+    P2, _ = shapes.torus(4000, 2, noise=0.0)
+    assert np.array_equal(P, P2)
+
+
+def test_union_and_large():
+    P, N = shapes.union(5000, 8, 11)
+    P2, _ = shapes.union(5000, 8, 11)
+    assert P.shape == (5000, 3) and np.array_equal(P, P2)
+    assert np.allclose(np.linalg.norm(N, axis=1), 1.0, atol=1e-5)
+    P, N, C = shapes.large(20_000, 5, m=16, shells=4, clean=True)
+    assert len(P) <= 20_000 and len(P) > 19_000
+    d = np.linalg.norm(P - C, axis=1)
+    diag = np.linalg.norm(C.max(0) - C.min(0))
+    assert 0.5 * 0.001 * diag < np.sqrt((d ** 2).mean() / 3) < 1.5 * 0.001 * diag
