@@ -67,17 +67,29 @@ __global__ void nonzero_gather_kernel(const int64_t* __restrict__ slot, const fl
     index[j] = int32_t(i);
 }
 
+// the zero sources, compacted: their indices and positions (slot[i] counts
+// the non-zero sources before i, so i − slot[i] counts the zero ones)
+__global__ void zero_gather_kernel(const int64_t* __restrict__ slot, const float* __restrict__ p, int64_t n,
+                                   float* __restrict__ zq, int32_t* __restrict__ zlist) {
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n || slot[i + 1] != slot[i]) return;
+    const int64_t j = i - slot[i];
+    for (int d = 0; d < 3; ++d) zq[3 * j + d] = p[3 * i + d];
+    zlist[j] = int32_t(i);
+}
+
 // SimpleUpdateNonzeroNormals (TurboDWG.cu:322-359): a zero source takes the
 // normalised sum of its nearest non-zero sources' normals.
-// knn holds positions in the list of non-zero sources; nz_index maps them
-// back to source indices.
-__global__ void zero_fill_kernel(const int32_t* __restrict__ knn, int k, const uint8_t* __restrict__ zero, int64_t n,
-                                 const int32_t* __restrict__ nz_index, float* __restrict__ nrm) {
-    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (i >= n || !zero[i]) return;
+// knn row t belongs to zero source zlist[t] and holds positions in the list
+// of non-zero sources; nz_index maps them back to source indices.
+__global__ void zero_fill_kernel(const int32_t* __restrict__ knn, int k, const int32_t* __restrict__ zlist,
+                                 int64_t nzero, const int32_t* __restrict__ nz_index, float* __restrict__ nrm) {
+    const int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (t >= nzero) return;
+    const int64_t i = zlist[t];
     float sx = 0.f, sy = 0.f, sz = 0.f;
     for (int j = 0; j < k; ++j) {
-        const int32_t c = knn[i * k + j];
+        const int32_t c = knn[t * k + j];
         if (c < 0) continue;
         const int64_t s = nz_index[c];
         sx += nrm[3 * s];
@@ -120,6 +132,8 @@ void diffuse_normals(dwg_ctx* ctx, const dwg_knn_index* I, const float* src, int
         int64_t* slot = static_cast<int64_t*>(ctx->buf("diff_nz_slot", sizeof(int64_t) * (n + 1)));
         float* nzp = static_cast<float*>(ctx->buf("diff_nz_pts", sizeof(float) * 3 * m));
         int32_t* nzi = static_cast<int32_t*>(ctx->buf("diff_nz_index", sizeof(int32_t) * m));
+        float* zq = static_cast<float*>(ctx->buf("diff_z_pts", sizeof(float) * 3 * nzero));
+        int32_t* zlist = static_cast<int32_t*>(ctx->buf("diff_z_list", sizeof(int32_t) * nzero));
         {
             ProfileScope ps(ctx, "diffuse_nonzero_index");
             hipLaunchKernelGGL(nonzero_flags_kernel, dim3(grid_for(n + 1, 256)), dim3(256), 0, ctx->stream, zero, n,
@@ -129,13 +143,17 @@ void diffuse_normals(dwg_ctx* ctx, const dwg_knn_index* I, const float* src, int
             hipLaunchKernelGGL(nonzero_gather_kernel, dim3(grid_for(n, 256)), dim3(256), 0, ctx->stream, slot, src, n,
                                nzp, nzi);
             check_launch();
+            hipLaunchKernelGGL(zero_gather_kernel, dim3(grid_for(n, 256)), dim3(256), 0, ctx->stream, slot, src, n, zq,
+                               zlist);
+            check_launch();
         }
         std::unique_ptr<dwg_knn_index> J(knn_build_like(ctx, I, nzp, m));
-        int32_t* zknn = static_cast<int32_t*>(ctx->buf("diff_zknn", sizeof(int32_t) * n * zero_k));
-        knn_query(ctx, J.get(), src, n, zero_k, nullptr, zero, zknn, nullptr, "knn_query_zero_fill");
+        // only the zero sources are queried
+        int32_t* zknn = static_cast<int32_t*>(ctx->buf("diff_zknn", sizeof(int32_t) * nzero * zero_k));
+        knn_query(ctx, J.get(), zq, nzero, zero_k, nullptr, nullptr, zknn, nullptr, "knn_query_zero_fill");
         ProfileScope ps(ctx, "diffuse_zero_fill");
-        hipLaunchKernelGGL(zero_fill_kernel, dim3(grid_for(n, 256)), dim3(256), 0, ctx->stream, zknn, zero_k, zero, n,
-                           nzi, nrm);
+        hipLaunchKernelGGL(zero_fill_kernel, dim3(grid_for(nzero, 256)), dim3(256), 0, ctx->stream, zknn, zero_k, zlist,
+                           nzero, nzi, nrm);
         check_launch();
     }
 }

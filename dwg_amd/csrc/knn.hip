@@ -720,6 +720,56 @@ __global__ void scatter_pairs_kernel(const int32_t* __restrict__ knn, int k, con
     }
 }
 
+// The same for 256 consecutive queries of qlist per block (qlist is in tile
+// order, so they are neighbours and share most of their neighbours): the
+// pairs are summed in an LDS hash keyed by point index and each key is
+// flushed once with 64-bit global atomics (integer sums: bit for bit the
+// per-pair result).
+__global__ __launch_bounds__(256) void scatter_pairs_hashed_kernel(const int32_t* __restrict__ knn, int k,
+                                                                   const int* __restrict__ qlist, int64_t nq,
+                                                                   KnnScatter sc) {
+    constexpr int kHash = 1024;
+    __shared__ int s_hkey[kHash];
+    __shared__ unsigned long long s_hacc[kHash][3];
+    for (int h = threadIdx.x; h < kHash; h += 256) {
+        s_hkey[h] = -1;
+        s_hacc[h][0] = s_hacc[h][1] = s_hacc[h][2] = 0ull;
+    }
+    __syncthreads();
+    const int64_t t = int64_t(blockIdx.x) * 256 + threadIdx.x;
+    if (t < nq) {
+        const int64_t qi = qlist ? int64_t(qlist[t]) : t;
+        long long v[3];
+#pragma unroll
+        for (int d = 0; d < 3; ++d) v[d] = __double2ll_rn(double(sc.val[3 * qi + d]) * kScatterFix);
+        for (int j = 0; j < k; ++j) {
+            const int32_t idx = knn[qi * k + j];
+            if (idx < 0) continue;
+            int h = int((uint32_t(idx) * 0x9E3779B1u) >> 22);  // 10 bits
+            bool done = false;
+            for (int probe = 0; probe < 8 && !done; ++probe) {
+                const int prev = atomicCAS(&s_hkey[h], -1, idx);
+                if (prev == -1 || prev == idx) {
+#pragma unroll
+                    for (int d = 0; d < 3; ++d) atomicAdd(&s_hacc[h][d], (unsigned long long)v[d]);
+                    done = true;
+                }
+                h = (h + 1) & (kHash - 1);
+            }
+            if (!done)
+#pragma unroll
+                for (int d = 0; d < 3; ++d) atomicAdd(&sc.acc[3 * int64_t(idx) + d], (unsigned long long)v[d]);
+        }
+    }
+    __syncthreads();
+    for (int h = threadIdx.x; h < kHash; h += 256) {
+        const int key = s_hkey[h];
+        if (key >= 0)
+#pragma unroll
+            for (int d = 0; d < 3; ++d) atomicAdd(&sc.acc[3 * int64_t(key) + d], s_hacc[h][d]);
+    }
+}
+
 __device__ __forceinline__ uint64_t shfl64(uint64_t v, int src) {
     const uint32_t lo = uint32_t(__shfl(int(uint32_t(v)), src, 64));
     const uint32_t hi = uint32_t(__shfl(int(uint32_t(v >> 32)), src, 64));
@@ -1134,8 +1184,8 @@ void knn_query(dwg_ctx* ctx, const dwg_knn_index* I, const float* q, int64_t nq,
     if (sc && n3 > 0) {
         const std::string name = base + ":scatter";
         ProfileScope pr(ctx, name.c_str());
-        hipLaunchKernelGGL(scatter_pairs_kernel, dim3(grid_for(n3 * k, 256)), dim3(256), 0, ctx->stream, out_idx, k,
-                           redo3, n3, *sc);
+        hipLaunchKernelGGL(scatter_pairs_hashed_kernel, dim3(grid_for(n3, 256)), dim3(256), 0, ctx->stream, out_idx,
+                           k, redo3, n3, *sc);
         check_launch();
     }
 }
