@@ -15,7 +15,7 @@ static void usage() {
                  "--in_name   The file name of the input point cloud (only supports .xyz currently)\n"
                  "--out_path   The absolute path of the output mesh/point cloud with normals\n"
                  "--output_mode   Ouput mode: 1 Mesh(.stl) (default)   2 Point cloud with normals(.xyz)   3 Both 1 and 2\n"
-                 "--in_normal   Initialization mode: 0 Random   1 Input   2 Random (PCA areas with --com_As 1)   3 Gauss map (default)   4 Learned MLP (needs --mlp)   5 PCA (k=10)\n"
+                 "--in_normal   Initialization mode: 0 Random   1 Input   3 Gauss map (default)\n"
                  "--lambda   Screening Coefficient\n"
                  "--max_depth   The max depth of octree\n"
                  "--use_radii   Turn on screening mode or not (0 turn off   1 turn on)\n"
@@ -28,6 +28,7 @@ static void usage() {
                  "--com_As   Computing area weight or not (0 turn off   1 turn on)\n"
                  "--use_leaves_generate   Recommend to open it (0 turn off   1 turn on), when the point cloud noise is high\n"
                  "MI355X build:\n"
+                 "--in_normal   also 2 Random (PCA normals for the areas with --com_As 1)   4 Learned MLP (needs --mlp)   5 PCA (k=10)\n"
                  "--source_mode   0 octree-leaf centres as sources (default)   1 the input points\n"
                  "--leaf_size   points per octree leaf in the winding-number sums (default 1 = octreeDWG)\n"
                  "--beta   Barnes-Hut accuracy (default 2.3; <= 0 direct sums)\n"

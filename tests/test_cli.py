@@ -1,0 +1,62 @@
+"""The DWG_HIP command line (the reference's DWG_CUDA, src/main.cpp:17-166):
+help text, output naming, .xyz in, binary STL and .xyz out."""
+import os
+import struct
+import subprocess
+
+import numpy as np
+import pytest
+
+from dwg_amd import shapes
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+EXE = os.path.join(ROOT, "dwg_amd", "DWG_HIP")
+REFERENCE_FLAGS = ["--in_path", "--in_name", "--out_path", "--output_mode", "--in_normal", "--lambda", "--max_depth",
+                   "--use_radii", "--kw", "--final_kw", "--grid_reso", "--final_reso", "--total_iter", "--test_iter",
+                   "--com_As", "--use_leaves_generate"]
+
+
+def _run(args, timeout=300):
+    return subprocess.run([EXE] + args, capture_output=True, text=True, timeout=timeout)
+
+
+def test_help_lists_reference_flags():
+    r = _run(["--help"])
+    assert r.returncode == 0
+    lines = r.stdout.splitlines()
+    for f in REFERENCE_FLAGS:
+        assert any(l.startswith(f + " ") for l in lines), f
+    # the reference's own wording for the initialisation modes
+    assert "--in_normal   Initialization mode: 0 Random   1 Input   3 Gauss map (default)" in lines
+
+
+def test_missing_input(tmp_path):
+    r = _run(["--in_path", str(tmp_path), "--in_name", "none.xyz", "--out_path", str(tmp_path)])
+    assert r.returncode == 2 and "cannot read point cloud" in r.stderr
+
+
+def _write_xyz(path, P, N):
+    with open(path, "w") as f:
+        for p, n in zip(P, N):
+            f.write(f"{p[0]:.7f} {p[1]:.7f} {p[2]:.7f} {n[0]:.6f} {n[1]:.6f} {n[2]:.6f}\n")
+
+
+@pytest.mark.gpu
+def test_reconstruct_files(tmp_path):
+    P, N = shapes.sphere(20_000, 41)
+    _write_xyz(tmp_path / "ball.xyz", 0.5 * P + 0.2, N)
+    r = _run(["--in_path", str(tmp_path), "--in_name", "ball.xyz", "--out_path", str(tmp_path), "--output_mode", "3",
+              "--grid_reso", "96", "--final_reso", "96", "--total_iter", "6", "--source_mode", "1"])
+    assert r.returncode == 0, r.stderr
+    base = tmp_path / "ball_md_10_gr_96_iter_6"
+    stl = open(str(base) + ".stl", "rb").read()
+    (count,) = struct.unpack("<I", stl[80:84])
+    assert len(stl) == 84 + 50 * count and count > 1000
+    assert f"Num triangles: {count}" in r.stdout
+    tri = np.frombuffer(stl[84:], dtype=np.dtype([("n", "<f4", 3), ("v", "<f4", (3, 3)), ("a", "<u2")]))
+    radius = np.linalg.norm(tri["v"].reshape(-1, 3) - 0.2, axis=1)
+    assert np.abs(radius - 0.5).max() < 0.02  # the mesh is back in the input frame
+    X = np.loadtxt(str(base) + ".xyz")
+    assert X.shape == (20_000, 6)
+    cos = np.einsum("ij,ij->i", X[:, 3:] / np.linalg.norm(X[:, 3:], axis=1, keepdims=True), N)
+    assert np.median(cos) > 0.99
