@@ -111,6 +111,11 @@ public:
     // GenerateMesh(grid) variant of --use_leaves_generate 1 (TurboDWG.cu:768-806).
     void GenerateMeshFromSources(const Grid& final_grid, int64_t* tri_count, HostMesh* tris);
 
+    // --test_iter dumps (TurboMC.cu:602-627): with a directory set, iteration
+    // it with it % test_iter == 0 writes its marching-cubes mesh there as
+    // <month><day><hour><minute><second>_<it>iter.stl
+    void SetDumpDirectory(const std::string& dir) { dump_dir_ = dir; }
+
     // exact Barnes-Hut term counts of one source and one grid evaluation
     void CountTerms(int64_t* src_terms, int64_t* grid_terms);
     int64_t num_sources() const { return n_; }
@@ -122,6 +127,7 @@ public:
 private:
     void check(int rc, const char* what) const;
     void mesh_pass(const Grid& g, float iso_scale, int64_t* tri_count, HostMesh* tris);
+    void dump_mesh(const dwg_grid& g, float iso, int64_t ntri, int it);
 
     dwg_ctx* ctx_;
     dwg_options opt_;
@@ -130,6 +136,7 @@ private:
     Grid grid_{};
     dwg_octree* tree_ = nullptr;
     dwg_knn_index* knn_ = nullptr;
+    std::string dump_dir_;
 };
 
 // test3D entry points (DWGProcess.h) over in-memory points.

@@ -9,6 +9,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <ctime>
 #include <fstream>
 #include <memory>
 #include <mutex>
@@ -342,8 +343,8 @@ double TurboDWG::TurboScreenedDWGLaunch(int total_iters, double* w_mean, int64_t
         }
         check(rc, "marching cubes");
         if (iter_tris) iter_tris[it] = ntri;
-        if (opt_.test_iter != 999999 && opt_.test_iter > 0 && it % opt_.test_iter == 0 && opt_.verbose)
-            std::printf("iter %d: periodic mesh dumps are written by the command-line tool only\n", it);
+        if (!dump_dir_.empty() && opt_.test_iter != 999999 && opt_.test_iter > 0 && it % opt_.test_iter == 0)
+            dump_mesh(g, iso, ntri, it);
         int64_t nz = 0;
         check(dwg_diffuse_normals(ctx_, knn_, pts_.ptr, n_, cent_.ptr, tri_nrm_.ptr, ntri, 15, 10, 10, nrm_.ptr, &nz),
               "diffuse");
@@ -353,6 +354,25 @@ double TurboDWG::TurboScreenedDWGLaunch(int total_iters, double* w_mean, int64_t
     }
     check(dwg_ctx_sync(ctx_), "sync");
     return seconds_since(t0);
+}
+
+void TurboDWG::dump_mesh(const dwg_grid& g, float iso, int64_t ntri, int it) {
+    // the iteration's mesh again, now with its vertices (the iteration itself
+    // only keeps centroids and face normals)
+    DevArray<float> tris(9 * std::max<int64_t>(ntri, 1)), c(3 * std::max<int64_t>(ntri, 1)),
+        nr(3 * std::max<int64_t>(ntri, 1));
+    int64_t n2 = 0;
+    check(dwg_marching_cubes(ctx_, w_grid_.ptr, &g, iso, std::max<int64_t>(ntri, 1), tris.ptr, c.ptr, nr.ptr, &n2),
+          "marching cubes (dump)");
+    std::vector<float> h(size_t(9 * n2));
+    if (n2 > 0) tris.download(h.data(), 9 * n2, ctx_->stream);
+    const std::time_t now = std::time(nullptr);
+    const std::tm lt = *std::localtime(&now);
+    const std::string name = dump_dir_ + std::to_string(lt.tm_mon + 1) + std::to_string(lt.tm_mday) +
+                             std::to_string(lt.tm_hour) + std::to_string(lt.tm_min) + std::to_string(lt.tm_sec) + "_" +
+                             std::to_string(it) + "iter.stl";
+    save_binary_stl(name, h.data(), n2);
+    std::printf("Mesh saved to %s (%lld triangle(s).\n", name.c_str(), (long long)n2);
 }
 
 void TurboDWG::CountTerms(int64_t* src_terms, int64_t* grid_terms) {
@@ -581,6 +601,11 @@ void DWGLaunch(dwg_ctx* ctx, const float* pts, const float* nrm, int64_t n, cons
         }
     }
     TurboDWG dwg(ctx, opt);
+    if (out_base) {  // the dumps go to the output directory (main.cpp:155-156)
+        const std::string ob(out_base);
+        const size_t slash = ob.find_last_of('/');
+        dwg.SetDumpDirectory(slash == std::string::npos ? std::string("./") : ob.substr(0, slash + 1));
+    }
     dwg.SetSources(S, ns, Sn.ptr, grid, nullptr, opt.com_As ? src_area.ptr : nullptr);
     dwg.UpdateRadii(opt.kw);
     if (opt.input_normal == 3) dwg.GaussMapInit();

@@ -70,3 +70,25 @@ def test_reconstruct_files(tmp_path):
     assert np.abs(np.linalg.norm(X[:, :3], axis=1) - 0.5).max() < 0.01
     cos = np.einsum("ij,ij->i", X[:, 3:] / np.linalg.norm(X[:, 3:], axis=1, keepdims=True), N)
     assert np.median(cos) > 0.99
+
+
+@pytest.mark.gpu
+def test_periodic_dumps(tmp_path):
+    """--test_iter t writes iteration it's mesh for every it % t == 0 as
+    <time>_<it>iter.stl in the output directory (src/TurboMC.cu:602-627)."""
+    import glob
+    import re
+
+    P, N = shapes.sphere(5_000, 42)
+    _write_xyz(tmp_path / "s.xyz", P, N)
+    r = _run(["--in_path", str(tmp_path), "--in_name", "s.xyz", "--out_path", str(tmp_path), "--grid_reso", "48",
+              "--final_reso", "48", "--total_iter", "5", "--test_iter", "2", "--source_mode", "1"])
+    assert r.returncode == 0, r.stderr
+    dumps = sorted(glob.glob(str(tmp_path / "*iter.stl")))
+    its = sorted(int(re.search(r"_(\d+)iter\.stl$", d).group(1)) for d in dumps)
+    assert its == [0, 2, 4], dumps
+    for d in dumps:
+        b = open(d, "rb").read()
+        (count,) = struct.unpack("<I", b[80:84])
+        assert count > 100 and len(b) == 84 + 50 * count
+    assert (tmp_path / "s_md_10_gr_48_iter_5.stl").exists()

@@ -3,7 +3,8 @@ queries), bit-exact against the oracle's brute-force ranking by
 (float d², index): k up to 32, queries outside the point bounds, a dense
 cluster that overflows one LDS stage, exact duplicates (ties), a skipped
 region large enough to send queries to the per-lane fallback, and a query
-mask.
+mask; and a sparse query set (fewer than one query per 8 points) that skips
+the tiles for the pyramid walk.
 """
 import numpy as np
 import pytest
@@ -84,3 +85,24 @@ def test_knn_far_queries(ctx):
         ref, rd2 = O.knn_brute(P, Q, k, return_d2=True)
         assert np.array_equal(got.cpu().numpy(), ref)
         assert np.array_equal(d2.cpu().numpy(), rd2)
+
+
+@pytest.mark.parametrize("k", [10, 16])
+def test_knn_sparse_queries_pyramid(ctx, k):
+    # 6,000 queries over 60,000 points, as the zero fill queries the non-zero
+    # sources: near and far queries and a skipped cap, all walked per lane
+    P = _noisy_sphere(60_000, 50 + k)
+    rng = np.random.default_rng(60 + k)
+    Q = np.concatenate([P[rng.integers(0, len(P), 3000)] + rng.normal(0, 0.003, (3000, 3)),
+                        rng.uniform(-0.7, 0.7, (2500, 3)),
+                        rng.uniform(-1.5, 1.5, (500, 3))]).astype(np.float32)
+    skip = (P[:, 2] > 0.35).astype(np.uint8)
+    idx = gpu.KnnIndex(ctx, P, k_hint=k)
+    ctx.profile_reset()
+    ctx.profile(True)
+    got, d2 = idx.query(Q, k, d2=True, skip=torch.from_numpy(skip))
+    ctx.profile(False)
+    assert ctx.counter("knn_query:sparse") == len(Q)
+    ref, rd2 = O.knn_brute(P, Q, k, skip=skip, return_d2=True)
+    assert np.array_equal(got.cpu().numpy(), ref)
+    assert np.array_equal(d2.cpu().numpy(), rd2)
