@@ -1042,6 +1042,9 @@ __global__ __launch_bounds__(kPyrThreads) void knn_pyramid_kernel(KnnGridArgs G,
 // far queries: the pyramid walk when there are enough of them to fill the
 // lanes, else (or without a pyramid) one wave per query on the coarse level
 constexpr int64_t kPyrMinQueries = 2048;
+// below this many queries (and fewer than one per 8 indexed points) a query
+// set skips the tiled passes for the pyramid walk
+constexpr int64_t kSparseMaxQueries = 65536;
 
 static void launch_far(dwg_ctx* ctx, const dwg_knn_index* I, const float* q, int64_t nq, const int* qlist, int k,
                        const uint8_t* skip, int32_t* out_idx, float* out_d2) {
@@ -1156,6 +1159,20 @@ void knn_query(dwg_ctx* ctx, const dwg_knn_index* I, const float* q, int64_t nq,
     const std::string base = tag ? tag : "knn_query";
     if (nq < 4096) {  // too few to pay for the bucketing
         launch_per_lane(ctx, I, q, nq, nullptr, k, skip, qmask, out_idx, out_d2);
+        if (sc) {
+            hipLaunchKernelGGL(scatter_pairs_kernel, dim3(grid_for(nq * k, 256)), dim3(256), 0, ctx->stream, out_idx, k,
+                               nullptr, nq, *sc);
+            check_launch();
+        }
+        return;
+    }
+    if (!qmask && nq < kSparseMaxQueries && nq * 8 < I->n && I->lvl[0].pyr_levels >= 2) {
+        // a few tens of thousands of queries spread thin over a large index
+        // (the zero fill's) leave the tiles nearly empty, so staging a tile's
+        // points serves one or two queries: walking the occupancy pyramid per
+        // lane costs less (1.2 vs 2.1 ms for 35k queries over 950k points)
+        ctx->count(base + ":sparse", nq);
+        launch_far(ctx, I, q, nq, nullptr, k, skip, out_idx, out_d2);
         if (sc) {
             hipLaunchKernelGGL(scatter_pairs_kernel, dim3(grid_for(nq * k, 256)), dim3(256), 0, ctx->stream, out_idx, k,
                                nullptr, nq, *sc);
