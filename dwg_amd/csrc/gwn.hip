@@ -46,6 +46,28 @@ __device__ __forceinline__ float gwn_term_r2(float dx, float dy, float dz, float
     }
 }
 
+// the same term in two factors, t = x · inv, with ok = (r > 1e-10) for the
+// screened form: a sum can then take it as fma(ok ? x : 0, inv, sum), one
+// rounding and one instruction fewer than t = x · inv; sum += ok ? t : 0
+template <bool SCREENED>
+__device__ __forceinline__ float gwn_term_split(float dx, float dy, float dz, float r2, float s, const float4 b,
+                                                float sigma, float sigma_l2e, float& inv, bool& ok) {
+    const float dot = __builtin_fmaf(dz, b.z, __builtin_fmaf(dy, b.y, dx * b.x));
+    const float r = fast_sqrt(r2);
+    if constexpr (!SCREENED) {
+        inv = fast_rcp(__builtin_fmaf(r2, r, kEpsPlain * kInv4Pi));
+        ok = true;
+        return dot;
+    } else {
+        const float rho = fmaxf(r, s);
+        const float e = fast_exp2(sigma_l2e * rho);
+        const float rho3 = rho * rho * rho;
+        inv = fast_rcp(__builtin_fmaf(e, rho3, kEpsScreened * kInv4Pi));
+        ok = r > 1e-10f;
+        return __builtin_fmaf(sigma, rho, 1.0f) * dot;
+    }
+}
+
 template <bool SCREENED>
 __device__ __forceinline__ float gwn_term_d(float dx, float dy, float dz, float s, const float4 b, float sigma,
                                             float sigma_l2e) {
@@ -401,8 +423,17 @@ __global__ __launch_bounds__(kTreeThreads) void gwn_tree_kernel(TreeArgs T, Quer
                 if (nk.x == -1) {
                     const float4 b = s_pay[wave][2 * c];
                     const float4 a = s_pay[wave][2 * c + 1];
-                    const float t = gwn_term<SCREENED>(qx, qy, qz, a, b, sigma, sl2e);
-                    part += act ? t : 0.f;
+                    if (BF16) {  // mixed precision: the term's last product fused into the sum
+                        const float dx = a.x - qx, dy = a.y - qy, dz = a.z - qz;
+                        float inv;
+                        bool ok;
+                        const float x = gwn_term_split<SCREENED>(dx, dy, dz, dist2(dx, dy, dz), a.w, b, sigma, sl2e,
+                                                                 inv, ok);
+                        part = __builtin_fmaf((act && ok) ? x : 0.f, inv, part);
+                    } else {
+                        const float t = gwn_term<SCREENED>(qx, qy, qz, a, b, sigma, sl2e);
+                        part += act ? t : 0.f;
+                    }
                 } else {
                     for (int j = nk.y; j < nk.y - nk.x; ++j) {
                         const float t = gwn_term<SCREENED>(qx, qy, qz, scalar_load(T.sa, j), scalar_load(T.sb, j), sigma, sl2e);
@@ -423,20 +454,24 @@ __global__ __launch_bounds__(kTreeThreads) void gwn_tree_kernel(TreeArgs T, Quer
             const uint64_t nm = mask & ~farm;
             if (mask & farm) {
                 const float4 e4 = s_pay[wave][2 * c];
-                float t;
                 if (BF16) {
                     // BF16 moments carry a relative error of 2^-9, far above
                     // the rounding of the centre: d = CM_f − q and its |d|²
-                    // from the far test serve as they are
-                    t = gwn_term_r2<SCREENED>(dx, dy, dz, r2, kScreenDefault, e4, sigma, sl2e);
+                    // from the far test serve as they are, and the term's
+                    // last product is fused into the sum
+                    float inv;
+                    bool ok;
+                    const float x = gwn_term_split<SCREENED>(dx, dy, dz, r2, kScreenDefault, e4, sigma, sl2e, inv, ok);
+                    part = __builtin_fmaf((act && far && ok) ? x : 0.f, inv, part);
                 } else {
                     // d = (CM_f − q) + δ: the first difference is exact for
                     // nearby values, so d carries a single rounding
                     const float4 lo = s_pay[wave][2 * c + 1];
-                    t = gwn_term_d<SCREENED>(dx + lo.x, dy + lo.y, dz + lo.z, kScreenDefault, e4, sigma, sl2e);
+                    const float t =
+                        gwn_term_d<SCREENED>(dx + lo.x, dy + lo.y, dz + lo.z, kScreenDefault, e4, sigma, sl2e);
+                    // the plain dipole has no r > 1e-10 guard in the reference either
+                    part += (act && far) ? t : 0.f;
                 }
-                // the plain dipole has no r > 1e-10 guard in the reference either
-                part += (act && far) ? t : 0.f;
                 if (COUNT && act && far) nterms += 1;
             }
             if (nm) {
