@@ -17,6 +17,7 @@ struct dwg_octree {
     float4* cm_r = nullptr;    // (float(CM), (β·R)² for the β of the last evaluation)
     float* radius = nullptr;   // float(R)
     float far_beta = -1.f;     // β that cm_r.w was computed for
+    float far_floor = -1.f;    // lower bound cm_r.w was clamped to (0 for the exact far test)
     float4* cm_lo = nullptr;   // (CM − float(CM), 0): residual of the rounded centre
     float4* ec = nullptr;      // (EC/4π, 0), refreshed per iteration
 

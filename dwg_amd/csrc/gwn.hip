@@ -458,11 +458,12 @@ __global__ __launch_bounds__(kTreeThreads) void gwn_tree_kernel(TreeArgs T, Quer
                     // BF16 moments carry a relative error of 2^-9, far above
                     // the rounding of the centre: d = CM_f − q and its |d|²
                     // from the far test serve as they are, and the term's
-                    // last product is fused into the sum
+                    // last product is fused into the sum.  r > 1e-10 holds
+                    // for every far node here (far radius floor)
                     float inv;
                     bool ok;
                     const float x = gwn_term_split<SCREENED>(dx, dy, dz, r2, kScreenDefault, e4, sigma, sl2e, inv, ok);
-                    part = __builtin_fmaf((act && far && ok) ? x : 0.f, inv, part);
+                    part = __builtin_fmaf((act && far) ? x : 0.f, inv, part);
                 } else {
                     // d = (CM_f − q) + δ: the first difference is exact for
                     // nearby values, so d carries a single rounding
@@ -523,13 +524,22 @@ static QueryArgs make_grid(const dwg_grid* g) {
 }
 
 // cm_r.w = (β·R)² for the traversal's far test d² > (β·R)², with the
-// rounding sequence float(β·R) then its square (the order the oracle uses)
-__global__ void far_radius_kernel(const float* __restrict__ r, int64_t m, float beta, float4* __restrict__ cm_r) {
+// rounding sequence float(β·R) then its square (the order the oracle uses),
+// clamped below by `floor`
+__global__ void far_radius_kernel(const float* __restrict__ r, int64_t m, float beta, float floor,
+                                  float4* __restrict__ cm_r) {
     const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i >= m) return;
     const float br = beta * r[i];
-    cm_r[i].w = br * br;
+    cm_r[i].w = fmaxf(br * br, floor);
 }
+
+// Far-radius floor of the mixed-precision walk: with d² > 4e-20 every far
+// node has r = |d| > 2e-10, so the screened term's r > 1e-10 guard always
+// holds there and the walk drops it.  Only a node whose points all lie within
+// 1e-10 of each other (duplicates) has (β·R)² below the floor; it is opened
+// instead, and its points' own guarded terms give the same sum.
+constexpr float kFarFloorBf16 = 4e-20f;
 
 // Far-field moments in the payload of every internal node: EC/4π in FP32,
 // or rounded to BF16 (to nearest even, kept in FP32 words) for BF16
@@ -547,13 +557,15 @@ static void launch_tree(dwg_ctx* ctx, const dwg_octree* Tc, const QueryArgs& Q, 
     DWG_REQUIRE(Tc && Tc->updated, "octree has no moments yet: call dwg_octree_update first");
     DWG_REQUIRE(Tc->max_level <= 21, "octree deeper than the traversal stack");
     auto* T = const_cast<dwg_octree*>(Tc);  // the far radii follow β; the tree itself is unchanged
-    if (!(T->far_beta == beta)) {
+    const bool want_bf16 = (flags & DWG_FAR_BF16) && !count;
+    const float floor = want_bf16 ? kFarFloorBf16 : 0.f;
+    if (!(T->far_beta == beta) || T->far_floor != floor) {
         hipLaunchKernelGGL(far_radius_kernel, dim3(grid_for(T->n_nodes, 256)), dim3(256), 0, ctx->stream, T->radius,
-                           T->n_nodes, beta, T->cm_r);
+                           T->n_nodes, beta, floor, T->cm_r);
         check_launch();
         T->far_beta = beta;
+        T->far_floor = floor;
     }
-    const bool want_bf16 = (flags & DWG_FAR_BF16) && !count;
     if (T->payload_bf16 != want_bf16) {
         hipLaunchKernelGGL(payload_precision_kernel, dim3(grid_for(T->n_nodes, 256)), dim3(256), 0, ctx->stream, T->kid,
                            T->ec, T->n_nodes, int(want_bf16), T->payload);
