@@ -546,7 +546,8 @@ __global__ __launch_bounds__(kTileThreads) void knn_tile_kernel(
     KnnGridArgs G, TileArgs T, const int64_t* __restrict__ cell_start, const float4* __restrict__ pts,
     const int* __restrict__ tiles, const int64_t* __restrict__ tile_start, const int* __restrict__ order,
     const float* __restrict__ q, int k, const uint8_t* __restrict__ skip, int32_t* __restrict__ out_idx,
-    float* __restrict__ out_d2, int* __restrict__ redo, unsigned int* __restrict__ n_redo, KnnScatter sc) {
+    float* __restrict__ out_d2, int* __restrict__ redo, float* __restrict__ redo_kth, unsigned int* __restrict__ n_redo,
+    KnnScatter sc) {
     constexpr int kRegion = kTile + 2 * HALO;
     constexpr int kRows = kRegion * kRegion;
     constexpr int kHash = SCATTER ? 512 : 1;
@@ -665,7 +666,10 @@ __global__ __launch_bounds__(kTileThreads) void knn_tile_kernel(
             exact = safe > 0.f && kd2 < safe * safe;
         }
         if (!exact) {
-            redo[atomicAdd(n_redo, 1u)] = qi;
+            // the k candidates it has bound the k-th distance from above
+            const unsigned int slot = atomicAdd(n_redo, 1u);
+            redo[slot] = qi;
+            if (redo_kth) redo_kth[slot] = kd2;
             continue;
         }
         knn_store<K>(best, k, qi, out_idx, out_d2);
@@ -974,6 +978,7 @@ __global__ __launch_bounds__(kPyrThreads) void knn_pyramid_kernel(KnnGridArgs G,
                                                                   const float* __restrict__ q, int64_t nq,
                                                                   const int* __restrict__ qlist, int k,
                                                                   const uint8_t* __restrict__ skip,
+                                                                  const float* __restrict__ init_kth,
                                                                   int32_t* __restrict__ out_idx,
                                                                   float* __restrict__ out_d2) {
     extern __shared__ uint64_t stk[];  // Y.levels × kPyrThreads entries
@@ -990,6 +995,10 @@ __global__ __launch_bounds__(kPyrThreads) void knn_pyramid_kernel(KnnGridArgs G,
     const float qx = q[3 * i], qy = q[3 * i + 1], qz = q[3 * i + 2];
     uint64_t best[K];
     knn_reset<K>(best, k);
+    // an upper bound of the k-th d² known beforehand (k points at most that
+    // far exist): blocks beyond it are never opened.  A point at exactly the
+    // bound has a box distance ≤ it, so ties are still seen.
+    const float cap = init_kth ? init_kth[t] : INFINITY;
     uint64_t* st = stk + threadIdx.x;
     const int top = Y.levels - 1;
     st[top * kPyrThreads] = pyr_entry(0, 0, 0, Y.mask[s_off[top]]);
@@ -1011,7 +1020,7 @@ __global__ __launch_bounds__(kPyrThreads) void knn_pyramid_kernel(KnnGridArgs G,
                 bc = c;
             }
         }
-        const float kth = best[0] == ~0ull ? INFINITY : __uint_as_float(uint32_t(best[0] >> 32));
+        const float kth = best[0] == ~0ull ? cap : fminf(cap, __uint_as_float(uint32_t(best[0] >> 32)));
         if (bc < 0 || !(bd <= kth)) {  // nothing left here can hold a closer point
             ++l;
             continue;
@@ -1047,7 +1056,7 @@ constexpr int64_t kPyrMinQueries = 2048;
 constexpr int64_t kSparseMaxQueries = 65536;
 
 static void launch_far(dwg_ctx* ctx, const dwg_knn_index* I, const float* q, int64_t nq, const int* qlist, int k,
-                       const uint8_t* skip, int32_t* out_idx, float* out_d2) {
+                       const uint8_t* skip, int32_t* out_idx, float* out_d2, const float* init_kth = nullptr) {
     if (nq == 0) return;
     const auto& L = I->lvl[0];
     if (L.pyr_levels < 2 || nq < kPyrMinQueries) {
@@ -1059,7 +1068,7 @@ static void launch_far(dwg_ctx* ctx, const dwg_knn_index* I, const float* q, int
     const size_t lds = sizeof(uint64_t) * size_t(L.pyr_levels) * kPyrThreads;
 #define DWG_KNNP(KK)                                                                                             \
     hipLaunchKernelGGL(knn_pyramid_kernel<KK>, grid, block, lds, ctx->stream, L.G, Y, L.cell_start, L.pts, q, nq, \
-                       qlist, k, skip, out_idx, out_d2)
+                       qlist, k, skip, init_kth, out_idx, out_d2)
     if (k <= 8) DWG_KNNP(8);
     else if (k <= 10) DWG_KNNP(10);
     else if (k <= 16) DWG_KNNP(16);
@@ -1090,7 +1099,8 @@ static void launch_per_lane(dwg_ctx* ctx, const dwg_knn_index* I, const float* q
 template <int HALO>
 static int64_t tiled_pass(dwg_ctx* ctx, const dwg_knn_index* I, const float* q, int64_t nq, const int* qlist,
                           const uint8_t* qmask, int k, const uint8_t* skip, int32_t* out_idx, float* out_d2,
-                          int* redo, const std::string& name, const KnnScatter* sc = nullptr) {
+                          int* redo, const std::string& name, const KnnScatter* sc = nullptr,
+                          float* redo_kth = nullptr) {
     const KnnGridArgs& G = I->lvl[0].G;
     TileArgs T;
     for (int d = 0; d < 3; ++d) T.tdim[d] = (G.dim[d] + kTile - 1) / kTile;
@@ -1129,11 +1139,11 @@ static int64_t tiled_pass(dwg_ctx* ctx, const dwg_knn_index* I, const float* q, 
         if (sc)                                                                                                        \
             hipLaunchKernelGGL((knn_tile_kernel<KK, HALO, true>), dim3(unsigned(nlist)), dim3(kTileThreads), 0,        \
                                ctx->stream, G, T, I->lvl[0].cell_start, I->lvl[0].pts, list, start, order, q, k, skip, \
-                               out_idx, out_d2, redo, n_redo, scv);                                                    \
+                               out_idx, out_d2, redo, redo_kth, n_redo, scv);                                                    \
         else                                                                                                           \
             hipLaunchKernelGGL((knn_tile_kernel<KK, HALO, false>), dim3(unsigned(nlist)), dim3(kTileThreads), 0,       \
                                ctx->stream, G, T, I->lvl[0].cell_start, I->lvl[0].pts, list, start, order, q, k, skip, \
-                               out_idx, out_d2, redo, n_redo, scv);                                                    \
+                               out_idx, out_d2, redo, redo_kth, n_redo, scv);                                                    \
     } while (0)
         if (k <= 8) DWG_KNNT(8);
         else if (k <= 10) DWG_KNNT(10);
@@ -1190,13 +1200,14 @@ void knn_query(dwg_ctx* ctx, const dwg_knn_index* I, const float* q, int64_t nq,
     const int64_t n2 = n1 > 0 ? tiled_pass<kHalo2>(ctx, I, q, n1, redo1, nullptr, k, skip, out_idx, out_d2, redo2,
                                                    base + ":tiles2", sc)
                               : 0;
+    float* kth3 = static_cast<float*>(ctx->buf("knnt_kth3", sizeof(float) * nq + 64));
     const int64_t n3 = n2 > 0 ? tiled_pass<kHalo3>(ctx, I, q, n2, redo2, nullptr, k, skip, out_idx, out_d2, redo3,
-                                                   base + ":tiles3", sc)
+                                                   base + ":tiles3", sc, kth3)
                               : 0;
     {
         const std::string name = base + ":redo";
         ProfileScope pr(ctx, name.c_str());
-        launch_far(ctx, I, q, n3, redo3, k, skip, out_idx, out_d2);
+        launch_far(ctx, I, q, n3, redo3, k, skip, out_idx, out_d2, kth3);
     }
     if (sc && n3 > 0) {
         const std::string name = base + ":scatter";
