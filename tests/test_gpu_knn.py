@@ -3,8 +3,9 @@ queries), bit-exact against the oracle's brute-force ranking by
 (float d², index): k up to 32, queries outside the point bounds, a dense
 cluster that overflows one LDS stage, exact duplicates (ties), a skipped
 region large enough to send queries to the per-lane fallback, and a query
-mask; and a sparse query set (fewer than one query per 8 points) that skips
-the tiles for the pyramid walk.
+mask; a sparse query set (fewer than one query per 8 points) that skips
+the tiles for the pyramid walk; and far queries of a dense query set, which
+share one pyramid walk per wave.
 """
 import numpy as np
 import pytest
@@ -104,5 +105,28 @@ def test_knn_sparse_queries_pyramid(ctx, k):
     ctx.profile(False)
     assert ctx.counter("knn_query:sparse") == len(Q)
     ref, rd2 = O.knn_brute(P, Q, k, skip=skip, return_d2=True)
+    assert np.array_equal(got.cpu().numpy(), ref)
+    assert np.array_equal(d2.cpu().numpy(), rd2)
+
+
+@pytest.mark.parametrize("k", [10, 16])
+def test_knn_far_queries_wave_shared(ctx, k):
+    # as many queries as points, most of them on a dense sheet far from every
+    # point (a spurious surface piece's centroids): the far ones share one
+    # pyramid walk per wave
+    rng = np.random.default_rng(70 + k)
+    S, _ = shapes.sphere(20_000, 71)
+    P = np.concatenate([0.05 * S[:10_000] + [-0.8, 0, 0], 0.05 * S[10_000:] + [0.8, 0.3, -0.2]]).astype(np.float32)
+    r = 0.3 * np.sqrt(rng.uniform(size=20_000))
+    phi = rng.uniform(0, 2 * np.pi, 20_000)
+    sheet = np.stack([rng.normal(0, 0.002, 20_000), r * np.cos(phi), r * np.sin(phi)], 1)
+    Q = np.concatenate([sheet, P[rng.integers(0, len(P), 4000)] + rng.normal(0, 0.003, (4000, 3))]).astype(np.float32)
+    idx = gpu.KnnIndex(ctx, P, k_hint=k)
+    ctx.profile_reset()
+    ctx.profile(True)
+    got, d2 = idx.query(Q, k, d2=True)
+    ctx.profile(False)
+    assert ctx.counter("knn_query:far_shared") >= 2048
+    ref, rd2 = O.knn_brute(P, Q, k, return_d2=True)
     assert np.array_equal(got.cpu().numpy(), ref)
     assert np.array_equal(d2.cpu().numpy(), rd2)
