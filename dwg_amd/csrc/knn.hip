@@ -933,6 +933,7 @@ static void launch_wave(dwg_ctx* ctx, const dwg_knn_index* I, const float* q, in
     if (k <= 8) DWG_KNNW(8);
     else if (k <= 10) DWG_KNNW(10);
     else if (k <= 16) DWG_KNNW(16);
+    else if (k <= 17) DWG_KNNW(17);  // the learned start's 16-point patches with the point itself
     else DWG_KNNW(32);
 #undef DWG_KNNW
     check_launch();
@@ -1203,6 +1204,7 @@ static void launch_far(dwg_ctx* ctx, const dwg_knn_index* I, const float* q, int
         if (k <= 8) DWG_KNNPW(8);
         else if (k <= 10) DWG_KNNPW(10);
         else if (k <= 16) DWG_KNNPW(16);
+        else if (k <= 17) DWG_KNNPW(17);  // the learned start's 16-point patches with the point itself
         else DWG_KNNPW(32);
 #undef DWG_KNNPW
         check_launch();
@@ -1216,6 +1218,7 @@ static void launch_far(dwg_ctx* ctx, const dwg_knn_index* I, const float* q, int
     if (k <= 8) DWG_KNNP(8);
     else if (k <= 10) DWG_KNNP(10);
     else if (k <= 16) DWG_KNNP(16);
+    else if (k <= 17) DWG_KNNP(17);  // the learned start's 16-point patches with the point itself
     else DWG_KNNP(32);
 #undef DWG_KNNP
     check_launch();
@@ -1233,6 +1236,7 @@ static void launch_per_lane(dwg_ctx* ctx, const dwg_knn_index* I, const float* q
     if (k <= 8) DWG_KNN(8);
     else if (k <= 10) DWG_KNN(10);
     else if (k <= 16) DWG_KNN(16);
+    else if (k <= 17) DWG_KNN(17);  // the learned start's 16-point patches with the point itself
     else DWG_KNN(32);
 #undef DWG_KNN
     check_launch();
@@ -1292,6 +1296,7 @@ static int64_t tiled_pass(dwg_ctx* ctx, const dwg_knn_index* I, const float* q, 
         if (k <= 8) DWG_KNNT(8);
         else if (k <= 10) DWG_KNNT(10);
         else if (k <= 16) DWG_KNNT(16);
+        else if (k <= 17) DWG_KNNT(17);  // the learned start's 16-point patches with the point itself
         else DWG_KNNT(32);
 #undef DWG_KNNT
         check_launch();
