@@ -78,6 +78,10 @@ struct dwg_ctx {
     void* pinned = nullptr;
 
     void* buf(const std::string& name, size_t bytes) { return scratch[name].ensure(bytes); }
+    size_t buf_size(const std::string& name) const {
+        const auto it = scratch.find(name);
+        return it == scratch.end() ? 0 : it->second.bytes;
+    }
     void count(const std::string& name, int64_t v) {
         if (profile) counters[name + profile_suffix] += v;
     }

@@ -16,6 +16,7 @@
 #include <random>
 #include <sstream>
 #include <stdexcept>
+#include <unordered_map>
 #include <unordered_set>
 
 #include "common.h"
@@ -33,14 +34,35 @@ static double seconds_since(Clock::time_point t) {
 }
 
 static std::mutex g_pinned_mu;
-static std::unordered_set<void*> g_pinned;
+static std::unordered_map<void*, size_t> g_pinned;             // live pinned buffers and their sizes
+static std::vector<std::pair<void*, size_t>> g_pinned_kept;    // released ones, kept for reuse
+constexpr size_t kPinnedKept = 2;
 
+// Pinned host memory for meshes handed to the caller.  Pinning a buffer
+// costs milliseconds per hundred MB, so up to two released buffers are kept
+// and reused by later allocations of up to the same size.
 float* alloc_host_pinned(size_t bytes) {
+    {
+        std::lock_guard<std::mutex> lock(g_pinned_mu);
+        size_t best = g_pinned_kept.size();
+        for (size_t i = 0; i < g_pinned_kept.size(); ++i) {
+            const size_t sz = g_pinned_kept[i].second;
+            if (sz >= bytes && sz <= 2 * bytes + (size_t(1) << 20) &&
+                (best == g_pinned_kept.size() || sz < g_pinned_kept[best].second))
+                best = i;
+        }
+        if (best < g_pinned_kept.size()) {
+            const auto kept = g_pinned_kept[best];
+            g_pinned_kept.erase(g_pinned_kept.begin() + std::ptrdiff_t(best));
+            g_pinned[kept.first] = kept.second;
+            return static_cast<float*>(kept.first);
+        }
+    }
     void* p = nullptr;
     if (hipHostMalloc(&p, bytes, hipHostMallocDefault) != hipSuccess || !p)
         throw dwg::Error(DWG_ERR_HIP, "pinned host allocation failed");
     std::lock_guard<std::mutex> lock(g_pinned_mu);
-    g_pinned.insert(p);
+    g_pinned[p] = bytes;
     return static_cast<float*>(p);
 }
 
@@ -48,8 +70,15 @@ void free_host(void* p) {
     if (!p) return;
     {
         std::lock_guard<std::mutex> lock(g_pinned_mu);
-        if (g_pinned.erase(p)) {
-            (void)hipHostFree(p);
+        const auto it = g_pinned.find(p);
+        if (it != g_pinned.end()) {
+            const size_t sz = it->second;
+            g_pinned.erase(it);
+            if (g_pinned_kept.size() < kPinnedKept) {
+                g_pinned_kept.emplace_back(p, sz);
+            } else {
+                (void)hipHostFree(p);
+            }
             return;
         }
     }
@@ -405,28 +434,42 @@ void TurboDWG::mesh_pass(const Grid& grid, float iso_scale, int64_t* tri_count, 
     // confined to the iterations
     const uint32_t flags = DWG_SCREENED | (opt_.use_radii ? DWG_USE_RADII : 0u) | (opt_.precision & ~DWG_FAR_BF16);
     const dwg_grid g = grid.abi();
-    DevArray<float> wg(grid.size());
+    // field and triangles in the context's scratch space: a second
+    // reconstruction on the same context allocates nothing here
+    float* wg = static_cast<float*>(ctx_->buf("final_field", sizeof(float) * size_t(grid.size())));
     const float* rad = opt_.use_radii ? radii_.ptr : nullptr;
     check(dwg_octree_update(ctx_, tree_, nrm_.ptr, area_.ptr, rad), "moments");
     if (opt_.As_threshold > 0) check(dwg_outlier_filter(ctx_, area_.ptr, n_, opt_.As_threshold), "outlier filter");
     const float beta = opt_.beta > 0 ? opt_.beta : 1e30f;  // β → ∞ takes every point term
     check(dwg_gwn_tree_self(ctx_, tree_, beta, opt_.sigma, flags, w_src_.ptr), "gwn points");
-    check(dwg_gwn_tree_grid(ctx_, tree_, &g, beta, opt_.sigma, flags, wg.ptr), "gwn final grid");
+    check(dwg_gwn_tree_grid(ctx_, tree_, &g, beta, opt_.sigma, flags, wg), "gwn final grid");
     double sum = 0;
     check(dwg_reduce_sum(ctx_, w_src_.ptr, n_, &sum), "reduce");
     const float iso = float(sum / double(n_)) * iso_scale;
     int64_t ntri = 0;
-    int rc = dwg_marching_cubes(ctx_, wg.ptr, &g, iso, 0, nullptr, nullptr, nullptr, &ntri);
-    if (rc != DWG_OK && rc != DWG_ERR_CAPACITY) check(rc, "marching cubes");
-    if (tris) {
-        DevArray<float> t(9 * std::max<int64_t>(ntri, 1));
-        check(dwg_marching_cubes(ctx_, wg.ptr, &g, iso, ntri, t.ptr, nullptr, nullptr, &ntri), "marching cubes");
+    if (!tris) {
+        const int rc = dwg_marching_cubes(ctx_, wg, &g, iso, 0, nullptr, nullptr, nullptr, &ntri);
+        if (rc != DWG_OK && rc != DWG_ERR_CAPACITY) check(rc, "marching cubes");
+    } else {
+        // count and emit in one call when the kept triangle buffer is large
+        // enough, else once more with a buffer of the counted size
+        size_t cap = ctx_->buf_size("final_tris") / (9 * sizeof(float));
+        float* t = static_cast<float*>(ctx_->buf("final_tris", sizeof(float) * 9 * std::max<size_t>(cap, 1)));
+        int rc = dwg_marching_cubes(ctx_, wg, &g, iso, int64_t(cap), t, nullptr, nullptr, &ntri);
+        if (rc == DWG_ERR_CAPACITY) {
+            cap = size_t(ntri + ntri / 8 + 1024);
+            t = static_cast<float*>(ctx_->buf("final_tris", sizeof(float) * 9 * cap));
+            rc = dwg_marching_cubes(ctx_, wg, &g, iso, int64_t(cap), t, nullptr, nullptr, &ntri);
+        }
+        check(rc, "marching cubes");
         // straight into the buffer the caller receives (no second host copy)
         free_host(tris->data);
         tris->data = nullptr;
         tris->data = alloc_host_pinned(sizeof(float) * 9 * size_t(std::max<int64_t>(ntri, 1)));
         tris->ntri = ntri;
-        t.download(tris->data, 9 * ntri, ctx_->stream);
+        DWG_HIP_CHECK(hipMemcpyAsync(tris->data, t, sizeof(float) * 9 * size_t(ntri), hipMemcpyDeviceToHost,
+                                     ctx_->stream));
+        DWG_HIP_CHECK(hipStreamSynchronize(ctx_->stream));
     }
     *tri_count = ntri;
 }

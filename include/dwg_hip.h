@@ -292,7 +292,9 @@ typedef struct dwg_result {
  * for the STL / .xyz outputs as the reference names them. */
 int dwg_launch(dwg_ctx* ctx, const float* h_pts, const float* h_nrm, int64_t n, const dwg_options* opt,
                const char* out_base, dwg_result* res);
-/* Releases memory the library allocated for a caller (dwg_result.mesh). */
+/* Releases memory the library allocated for a caller (dwg_result.mesh).  Up
+ * to two released pinned buffers are kept for reuse by later meshes of up to
+ * their size (pinning costs milliseconds per hundred MB). */
 void dwg_free(void* p);
 
 #ifdef __cplusplus
