@@ -275,8 +275,8 @@ def main():
         large_config(args)
         return
     if args.gpus != 1:
-        print(f"bench.py measures one GPU only: --gpus {args.gpus} is not supported in this build "
-              "(every box has one GPU; the sharding design is described in DESIGN.md)", file=sys.stderr)
+        print(f"bench.py measures one GPU only: --gpus {args.gpus} is not supported (the reconstruction "
+              "runs on a single GPU; sharding it by grid slabs across GPUs is not implemented)", file=sys.stderr)
         sys.exit(2)
     import torch
     from dwg_amd import gpu
