@@ -50,18 +50,26 @@ __device__ __forceinline__ int cube_case(const McArgs& A, int x, int y, int z, f
 // One block per brick of 8 × 8 × 4 cubes (x fastest inside the brick,
 // bricks in x, y, z order): a surface crosses few bricks, so the emit pass
 // returns at once from the many bricks without triangles.
-constexpr int kBx = 8, kBy = 8, kBz = 4;
-static_assert(kBx * kBy * kBz == kMcThreads, "one cube per thread");
+// a brick is 8 × 8 × 16 cubes: each of the 256 threads takes a column of
+// kZPer = 4 cubes along z, so a launch has a quarter of the blocks a cube
+// per thread would need (short blocks are bound by their dispatch), and
+// consecutive cubes of a column share 4 corner loads
+constexpr int kBx = 8, kBy = 8, kZPer = 4, kBz = 4 * kZPer;
+static_assert(kBx * kBy * (kBz / kZPer) == kMcThreads, "one column of kZPer cubes per thread");
 
 struct McBricks {
     int nbx, nby, nbz;
 };
 
-__device__ __forceinline__ bool cube_of(const McArgs& A, const McBricks& B, int& x, int& y, int& z) {
+// the first cube (x, y, z0) of this thread's column
+__device__ __forceinline__ void column_of(const McBricks& B, int& x, int& y, int& z0) {
     const int64_t b = blockIdx.x;
     x = int(b % B.nbx) * kBx + (threadIdx.x & (kBx - 1));
     y = int((b / B.nbx) % B.nby) * kBy + ((threadIdx.x / kBx) & (kBy - 1));
-    z = int(b / (int64_t(B.nbx) * B.nby)) * kBz + threadIdx.x / (kBx * kBy);
+    z0 = int(b / (int64_t(B.nbx) * B.nby)) * kBz + (threadIdx.x / (kBx * kBy)) * kZPer;
+}
+
+__device__ __forceinline__ bool cube_valid(const McArgs& A, int x, int y, int z) {
     return x < A.rx - 1 && y < A.ry - 1 && z < A.rz - 1;
 }
 
@@ -92,32 +100,12 @@ __device__ int block_exclusive_sum(int v, int& total) {
     return before + incl - v;
 }
 
-__global__ __launch_bounds__(kMcThreads) void mc_count_kernel(McArgs A, McBricks B, int64_t* __restrict__ tile_counts) {
-    int x, y, z, cnt = 0;
-    if (cube_of(A, B, x, y, z)) {
-        float v[8];
-        cnt = int(c_mc_rows[cube_case(A, x, y, z, v)] >> 60);
-    }
-    int total;
-    (void)block_exclusive_sum(cnt, total);
-    if (threadIdx.x == 0) tile_counts[blockIdx.x] = total;
-}
-
-__global__ __launch_bounds__(kMcThreads) void mc_emit_kernel(McArgs A, McBricks B, const int64_t* __restrict__ tile_offsets,
-                                                             float* __restrict__ tris, float* __restrict__ cent,
-                                                             float* __restrict__ nrm) {
-    if (tile_offsets[blockIdx.x + 1] == tile_offsets[blockIdx.x]) return;  // no triangle in this brick
-    int x, y, z, cnt = 0, cs = 0;
-    float v[8];
-    if (cube_of(A, B, x, y, z)) {
-        cs = cube_case(A, x, y, z, v);
-        cnt = int(c_mc_rows[cs] >> 60);
-    }
-    int total;
-    const int64_t base = tile_offsets[blockIdx.x] + block_exclusive_sum(cnt, total);
-    if (cnt == 0) return;
-    const uint64_t row = c_mc_rows[cs];
-    for (int t = 0; t < cnt; ++t) {
+// the nt triangles of the cube (x, y, z) with corner values v and table row
+// `row`, written from slot o on
+__device__ __forceinline__ void emit_cube(const McArgs& A, int x, int y, int z, const float (&v)[8], uint64_t row,
+                                          int nt, int64_t o0, float* __restrict__ tris, float* __restrict__ cent,
+                                          float* __restrict__ nrm) {
+    for (int t = 0; t < nt; ++t) {
         float P[3][3];
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
@@ -134,7 +122,7 @@ __global__ __launch_bounds__(kMcThreads) void mc_emit_kernel(McArgs A, McBricks 
 #pragma unroll
             for (int d = 0; d < 3; ++d) P[k][d] = pa[d] - ell * (pb[d] - pa[d]);
         }
-        const int64_t o = base + t;
+        const int64_t o = o0 + t;
         if (tris) {
 #pragma unroll
             for (int k = 0; k < 3; ++k)
@@ -155,6 +143,53 @@ __global__ __launch_bounds__(kMcThreads) void mc_emit_kernel(McArgs A, McBricks 
         }
     }
 }
+
+__global__ __launch_bounds__(kMcThreads) void mc_count_kernel(McArgs A, McBricks B, int64_t* __restrict__ tile_counts) {
+    int x, y, z0, cnt = 0;
+    column_of(B, x, y, z0);
+#pragma unroll
+    for (int j = 0; j < kZPer; ++j) {
+        if (cube_valid(A, x, y, z0 + j)) {
+            float v[8];
+            cnt += int(c_mc_rows[cube_case(A, x, y, z0 + j, v)] >> 60);
+        }
+    }
+    int total;
+    (void)block_exclusive_sum(cnt, total);
+    if (threadIdx.x == 0) tile_counts[blockIdx.x] = total;
+}
+
+__global__ __launch_bounds__(kMcThreads) void mc_emit_kernel(McArgs A, McBricks B, const int64_t* __restrict__ tile_offsets,
+                                                             float* __restrict__ tris, float* __restrict__ cent,
+                                                             float* __restrict__ nrm) {
+    if (tile_offsets[blockIdx.x + 1] == tile_offsets[blockIdx.x]) return;  // no triangle in this brick
+    int x, y, z0, cnt = 0;
+    column_of(B, x, y, z0);
+    int cases[kZPer];
+#pragma unroll
+    for (int j = 0; j < kZPer; ++j) {
+        cases[j] = 0;
+        if (cube_valid(A, x, y, z0 + j)) {
+            float v[8];
+            cases[j] = cube_case(A, x, y, z0 + j, v);
+            cnt += int(c_mc_rows[cases[j]] >> 60);
+        }
+    }
+    int total;
+    int64_t o = tile_offsets[blockIdx.x] + block_exclusive_sum(cnt, total);
+    if (cnt == 0) return;
+    for (int j = 0; j < kZPer; ++j) {  // the column's cubes in z order, each in table order
+        const int z = z0 + j;
+        const uint64_t row = c_mc_rows[cases[j]];
+        const int nt = int(row >> 60);
+        if (nt == 0) continue;
+        float v[8];
+        (void)cube_case(A, x, y, z, v);
+        emit_cube(A, x, y, z, v, row, nt, o, tris, cent, nrm);
+        o += nt;
+    }
+}
+
 
 int64_t marching_cubes(dwg_ctx* ctx, const float* field, const dwg_grid* g, float iso, int64_t capacity, float* tris,
                        float* cent, float* nrm) {

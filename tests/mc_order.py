@@ -1,15 +1,15 @@
 """Puts marching-cubes output in the GPU kernel's order.
 
-The GPU emits one brick of 8 × 8 × 4 cubes per block: bricks in x, y, z
-order, cubes in x, y, z order inside a brick, each cube's triangles in table
-order.  The oracle emits in plain cube order (x fastest).  `to_brick_order`
+The GPU emits one brick of 8 × 8 × 16 cubes per block: bricks in x, y, z
+order; inside a brick, threads in x, y, z order, each thread a column of 4
+cubes along z in z order; each cube's triangles in table order.  The oracle emits in plain cube order (x fastest).  `to_brick_order`
 returns the permutation that takes the oracle's triangles to the GPU's.
 """
 import numpy as np
 
 import oracle as O
 
-BX, BY, BZ = 8, 8, 4
+BX, BY, BZ, ZPER = 8, 8, 16, 4
 
 
 def _corner(i):
@@ -34,5 +34,6 @@ def to_brick_order(field, res, iso):
     z = cube // ((rx - 1) * (ry - 1))
     nbx, nby = (rx - 1 + BX - 1) // BX, (ry - 1 + BY - 1) // BY
     brick = (x // BX) + nbx * ((y // BY) + nby * (z // BZ))
-    inner = (x % BX) + BX * ((y % BY) + BY * (z % BZ))
+    thread = (x % BX) + BX * ((y % BY) + BY * ((z % BZ) // ZPER))
+    inner = thread * ZPER + z % ZPER
     return np.lexsort((np.arange(len(cube)), inner, brick))
