@@ -496,28 +496,55 @@ __device__ __forceinline__ int tile_of_point(const KnnGridArgs& G, const TileArg
 }
 
 // query t of the pass is qlist[t] (or t); masked queries are left out
+// Queries arrive in runs that share a tile (triangle centroids in brick
+// order): each wave adds one run with one atomic.  A run starts at lane 0
+// and wherever the tile changes; its first lane (the head) reserves the
+// run's slots, the others take theirs from the head in lane order.
+__device__ __forceinline__ uint64_t run_starts(int tl) {
+    const int lane = threadIdx.x & 63;
+    const int prev = __shfl_up(tl, 1, 64);
+    return __ballot(lane == 0 || tl != prev);
+}
+
+// lanes from `lane` up to the next run start (or the end of the wave)
+__device__ __forceinline__ int run_length(uint64_t starts, int lane) {
+    const uint64_t after = lane == 63 ? 0ull : starts >> (lane + 1);
+    return after ? __builtin_ctzll(after) + 1 : 64 - lane;
+}
+
+// the run start at or before `lane`
+__device__ __forceinline__ int run_head(uint64_t starts, int lane) {
+    const uint64_t upto = lane == 63 ? starts : starts & ((2ull << lane) - 1ull);
+    return 63 - __builtin_clzll(upto);
+}
+
 __global__ void tile_count_kernel(const float* __restrict__ q, int64_t nq, const int* __restrict__ qlist,
                                   const uint8_t* __restrict__ qmask, KnnGridArgs G, TileArgs T,
                                   int* __restrict__ tile_of, unsigned long long* __restrict__ count) {
     const int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (t >= nq) return;
-    const int64_t i = qlist ? int64_t(qlist[t]) : t;
-    if (qmask && !qmask[i]) {
-        tile_of[t] = -1;
-        return;
+    int tl = -1;  // every lane takes part in the run detection
+    if (t < nq) {
+        const int64_t i = qlist ? int64_t(qlist[t]) : t;
+        if (!qmask || qmask[i]) tl = tile_of_point(G, T, q[3 * i], q[3 * i + 1], q[3 * i + 2]);
+        tile_of[t] = tl;
     }
-    const int tl = tile_of_point(G, T, q[3 * i], q[3 * i + 1], q[3 * i + 2]);
-    tile_of[t] = tl;
-    atomicAdd(&count[tl], 1ull);
+    const int lane = threadIdx.x & 63;
+    const uint64_t starts = run_starts(tl);
+    if (tl >= 0 && ((starts >> lane) & 1ull)) atomicAdd(&count[tl], (unsigned long long)run_length(starts, lane));
 }
 
 __global__ void tile_scatter_kernel(const int* __restrict__ tile_of, int64_t nq, const int* __restrict__ qlist,
                                     unsigned long long* __restrict__ cursor, int* __restrict__ order) {
     const int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (t >= nq) return;
-    const int tl = tile_of[t];
-    if (tl < 0) return;
-    order[atomicAdd(&cursor[tl], 1ull)] = qlist ? qlist[t] : int(t);
+    const int tl = t < nq ? tile_of[t] : -1;
+    const int lane = threadIdx.x & 63;
+    const uint64_t starts = run_starts(tl);
+    int base = 0;
+    if (tl >= 0 && ((starts >> lane) & 1ull))
+        base = int(atomicAdd(&cursor[tl], (unsigned long long)run_length(starts, lane)));
+    const int head = run_head(starts, lane);
+    const int hb = __shfl(base, head, 64);
+    if (tl >= 0) order[hb + (lane - head)] = qlist ? qlist[t] : int(t);
 }
 
 __global__ void tile_flags_kernel(const int64_t* __restrict__ start, int64_t ntiles, int64_t* __restrict__ flags) {
