@@ -33,24 +33,58 @@ __device__ __forceinline__ int cell_coord(float v, float lo, float inv_cell, int
     return min(max(c, 0), dim - 1);
 }
 
+// Points and queries arrive in runs that share a cell or tile (sources in
+// tree order, triangle centroids in brick order): each wave adds one run
+// with one atomic.  A run starts at lane 0 and wherever the key changes; its
+// first lane (the head) reserves the run's slots, the others take theirs
+// from the head in lane order.
+__device__ __forceinline__ uint64_t run_starts(int tl) {
+    const int lane = threadIdx.x & 63;
+    const int prev = __shfl_up(tl, 1, 64);
+    return __ballot(lane == 0 || tl != prev);
+}
+
+// lanes from `lane` up to the next run start (or the end of the wave)
+__device__ __forceinline__ int run_length(uint64_t starts, int lane) {
+    const uint64_t after = lane == 63 ? 0ull : starts >> (lane + 1);
+    return after ? __builtin_ctzll(after) + 1 : 64 - lane;
+}
+
+// the run start at or before `lane`
+__device__ __forceinline__ int run_head(uint64_t starts, int lane) {
+    const uint64_t upto = lane == 63 ? starts : starts & ((2ull << lane) - 1ull);
+    return 63 - __builtin_clzll(upto);
+}
+
 __global__ void knn_count_kernel(const float* __restrict__ p, int64_t n, KnnGridArgs G,
                                  unsigned long long* __restrict__ counts, int* __restrict__ cell_of) {
-    int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const int cx = cell_coord(p[3 * i], G.lo[0], G.inv_cell, G.dim[0]);
-    const int cy = cell_coord(p[3 * i + 1], G.lo[1], G.inv_cell, G.dim[1]);
-    const int cz = cell_coord(p[3 * i + 2], G.lo[2], G.inv_cell, G.dim[2]);
-    const int64_t c = cx + int64_t(G.dim[0]) * (cy + int64_t(G.dim[1]) * cz);
-    cell_of[i] = int(c);
-    atomicAdd(&counts[c], 1ull);
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    int c = -1;  // every lane takes part in the run detection
+    if (i < n) {
+        const int cx = cell_coord(p[3 * i], G.lo[0], G.inv_cell, G.dim[0]);
+        const int cy = cell_coord(p[3 * i + 1], G.lo[1], G.inv_cell, G.dim[1]);
+        const int cz = cell_coord(p[3 * i + 2], G.lo[2], G.inv_cell, G.dim[2]);
+        c = int(cx + int64_t(G.dim[0]) * (cy + int64_t(G.dim[1]) * cz));
+        cell_of[i] = c;
+    }
+    const int lane = threadIdx.x & 63;
+    const uint64_t starts = run_starts(c);
+    if (c >= 0 && ((starts >> lane) & 1ull)) atomicAdd(&counts[c], (unsigned long long)run_length(starts, lane));
 }
 
 __global__ void knn_scatter_kernel(const float* __restrict__ p, int64_t n, const int* __restrict__ cell_of,
                                    unsigned long long* __restrict__ cursor, float4* __restrict__ out) {
-    int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const unsigned long long s = atomicAdd(&cursor[cell_of[i]], 1ull);
-    out[s] = make_float4(p[3 * i], p[3 * i + 1], p[3 * i + 2], __int_as_float(int(i)));
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    const int c = i < n ? cell_of[i] : -1;
+    const int lane = threadIdx.x & 63;
+    const uint64_t starts = run_starts(c);
+    int64_t base = 0;
+    if (c >= 0 && ((starts >> lane) & 1ull))
+        base = int64_t(atomicAdd(&cursor[c], (unsigned long long)run_length(starts, lane)));
+    const int head = run_head(starts, lane);
+    const int64_t hb = (int64_t(__shfl(int(uint32_t(base >> 32)), head, 64)) << 32) |
+                       int64_t(uint32_t(__shfl(int(uint32_t(base)), head, 64)));
+    if (c >= 0) out[hb + (lane - head)] = make_float4(p[3 * i], p[3 * i + 1], p[3 * i + 2], __int_as_float(int(i)));
 }
 
 __global__ void copy_i64_kernel(const int64_t* __restrict__ a, unsigned long long* __restrict__ b, int64_t n) {
@@ -496,28 +530,6 @@ __device__ __forceinline__ int tile_of_point(const KnnGridArgs& G, const TileArg
 }
 
 // query t of the pass is qlist[t] (or t); masked queries are left out
-// Queries arrive in runs that share a tile (triangle centroids in brick
-// order): each wave adds one run with one atomic.  A run starts at lane 0
-// and wherever the tile changes; its first lane (the head) reserves the
-// run's slots, the others take theirs from the head in lane order.
-__device__ __forceinline__ uint64_t run_starts(int tl) {
-    const int lane = threadIdx.x & 63;
-    const int prev = __shfl_up(tl, 1, 64);
-    return __ballot(lane == 0 || tl != prev);
-}
-
-// lanes from `lane` up to the next run start (or the end of the wave)
-__device__ __forceinline__ int run_length(uint64_t starts, int lane) {
-    const uint64_t after = lane == 63 ? 0ull : starts >> (lane + 1);
-    return after ? __builtin_ctzll(after) + 1 : 64 - lane;
-}
-
-// the run start at or before `lane`
-__device__ __forceinline__ int run_head(uint64_t starts, int lane) {
-    const uint64_t upto = lane == 63 ? starts : starts & ((2ull << lane) - 1ull);
-    return 63 - __builtin_clzll(upto);
-}
-
 __global__ void tile_count_kernel(const float* __restrict__ q, int64_t nq, const int* __restrict__ qlist,
                                   const uint8_t* __restrict__ qmask, KnnGridArgs G, TileArgs T,
                                   int* __restrict__ tile_of, unsigned long long* __restrict__ count) {
