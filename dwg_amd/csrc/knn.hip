@@ -705,8 +705,14 @@ __global__ __launch_bounds__(kTileThreads) void knn_tile_kernel(
             exact = safe > 0.f && kd2 < safe * safe;
         }
         if (!exact) {
-            // the k candidates it has bound the k-th distance from above
-            const unsigned int slot = atomicAdd(n_redo, 1u);
+            // the wave's lanes that give up here take consecutive slots from
+            // one atomic; the k candidates a lane has bound its k-th distance
+            const uint64_t m = __ballot(1);
+            const int lane = threadIdx.x & 63, leader = __builtin_ctzll(m);
+            unsigned int base = 0;
+            if (lane == leader) base = atomicAdd(n_redo, (unsigned int)__builtin_popcountll(m));
+            base = unsigned(__shfl(int(base), leader, 64));
+            const unsigned int slot = base + unsigned(__builtin_popcountll(m & ((1ull << lane) - 1ull)));
             redo[slot] = qi;
             if (redo_kth) redo_kth[slot] = kd2;
             continue;
