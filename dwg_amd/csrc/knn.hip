@@ -1102,7 +1102,7 @@ __global__ __launch_bounds__(kPyrThreads) void knn_pyramid_kernel(KnnGridArgs G,
 // and the lanes stay in step.  Children are taken nearest-first for the
 // wave's first active lane; each lane still prunes with its own k-th key,
 // so its result is exactly its own k nearest points.
-constexpr int kPwWaves = 4;
+constexpr int kPwWaves = 2;  // small blocks free their slots sooner: the walks end at very different times
 constexpr int kPwStack = 128;  // ≥ 7 · kMaxPyr + 1 entries
 
 template <int K>
