@@ -1015,7 +1015,7 @@ __device__ __forceinline__ float pyr_box_d2(const KnnGridArgs& G, float w, float
     return __builtin_fmaf(gz, gz, __builtin_fmaf(gy, gy, gx * gx));
 }
 
-constexpr int kPyrThreads = 256;
+constexpr int kPyrThreads = 64;  // one wave per block: a block frees its slot when its slowest walk ends
 
 template <int K>
 __global__ __launch_bounds__(kPyrThreads) void knn_pyramid_kernel(KnnGridArgs G, PyrArgs Y,
