@@ -22,10 +22,12 @@ DWG_SCREENED = 1
 DWG_USE_RADII = 2
 DWG_KAHAN = 4
 DWG_FAR_BF16 = 8
+DWG_NOALLOC_MAX_SOURCES = 262144
 
 EXPORTS = [
     "dwg_ctx_create", "dwg_ctx_destroy", "dwg_ctx_error", "dwg_ctx_sync", "dwg_ctx_profile",
-    "dwg_ctx_profile_get", "dwg_ctx_profile_reset", "dwg_ctx_counter_get", "dwg_gwn_direct", "dwg_octree_build", "dwg_octree_build_device",
+    "dwg_ctx_profile_get", "dwg_ctx_profile_reset", "dwg_ctx_counter_get", "dwg_gwn_direct",
+    "dwg_gwn_direct_noalloc", "dwg_octree_build", "dwg_octree_build_device",
     "dwg_octree_leaves",
     "dwg_octree_destroy", "dwg_octree_info", "dwg_octree_update", "dwg_gwn_tree", "dwg_gwn_tree_self", "dwg_gwn_tree_grid",
     "dwg_gwn_tree_terms", "dwg_gwn_tree_grid_terms", "dwg_knn_build", "dwg_knn_destroy", "dwg_knn_query",
@@ -92,6 +94,7 @@ def load():
     L.dwg_ctx_profile_reset.argtypes = [_vp]
     L.dwg_ctx_counter_get.argtypes = [_vp, ctypes.c_char_p, P(_i64)]
     L.dwg_gwn_direct.argtypes = [_vp, _vp, _vp, _vp, _vp, _i64, _vp, _i64, _f, _u32, _vp, _vp]
+    L.dwg_gwn_direct_noalloc.argtypes = [_vp, _vp, _vp, _vp, _vp, _i64, _vp, _i64, _f, _u32, _vp, _vp]
     L.dwg_octree_build.argtypes = [_vp, _vp, _vp, _i64, _int, _int, P(_vp)]
     L.dwg_octree_build_device.argtypes = [_vp, _vp, _vp, _i64, _int, _int, P(_vp)]
     L.dwg_octree_destroy.argtypes = [_vp]

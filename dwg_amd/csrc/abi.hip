@@ -47,6 +47,8 @@ int dwg_ctx_create(int device, void* stream, dwg_ctx** out) {
         DWG_HIP_CHECK(hipGetDeviceCount(&count));
         DWG_REQUIRE(device >= 0 && device < count, "no such HIP device");
         set_device(ctx);
+        // per-device address of the allocation-free entry point's scratch array
+        dwg::noalloc_init(device);
     });
     if (rc != DWG_OK) {
         delete ctx;
@@ -139,6 +141,18 @@ int dwg_gwn_direct(dwg_ctx* ctx, const float* d_pts, const float* d_nrm, const f
         DWG_REQUIRE(nq == 0 || (d_q && d_out_w), "null query/output arrays");
         set_device(ctx);
         dwg::gwn_direct(ctx, d_pts, d_nrm, d_area, d_radii, n, d_q, nq, sigma, flags, d_out_w, d_out_grad);
+    });
+}
+
+int dwg_gwn_direct_noalloc(dwg_ctx* ctx, const float* d_pts, const float* d_nrm, const float* d_area,
+                           const float* d_radii, int64_t n, const float* d_q, int64_t nq, float sigma, uint32_t flags,
+                           float* d_out_w, float* d_out_grad) {
+    if (!ctx) return DWG_ERR_ARG;
+    return guarded(ctx, [&] {
+        DWG_REQUIRE(n == 0 || (d_pts && d_nrm), "null source arrays");
+        DWG_REQUIRE(nq == 0 || (d_q && d_out_w), "null query/output arrays");
+        set_device(ctx);
+        dwg::gwn_direct_noalloc(ctx, d_pts, d_nrm, d_area, d_radii, n, d_q, nq, sigma, flags, d_out_w, d_out_grad);
     });
 }
 

@@ -89,20 +89,21 @@ struct dwg_ctx {
 
 namespace dwg {
 
-// Brackets a kernel launch with events when profiling is on.
+// Brackets a kernel launch with events when profiling is on (a null
+// context records nothing).
 struct ProfileScope {
     dwg_ctx* ctx;
     const char* name;
     hipEvent_t a = nullptr, b = nullptr;
     ProfileScope(dwg_ctx* c, const char* n) : ctx(c), name(n) {
-        if (ctx->profile) {
+        if (ctx && ctx->profile) {
             DWG_HIP_CHECK(hipEventCreate(&a));
             DWG_HIP_CHECK(hipEventCreate(&b));
             DWG_HIP_CHECK(hipEventRecord(a, ctx->stream));
         }
     }
     ~ProfileScope() {
-        if (ctx->profile && a) {
+        if (ctx && ctx->profile && a) {
             (void)hipEventRecord(b, ctx->stream);
             ctx->times[std::string(name) + ctx->profile_suffix].pending.emplace_back(a, b);
         }

@@ -5,6 +5,9 @@
 
 namespace dwg {
 
+// sources the allocation-free direct sums can take (DWG_NOALLOC_MAX_SOURCES)
+constexpr int64_t kNoAllocSources = DWG_NOALLOC_MAX_SOURCES;
+
 }  // namespace dwg
 
 struct dwg_octree {
@@ -41,6 +44,14 @@ void octree_update(dwg_ctx* ctx, dwg_octree* T, const float* nrm, const float* a
 
 void gwn_direct(dwg_ctx* ctx, const float* pts, const float* nrm, const float* area, const float* radii, int64_t n,
                 const float* q, int64_t nq, float sigma, uint32_t flags, float* out_w, float* out_g);
+// device address of the fixed scratch array of `device` (must be current),
+// looked up once per device; called when a context is created
+void noalloc_init(int device);
+// gwn_direct with the packed sources in that array: no allocation, n ≤
+// kNoAllocSources; concurrent calls on one device share the array
+void gwn_direct_noalloc(dwg_ctx* ctx, const float* pts, const float* nrm, const float* area, const float* radii,
+                        int64_t n, const float* q, int64_t nq, float sigma, uint32_t flags, float* out_w,
+                        float* out_g);
 void gwn_tree(dwg_ctx* ctx, const dwg_octree* T, const float* q, int64_t nq, float beta, float sigma, uint32_t flags,
               float* out);
 // sums at the tree's own points (evaluated in tree order), out in caller order

@@ -17,6 +17,9 @@
 // pre-scaled by 1/(4π) when sources are packed), which is the same value.
 #include <algorithm>
 #include <cmath>
+#include <mutex>
+#include <string>
+#include <vector>
 
 #include "gwn.h"
 #include "scan.h"
@@ -177,23 +180,27 @@ __global__ __launch_bounds__(kDirectThreads) void gwn_direct_grad_kernel(
     }
 }
 
-void gwn_direct(dwg_ctx* ctx, const float* pts, const float* nrm, const float* area, const float* radii, int64_t n,
-                const float* q, int64_t nq, float sigma, uint32_t flags, float* out_w, float* out_g) {
+static void check_direct_args(int64_t n, int64_t nq, const float* radii, uint32_t flags, const float* out_g) {
     DWG_REQUIRE(n >= 0 && nq >= 0, "negative size");
     DWG_REQUIRE(!(flags & DWG_USE_RADII) || radii, "DWG_USE_RADII needs radii");
     DWG_REQUIRE(!(out_g && (flags & DWG_SCREENED)), "gradient is defined for the plain kernel only");
-    if (nq == 0) return;
-    float4* sa = static_cast<float4*>(ctx->buf("direct_sa", sizeof(float4) * (n + 1)));
-    float4* sb = static_cast<float4*>(ctx->buf("direct_sb", sizeof(float4) * (n + 1)));
+}
+
+// packs the n sources into sa/sb (room for n + 1 each) and runs the direct
+// kernel; `profile` off records no events (the allocation-free entry point)
+static void direct_sums(dwg_ctx* ctx, float4* sa, float4* sb, const float* pts, const float* nrm, const float* area,
+                        const float* radii, int64_t n, const float* q, int64_t nq, float sigma, uint32_t flags,
+                        float* out_w, float* out_g, bool profile) {
     if (n > 0) {
-        ProfileScope ps(ctx, "pack_sources");
+        ProfileScope ps(profile ? ctx : nullptr, "pack_sources");
         hipLaunchKernelGGL(pack_sources_kernel, dim3(grid_for(n, 256)), dim3(256), 0, ctx->stream, pts, nrm, area,
                            radii, nullptr, n, (flags & DWG_USE_RADII) ? 1 : 0, sa, sb, nullptr);
         check_launch();
     }
     const bool kahan = flags & DWG_KAHAN;
     dim3 grid(grid_for(nq, kDirectThreads)), block(kDirectThreads);
-    ProfileScope ps(ctx, out_g ? "gwn_direct_grad" : ((flags & DWG_SCREENED) ? "gwn_direct_screened" : "gwn_direct_plain"));
+    ProfileScope ps(profile ? ctx : nullptr,
+                    out_g ? "gwn_direct_grad" : ((flags & DWG_SCREENED) ? "gwn_direct_screened" : "gwn_direct_plain"));
     if (out_g) {
         if (kahan) hipLaunchKernelGGL(gwn_direct_grad_kernel<true>, grid, block, 0, ctx->stream, sa, sb, n, q, nq, out_w, out_g);
         else hipLaunchKernelGGL(gwn_direct_grad_kernel<false>, grid, block, 0, ctx->stream, sa, sb, n, q, nq, out_w, out_g);
@@ -205,6 +212,63 @@ void gwn_direct(dwg_ctx* ctx, const float* pts, const float* nrm, const float* a
         else hipLaunchKernelGGL((gwn_direct_kernel<false, false>), grid, block, 0, ctx->stream, sa, sb, n, q, nq, sigma, out_w);
     }
     check_launch();
+}
+
+void gwn_direct(dwg_ctx* ctx, const float* pts, const float* nrm, const float* area, const float* radii, int64_t n,
+                const float* q, int64_t nq, float sigma, uint32_t flags, float* out_w, float* out_g) {
+    check_direct_args(n, nq, radii, flags, out_g);
+    if (nq == 0) return;
+    float4* sa = static_cast<float4*>(ctx->buf("direct_sa", sizeof(float4) * (n + 1)));
+    float4* sb = static_cast<float4*>(ctx->buf("direct_sb", sizeof(float4) * (n + 1)));
+    direct_sums(ctx, sa, sb, pts, nrm, area, radii, n, q, nq, sigma, flags, out_w, out_g, true);
+}
+
+// ---------------------------------------------------------------------------
+// allocation-free direct sums: the packed sources live in a fixed device
+// array instead of the context's growable buffers
+// ---------------------------------------------------------------------------
+// sa of source i at scratch[i], sb at scratch[kNoAllocSources + 1 + i]
+// (8 MiB on every device the library runs on, held until the process ends)
+static __device__ __align__(256) float4 scratch[2 * (kNoAllocSources + 1)];
+
+namespace {
+std::mutex g_scratch_mu;
+std::vector<float4*> g_scratch_ptr;  // device address of `scratch` per device ordinal
+}  // namespace
+
+void noalloc_init(int device) {
+    // the symbol's address differs per device: look it up with that device
+    // current, once, and keep it (host code never uses the symbol itself)
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (device < 0) throw Error(DWG_ERR_ARG, "bad device ordinal");
+    if (size_t(device) < g_scratch_ptr.size() && g_scratch_ptr[size_t(device)]) return;
+    int cur = -1;
+    DWG_HIP_CHECK(hipGetDevice(&cur));
+    DWG_REQUIRE(cur == device, "noalloc_init: the device must be current");
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+        throw Error(DWG_ERR_HIP, std::string("hipGetSymbolAddress(scratch) failed: ") + hipGetErrorString(e));
+    if (g_scratch_ptr.size() <= size_t(device)) g_scratch_ptr.resize(size_t(device) + 1, nullptr);
+    g_scratch_ptr[size_t(device)] = static_cast<float4*>(p);
+}
+
+void gwn_direct_noalloc(dwg_ctx* ctx, const float* pts, const float* nrm, const float* area, const float* radii,
+                        int64_t n, const float* q, int64_t nq, float sigma, uint32_t flags, float* out_w,
+                        float* out_g) {
+    check_direct_args(n, nq, radii, flags, out_g);
+    if (n > kNoAllocSources)
+        throw Error(DWG_ERR_ARG, "dwg_gwn_direct_noalloc: " + std::to_string(n) + " sources exceed its limit of " +
+                                     std::to_string(kNoAllocSources) + " (use dwg_gwn_direct)");
+    if (nq == 0) return;
+    float4* base = nullptr;
+    {
+        std::lock_guard<std::mutex> lock(g_scratch_mu);
+        if (size_t(ctx->device) < g_scratch_ptr.size()) base = g_scratch_ptr[size_t(ctx->device)];
+    }
+    if (!base) throw Error(DWG_ERR_STATE, "allocation-free scratch of this device was not set up");
+    direct_sums(ctx, base, base + (kNoAllocSources + 1), pts, nrm, area, radii, n, q, nq, sigma, flags, out_w, out_g,
+                false);
 }
 
 }  // namespace dwg

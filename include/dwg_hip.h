@@ -75,6 +75,19 @@ int dwg_gwn_direct(dwg_ctx* ctx, const float* d_pts, const float* d_nrm, const f
                    const float* d_radii, int64_t n, const float* d_q, int64_t nq, float sigma,
                    uint32_t flags, float* d_out_w, float* d_out_grad);
 
+/* No counterpart in the reference: dwg_gwn_direct for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  Same arguments, kernels and results;
+ * the packed sources go to a fixed device array of the library (8 MiB per
+ * device, held until the process ends) instead of context buffers, so the
+ * call allocates nothing.  n ≤ DWG_NOALLOC_MAX_SOURCES, else DWG_ERR_ARG.
+ * Every call on one device shares that array: calls that overlap on a
+ * device (other streams or contexts) race. */
+#define DWG_NOALLOC_MAX_SOURCES 262144
+int dwg_gwn_direct_noalloc(dwg_ctx* ctx, const float* d_pts, const float* d_nrm, const float* d_area,
+                           const float* d_radii, int64_t n, const float* d_q, int64_t nq, float sigma,
+                           uint32_t flags, float* d_out_w, float* d_out_grad);
+
 /* Octree over sources: octreeDWG split rule (a node with more than
  * leaf_size points and depth < max_depth is split into 8 children,
  * octree.cpp:171-335; the reference uses leaf_size 1) plus the order-0
