@@ -19,8 +19,12 @@ time, the speed-up and whether learned+mixed met the equal-error gate
 
 Also reported: the roofline of the dominant kernel (algorithmic FLOPs from
 exact Barnes-Hut term counts × FLOPs per term over the kernel's HIP-event
-time on the library's stream) and a CPU baseline measured on a bounded
-sample with the FP64 CPU restatement.
+time on the library's stream); the plain method at fewer iterations, down
+to the fewest that still meets the gate, and the speed-up against that run;
+the direct all-pairs sums' FLOP rate on the small sphere configuration; a
+CPU baseline measured on a bounded sample with the FP64 CPU restatement, and
+on the same sample the error of the GPU sums (FP32, FP32 + Kahan, BF16 far
+field + Kahan) against FP64.
 """
 from __future__ import annotations
 
@@ -116,21 +120,21 @@ def large_config(args):
     plain_t, plain_r = timed(ctx, P, plain, 1, "plain")
     learned_t, learned_r = timed(ctx, P, learned, 1, "learned+mixed", mlp=net)
     q_plain, q_learned = quality(ctx, plain_r, Ntrue, S), quality(ctx, learned_r, Ntrue, S)
-    gate = bool(q_learned["mean_angle_deg"] <= q_plain["mean_angle_deg"] + 0.5
-                and q_learned["chamfer"] <= 1.01 * q_plain["chamfer"])
+    passed = gate(q_learned, q_plain)
     print(json.dumps({"config": "large: 5M points, 128 primitives + 16 shells, 5% holes, 1024^3 grids",
                       "n_points": int(len(P)), "n_sources": int(learned_r.n_sources),
                       "learned_mixed_s": round(learned_t[0], 3), "plain_fp32_s": round(plain_t[0], 3),
-                      "speedup_vs_plain": round(plain_t[0] / learned_t[0], 3), "equal_error": gate,
+                      "speedup_vs_plain": round(plain_t[0] / learned_t[0], 3), "equal_error": passed,
                       "learned_iterations": args.learned_iters, "quality": q_learned, "plain_quality": q_plain}),
           flush=True)
 
 
-def direct_sums(reps=5):
+def direct_sums(ctx=None, reps=5):
     """FLOP rate of the direct all-pairs sums (plain and screened) for the
     small sphere configuration: 10,000 area-uniform sphere points (seed 1),
     queries at the 128^3 points of a grid over the bounding box padded by
-    10%.  One JSON line per kernel."""
+    10%.  Per kernel: mean HIP-event time on the library's stream over
+    `reps` launches after one untimed launch."""
     import torch
     from dwg_amd import gpu, shapes
     P, N = shapes.sphere(10_000, 1)
@@ -139,22 +143,28 @@ def direct_sums(reps=5):
     pad = 0.1 * (hi - lo)
     axes = [np.linspace(lo[d] - pad[d], hi[d] + pad[d], 128, dtype=np.float32) for d in range(3)]
     Q = np.stack(np.meshgrid(*axes, indexing="ij"), -1).reshape(-1, 3)
-    ctx = gpu.Context(0)
+    ctx = ctx or gpu.Context(0)
     Pt, Nt, At, Qt = (torch.from_numpy(np.ascontiguousarray(a)).cuda() for a in (P, N, A, Q))
+    out = []
     for screened, flops in ((False, FLOPS_PER_TERM["plain"]), (True, FLOPS_PER_TERM["screened"])):
+        name = "gwn_direct_screened" if screened else "gwn_direct_plain"
         ctx.gwn_direct(Pt, Nt, At, Qt, screened=screened, kahan=False)
         torch.cuda.synchronize()
-        t0 = time.perf_counter()
+        ctx.profile_reset()
+        ctx.profile(True)
         for _ in range(reps):
             ctx.gwn_direct(Pt, Nt, At, Qt, screened=screened, kahan=False)
-        torch.cuda.synchronize()
-        ms = (time.perf_counter() - t0) / reps * 1e3
+        ctx.profile(False)
+        tot, cnt = ctx.profile_get(name)
+        ms = tot / cnt
         pairs = len(P) * len(Q)
         tf = pairs * flops / (ms * 1e-3) / 1e12
-        print(json.dumps({"kernel": "gwn_direct_kernel", "screened": screened, "pairs": pairs, "ms": round(ms, 3),
-                          "tpairs_per_s": round(pairs / (ms * 1e-3) / 1e12, 3), "tflops": round(tf, 2),
-                          "frac_of_157.3": round(tf / PEAK_FP32, 4), "frac_of_78.6_non_packed": round(tf / 78.6, 4)}),
-              flush=True)
+        out.append({"kernel": f"gwn_direct_kernel<{'screened' if screened else 'plain'}>",
+                    "workload": "10,000 sphere sources x 128^3 grid queries", "pairs": pairs,
+                    "flops_per_pair": flops, "avg_launch_ms": round(ms, 3),
+                    "tpairs_per_s": round(pairs / (ms * 1e-3) / 1e12, 3), "tflops": round(tf, 2),
+                    "frac_of_157.3": round(tf / PEAK_FP32, 4), "frac_of_78.6_non_packed": round(tf / 78.6, 4)})
+    return out
 
 
 def measured_traffic():
@@ -169,20 +179,21 @@ def measured_traffic():
     return None
 
 
-def cpu_baseline(P, grid_reso, iters, budget_s=20.0):
+def cpu_baseline(P, grid_reso, iters, normals=None, budget_s=20.0):
     """FP64 CPU restatement (the oracle) on a bounded sample of the same work:
     octree + moments over all sources, then screened Barnes-Hut sums at a
     random subset of the source and grid queries; the per-query rates are
-    scaled to the full reconstruction (iters × (sources + grid) queries)."""
+    scaled to the full reconstruction (iters × (sources + grid) queries).
+    Also returns the sample (points, normals, areas, queries and their FP64
+    sums) so that the GPU sums can be checked against it."""
     import oracle as O
-    from dwg_amd.pipeline import options  # noqa: F401  (same defaults)
     cores = O.threads()
     c = P.astype(np.float64).mean(0)
     Pn = P.astype(np.float64) - c
     Pn /= 2 * np.abs(Pn).max()
+    N0 = O.gauss_map(Pn) if normals is None else np.asarray(normals, np.float64)
     t0 = time.perf_counter()
     tree = O.Octree(Pn, 10, 1)
-    N0 = O.gauss_map(Pn)
     A = np.full(len(Pn), 2e-6)
     tree.precompute(N0, A)
     t_build = time.perf_counter() - t0
@@ -197,20 +208,74 @@ def cpu_baseline(P, grid_reso, iters, budget_s=20.0):
         gq = np.stack([origin[0] + (gi % res[0]) * h, origin[1] + ((gi // res[0]) % res[1]) * h,
                        origin[2] + (gi // (res[0] * res[1])) * h], -1)
         t1 = time.perf_counter()
-        tree.eval(qs, 2.3, kind="screened", sigma=10.0)
+        w_src = tree.eval(qs, 2.3, kind="screened", sigma=10.0)
         t_src = time.perf_counter() - t1
         t1 = time.perf_counter()
-        tree.eval(gq, 2.3, kind="screened", sigma=10.0)
+        w_grid = tree.eval(gq, 2.3, kind="screened", sigma=10.0)
         t_grid = time.perf_counter() - t1
         if t_src + t_grid > budget_s / 4 or m >= 2_000_000:
             break
         m *= 4
     per_iter = t_src / m * len(Pn) + t_grid / m * G
     total = t_build + (iters + 1) * per_iter
-    return {"value": round(total, 1), "unit": "s", "cores": cores, "kind": "port",
+    base = {"value": round(total, 1), "unit": "s", "cores": cores, "kind": "port",
             "sample": f"FP64 oracle: octree+moments over {len(Pn)} points ({t_build:.1f} s) and screened "
                       f"Barnes-Hut sums at {m} source + {m} grid queries, scaled to {iters + 1} evaluations of "
                       f"{len(Pn)} sources + {G} grid points (MC/kNN/diffusion not counted)"}
+    sample = {"P": Pn, "N": N0, "A": A, "Q": np.concatenate([qs, gq]), "w": np.concatenate([w_src, w_grid])}
+    return base, sample
+
+
+def precision_check(ctx, sample, n_direct=256):
+    """Error of the GPU sums against the FP64 oracle on the CPU-baseline
+    sample: the same octree (the 1M points as sources, the plain run's final
+    normals, areas 2e-6), β = 2.3, σ = 10, at its source and grid queries,
+    for FP32, FP32 + Kahan and BF16 far field + Kahan (the learned+mixed
+    iterations' mode); and against the FP64 all-pairs sum at n_direct of them
+    (which adds the Barnes-Hut truncation).  Errors relative to max|w|."""
+    import oracle as O
+    from dwg_amd import gpu
+    P, N, A, Q, ref = sample["P"], sample["N"], sample["A"], sample["Q"], sample["w"]
+    t = gpu.Octree(ctx, P, A, max_depth=10)
+    t.update(N.astype(np.float32), A.astype(np.float32))
+    Qf = Q.astype(np.float32)
+    scale = float(np.abs(ref).max())
+    sub = np.random.default_rng(1).choice(len(Q), n_direct, replace=False)
+    ref_d = O.direct(P, N, A, Qf[sub].astype(np.float64), kind="screened", sigma=10.0)
+    out = {"reference": f"FP64 oracle tree (same beta, same term set) at {len(Q)} queries; FP64 direct sum at "
+                        f"{n_direct} of them", "max_abs_w": round(scale, 6)}
+    for name, kw in (("fp32", {}), ("fp32_kahan", {"kahan": True}), ("bf16_far_kahan", {"kahan": True, "bf16": True})):
+        w = t.eval(Qf, 2.3, sigma=10.0, **kw).cpu().numpy().astype(np.float64)
+        d = np.abs(w - ref) / scale
+        dd = np.abs(w[sub] - ref_d) / scale
+        out[name] = {"vs_fp64_tree_max": float(f"{d.max():.3e}"), "vs_fp64_tree_mean": float(f"{d.mean():.3e}"),
+                     "vs_fp64_direct_mean": float(f"{dd.mean():.3e}")}
+    out["fp64_tree_vs_fp64_direct_mean"] = float(f"{(np.abs(ref[sub] - ref_d) / scale).mean():.3e}")
+    return out
+
+
+def gate(q, q_plain):
+    return bool(q["mean_angle_deg"] <= q_plain["mean_angle_deg"] + 0.5 and q["chamfer"] <= 1.01 * q_plain["chamfer"])
+
+
+def plain_gate_scan(ctx, P, args, Ntrue, S, q_plain):
+    """Plain FP32 at fewer iterations (even counts only: the normals' sign
+    alternates each iteration), from iters − 2 down until one misses the
+    equal-error gate against the full plain run.  Returns the fewest count
+    that passed (the full count if none), its one-step wall time, and the
+    scan."""
+    scan = []
+    best = (args.iters, None)
+    for k in range(args.iters - 2, 1, -2):
+        t, r = timed(ctx, P, dict(plain_options(args), total_iter=k), 1, f"plain {k} iterations")
+        q = quality(ctx, r, Ntrue, S)
+        ok = gate(q, q_plain)
+        scan.append({"iterations": k, "seconds": round(t[0], 3), "mean_angle_deg": q["mean_angle_deg"],
+                     "chamfer": q["chamfer"], "gate": ok})
+        if not ok:
+            break
+        best = (k, t[0])
+    return best, scan
 
 
 def quality(ctx, r, Ntrue, S):
@@ -249,6 +314,8 @@ def main():
     ap.add_argument("--seed", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-quality", action="store_true")
+    ap.add_argument("--no-gate-scan", action="store_true",
+                    help="skip the plain runs at fewer iterations that find the fewest meeting the gate")
     ap.add_argument("--large", action="store_true",
                     help="only run the large configuration: 5M points (128 primitives + 16 thin shells, 5%% in "
                          "holes), 1024^3 grids, learned+mixed vs plain FP32; prints its own JSON line")
@@ -269,7 +336,8 @@ def main():
         print(json.dumps({"copy_bytes_read": a.numel() * 4, "copy_bytes_written": b.numel() * 4}))
         return
     if args.direct_sums:
-        direct_sums()
+        for line in direct_sums():
+            print(json.dumps(line), flush=True)
         return
     if args.large:
         large_config(args)
@@ -314,18 +382,27 @@ def main():
     avg_ms = g_ms / g_cnt
     achieved = prof.terms_grid * FLOPS_PER_TERM["screened"] / (avg_ms * 1e-3) / 1e12
     roofline = {"kernel": "gwn_tree_grid_screened (gwn_tree_kernel<screened, BF16 far field, grid>)",
-                "bound": "mfma", "pipe": "fp32 VALU",
+                "bound": "valu", "pipe": "fp32 VALU (instruction issue)",
                 "achieved": round(achieved, 2), "peak": PEAK_FP32, "unit": "TFLOP/s",
                 "frac": round(achieved / PEAK_FP32, 4), "traffic": measured_traffic(),
                 "terms_per_launch": int(prof.terms_grid), "flops_per_term": FLOPS_PER_TERM["screened"],
                 "avg_launch_ms": round(avg_ms, 3), "launches": int(g_cnt), "dominant_by_time": dom}
-    q_plain = q_learned = gate = None
+    q_plain = q_learned = passed = None
+    at_gate = {}
     if not args.no_quality:
         S = surface_sample(2_000_000, args.seed)
         q_plain, q_learned = quality(ctx, plain_r, Ntrue, S), quality(ctx, learned_r, Ntrue, S)
-        gate = bool(q_learned["mean_angle_deg"] <= q_plain["mean_angle_deg"] + 0.5
-                    and q_learned["chamfer"] <= 1.01 * q_plain["chamfer"])
-    cpu = None if args.no_cpu else cpu_baseline(P, args.grid, args.iters)
+        passed = gate(q_learned, q_plain)
+        if not args.no_gate_scan:
+            (k, tk), scan = plain_gate_scan(ctx, P, args, Ntrue, S, q_plain)
+            tk = plain_s if tk is None else tk
+            at_gate = {"plain_iters_to_gate": k, "plain_at_gate_s": round(tk, 3),
+                       "speedup_vs_plain_at_gate": round(tk / value, 3), "plain_gate_scan": scan}
+    direct = direct_sums(ctx)
+    cpu = precision = None
+    if not args.no_cpu:
+        cpu, sample = cpu_baseline(P, args.grid, args.iters, normals=plain_r.normals)
+        precision = precision_check(ctx, sample)
     out = {"metric": METRIC, "value": round(value, 3), "unit": "s", "n_gpus": 1, "steps": args.steps,
            "warmup": args.warmup, "ms_per_step": round(value * 1000, 1), "higher_is_better": False,
            "scaling": "strong", "vs_baseline": None, "dtype": "fp32+bf16", "data": "synthetic",
@@ -341,8 +418,8 @@ def main():
                                      "phases_s": {"setup": round(plain_r.time_setup_s, 3),
                                                   "iterations": round(plain_r.time_iter_s, 3),
                                                   "final": round(plain_r.time_final_s, 3)}},
-                      "speedup_vs_plain": round(plain_s / value, 3), "equal_error": gate},
-           "roofline": roofline, "cpu_baseline": cpu}
+                      "speedup_vs_plain": round(plain_s / value, 3), "equal_error": passed, **at_gate},
+           "roofline": roofline, "direct_sums": direct, "precision": precision, "cpu_baseline": cpu}
     print(json.dumps(out), flush=True)
 
 

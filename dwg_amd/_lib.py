@@ -22,17 +22,18 @@ DWG_SCREENED = 1
 DWG_USE_RADII = 2
 DWG_KAHAN = 4
 DWG_FAR_BF16 = 8
+DWG_NEAR_FP64 = 16
 DWG_NOALLOC_MAX_SOURCES = 262144
 
 EXPORTS = [
     "dwg_ctx_create", "dwg_ctx_destroy", "dwg_ctx_error", "dwg_ctx_sync", "dwg_ctx_profile",
     "dwg_ctx_profile_get", "dwg_ctx_profile_reset", "dwg_ctx_counter_get", "dwg_gwn_direct",
     "dwg_gwn_direct_noalloc", "dwg_octree_build", "dwg_octree_build_device",
-    "dwg_octree_leaves",
-    "dwg_octree_destroy", "dwg_octree_info", "dwg_octree_update", "dwg_gwn_tree", "dwg_gwn_tree_self", "dwg_gwn_tree_grid",
+    "dwg_octree_leaves", "dwg_octree_leaf_normals",
+    "dwg_octree_destroy", "dwg_octree_info", "dwg_octree_update", "dwg_octree_set_point_areas", "dwg_gwn_tree", "dwg_gwn_tree_grad", "dwg_gwn_tree_self", "dwg_gwn_tree_grid",
     "dwg_gwn_tree_terms", "dwg_gwn_tree_grid_terms", "dwg_knn_build", "dwg_knn_destroy", "dwg_knn_query",
     "dwg_marching_cubes", "dwg_diffuse_normals", "dwg_radii", "dwg_point_areas", "dwg_pca_normals", "dwg_gauss_map", "dwg_transfer_normals",
-    "dwg_outlier_filter", "dwg_reduce_sum", "dwg_bounds", "dwg_options_default", "dwg_launch",
+    "dwg_outlier_filter", "dwg_normal_change", "dwg_reduce_sum", "dwg_bounds", "dwg_options_default", "dwg_launch",
     "dwg_free", "dwg_mlp_num_params", "dwg_mlp_create", "dwg_mlp_load", "dwg_mlp_save", "dwg_mlp_destroy",
     "dwg_mlp_get", "dwg_mlp_set", "dwg_mlp_patches", "dwg_mlp_patches_from", "dwg_mlp_init", "dwg_mlp_forward", "dwg_mlp_loss_grad", "dwg_mlp_adam",
     "dwg_mlp_augment", "dwg_gemm_bf16",
@@ -53,7 +54,7 @@ class DwgOptions(ctypes.Structure):
                 ("source_mode", ctypes.c_int32), ("leaf_size", ctypes.c_int32), ("beta", ctypes.c_float),
                 ("precision", ctypes.c_uint32), ("seed", ctypes.c_uint64), ("normalize_input", ctypes.c_int32),
                 ("verbose", ctypes.c_int32), ("count_terms", ctypes.c_int32), ("mlp", ctypes.c_void_p),
-                ("sigma_warm_iters", ctypes.c_int32)]
+                ("sigma_warm_iters", ctypes.c_int32), ("residual_tol", ctypes.c_float)]
 
 
 class DwgResult(ctypes.Structure):
@@ -62,7 +63,8 @@ class DwgResult(ctypes.Structure):
                 ("scale", ctypes.c_double), ("time_setup_s", ctypes.c_double), ("time_iter_s", ctypes.c_double),
                 ("time_final_s", ctypes.c_double), ("time_total_s", ctypes.c_double), ("w_mean", ctypes.c_void_p),
                 ("iter_tris", ctypes.c_void_p), ("iter_zeros", ctypes.c_void_p), ("terms_src", ctypes.c_int64),
-                ("terms_grid", ctypes.c_int64), ("grid_points", ctypes.c_int64)]
+                ("terms_grid", ctypes.c_int64), ("grid_points", ctypes.c_int64), ("source_normals", ctypes.c_void_p),
+                ("residual", ctypes.c_void_p), ("iterations", ctypes.c_int32)]
 
 
 _vp = ctypes.c_void_p
@@ -99,10 +101,13 @@ def load():
     L.dwg_octree_build_device.argtypes = [_vp, _vp, _vp, _i64, _int, _int, P(_vp)]
     L.dwg_octree_destroy.argtypes = [_vp]
     L.dwg_octree_leaves.argtypes = [_vp, _vp, _int, _vp, _vp, P(_i64)]
+    L.dwg_octree_leaf_normals.argtypes = [_vp, _vp, _int, _vp, _vp, _vp, P(_i64)]
     L.dwg_octree_destroy.restype = None
     L.dwg_octree_info.argtypes = [_vp, P(_i64), P(_i64), P(_int)]
     L.dwg_octree_update.argtypes = [_vp, _vp, _vp, _vp, _vp]
+    L.dwg_octree_set_point_areas.argtypes = [_vp, _vp, _vp, _vp]
     L.dwg_gwn_tree.argtypes = [_vp, _vp, _vp, _i64, _f, _f, _u32, _vp]
+    L.dwg_gwn_tree_grad.argtypes = [_vp, _vp, _vp, _i64, _f, _f, _u32, _vp, _vp]
     L.dwg_gwn_tree_self.argtypes = [_vp, _vp, _f, _f, _u32, _vp]
     L.dwg_gwn_tree_grid.argtypes = [_vp, _vp, P(DwgGrid), _f, _f, _u32, _vp]
     L.dwg_gwn_tree_terms.argtypes = [_vp, _vp, _vp, _i64, _f, _vp]
@@ -119,6 +124,7 @@ def load():
     L.dwg_gauss_map.argtypes = [_vp, _vp, _i64, P(_f), _vp]
     L.dwg_transfer_normals.argtypes = [_vp, _vp, _vp, _vp, _i64, _int, _vp]
     L.dwg_outlier_filter.argtypes = [_vp, _vp, _i64, _f]
+    L.dwg_normal_change.argtypes = [_vp, _vp, _vp, _i64, P(ctypes.c_double)]
     L.dwg_reduce_sum.argtypes = [_vp, _vp, _i64, P(ctypes.c_double)]
     L.dwg_bounds.argtypes = [_vp, _vp, _i64, P(_f), P(_f)]
     L.dwg_options_default.argtypes = [P(DwgOptions)]

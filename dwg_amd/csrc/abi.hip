@@ -196,6 +196,23 @@ int dwg_octree_leaves(dwg_ctx* ctx, const dwg_octree* tree, int min_num, double*
     });
 }
 
+int dwg_octree_leaf_normals(dwg_ctx* ctx, const dwg_octree* tree, int min_num, const double* d_pts,
+                            const double* d_nrm, float* d_out, int64_t* out_count) {
+    if (!ctx || !tree || !out_count) return DWG_ERR_ARG;
+    return guarded(ctx, [&] {
+        DWG_REQUIRE(d_pts && d_nrm && d_out, "null input or output");
+        DWG_REQUIRE(min_num >= 0, "min_num must be ≥ 0");
+        set_device(ctx);
+        const int64_t nl_max = tree->n_leaves;
+        auto* cen = static_cast<double*>(ctx->buf("leaf_normals_centres", sizeof(double) * 3 * size_t(nl_max + 1)));
+        auto* wid = static_cast<double*>(ctx->buf("leaf_normals_widths", sizeof(double) * size_t(nl_max + 1)));
+        auto* nodes = static_cast<int*>(ctx->buf("leaf_normals_nodes", sizeof(int) * size_t(nl_max + 1)));
+        const int64_t nl = dwg::octree_leaves(ctx, tree, min_num, cen, wid, nodes);
+        dwg::leaf_smooth_normals(ctx, tree, nodes, cen, wid, nl, d_pts, d_nrm, d_out);
+        *out_count = nl;
+    });
+}
+
 int dwg_octree_info(const dwg_octree* tree, int64_t* n_nodes, int64_t* n_leaves, int* max_level) {
     if (!tree) return DWG_ERR_ARG;
     if (n_nodes) *n_nodes = tree->n_nodes;
@@ -213,6 +230,15 @@ int dwg_octree_update(dwg_ctx* ctx, dwg_octree* tree, const float* d_nrm, const 
     });
 }
 
+int dwg_octree_set_point_areas(dwg_ctx* ctx, dwg_octree* tree, const float* d_nrm, const float* d_area) {
+    if (!ctx || !tree) return DWG_ERR_ARG;
+    return guarded(ctx, [&] {
+        DWG_REQUIRE(d_nrm, "null normals");
+        set_device(ctx);
+        dwg::octree_set_point_areas(ctx, tree, d_nrm, d_area);
+    });
+}
+
 int dwg_gwn_tree(dwg_ctx* ctx, const dwg_octree* tree, const float* d_q, int64_t nq, float beta, float sigma,
                  uint32_t flags, float* d_out) {
     if (!ctx || !tree) return DWG_ERR_ARG;
@@ -220,6 +246,16 @@ int dwg_gwn_tree(dwg_ctx* ctx, const dwg_octree* tree, const float* d_q, int64_t
         DWG_REQUIRE(nq == 0 || d_out, "null output");
         set_device(ctx);
         dwg::gwn_tree(ctx, tree, d_q, nq, beta, sigma, flags, d_out);
+    });
+}
+
+int dwg_gwn_tree_grad(dwg_ctx* ctx, const dwg_octree* tree, const float* d_q, int64_t nq, float beta, float sigma,
+                      uint32_t flags, float* d_out_w, float* d_out_grad) {
+    if (!ctx || !tree) return DWG_ERR_ARG;
+    return guarded(ctx, [&] {
+        DWG_REQUIRE(nq == 0 || (d_q && d_out_w), "null query or output array");
+        set_device(ctx);
+        dwg::gwn_tree_grad(ctx, tree, d_q, nq, beta, sigma, flags, d_out_w, d_out_grad);
     });
 }
 
@@ -369,6 +405,15 @@ int dwg_outlier_filter(dwg_ctx* ctx, float* d_area, int64_t n, float threshold) 
     return guarded(ctx, [&] {
         set_device(ctx);
         dwg::outlier_filter(ctx, d_area, n, threshold);
+    });
+}
+
+int dwg_normal_change(dwg_ctx* ctx, const float* d_old, const float* d_new, int64_t n, double* h_residual) {
+    if (!ctx || !h_residual) return DWG_ERR_ARG;
+    return guarded(ctx, [&] {
+        DWG_REQUIRE(n == 0 || (d_old && d_new), "null normals");
+        set_device(ctx);
+        *h_residual = dwg::normal_change(ctx, d_old, d_new, n);
     });
 }
 

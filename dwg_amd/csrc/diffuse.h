@@ -14,4 +14,6 @@ void point_areas(dwg_ctx* ctx, const dwg_knn_index* I, const float* p, int64_t n
 void transfer_normals(dwg_ctx* ctx, const dwg_knn_index* I, const float* src_nrm, const float* q, int64_t nq, int k,
                       float* out);
 void outlier_filter(dwg_ctx* ctx, float* area, int64_t n, float threshold);
+// mean over i of 1 − |cos ∠(old_i, new_i)|, FP64 on the GPU (n×3 normals)
+double normal_change(dwg_ctx* ctx, const float* old_nrm, const float* new_nrm, int64_t n);
 }  // namespace dwg

@@ -497,20 +497,124 @@ void transfer_normals(dwg_ctx* ctx, const dwg_knn_index* I, const float* src_nrm
     check_launch();
 }
 
+// Convergence residual of one diffusion step, reduced in FP64: the mean over
+// the sources of 1 − |cos ∠(n_old, n_new)|.  The absolute value makes it
+// blind to the global sign flip every iteration makes (the face normals
+// point towards increasing winding numbers); a source whose old or new
+// normal is zero counts 1.
+__global__ __launch_bounds__(256) void normal_change_kernel(const float* __restrict__ a, const float* __restrict__ b,
+                                                            int64_t n, double* __restrict__ partial) {
+    __shared__ double lds[256];
+    double s = 0;
+    for (int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x; i < n; i += int64_t(gridDim.x) * 256) {
+        const double ax = a[3 * i], ay = a[3 * i + 1], az = a[3 * i + 2];
+        const double bx = b[3 * i], by = b[3 * i + 1], bz = b[3 * i + 2];
+        const double la = ax * ax + ay * ay + az * az, lb = bx * bx + by * by + bz * bz;
+        s += (la > 0 && lb > 0) ? 1.0 - fabs(ax * bx + ay * by + az * bz) / sqrt(la * lb) : 1.0;
+    }
+    lds[threadIdx.x] = s;
+    __syncthreads();
+    for (int off = 128; off > 0; off >>= 1) {
+        if (threadIdx.x < off) lds[threadIdx.x] += lds[threadIdx.x + off];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partial[blockIdx.x] = lds[0];
+}
+
+__global__ void sum_partials_kernel(const double* __restrict__ partial, int m, double* __restrict__ out) {
+    __shared__ double lds[256];
+    double s = 0;
+    for (int i = threadIdx.x; i < m; i += 256) s += partial[i];
+    lds[threadIdx.x] = s;
+    __syncthreads();
+    for (int off = 128; off > 0; off >>= 1) {
+        if (threadIdx.x < off) lds[threadIdx.x] += lds[threadIdx.x + off];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *out = lds[0];
+}
+
+double normal_change(dwg_ctx* ctx, const float* old_nrm, const float* new_nrm, int64_t n) {
+    if (n <= 0) return 0.0;
+    constexpr int blocks = 512;
+    auto* partial = static_cast<double*>(ctx->buf("normal_change", sizeof(double) * (blocks + 1)));
+    {
+        ProfileScope ps(ctx, "normal_change");
+        hipLaunchKernelGGL(normal_change_kernel, dim3(blocks), dim3(256), 0, ctx->stream, old_nrm, new_nrm, n, partial);
+        check_launch();
+        hipLaunchKernelGGL(sum_partials_kernel, dim3(1), dim3(256), 0, ctx->stream, partial, blocks, partial + blocks);
+        check_launch();
+    }
+    return read_scalar<double>(ctx, partial + blocks) / double(n);
+}
+
 // OutlierFilter (TurboDWG.cu:991-1009): the ceil(n·thr) largest areas are
 // divided by 1000.  Ties are ordered by index (stable), on the host.
+// OutlierFilter (TurboDWG.cu:991-1009): the reference sorts the areas in
+// descending order (a stable radix sort, so equal areas keep ascending index
+// order) and divides the first ceil(n·threshold) by 1000.  Here a radix
+// select finds the m-th largest key in four 8-bit passes; every area above
+// it, and the first of those equal to it in index order, are divided.
+__device__ __forceinline__ uint32_t float_key(float f) {  // order-preserving
+    const uint32_t u = __float_as_uint(f);
+    return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+
+__global__ __launch_bounds__(256) void select_hist_kernel(const float* __restrict__ v, int64_t n, uint32_t prefix,
+                                                          uint32_t mask, int shift, unsigned int* __restrict__ hist) {
+    __shared__ unsigned int h[256];
+    h[threadIdx.x] = 0;
+    __syncthreads();
+    for (int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x; i < n; i += int64_t(gridDim.x) * 256) {
+        const uint32_t k = float_key(v[i]);
+        if ((k & mask) == prefix) atomicAdd(&h[(k >> shift) & 0xFFu], 1u);
+    }
+    __syncthreads();
+    if (h[threadIdx.x]) atomicAdd(&hist[threadIdx.x], h[threadIdx.x]);
+}
+
+__global__ void equal_flags_kernel(const float* __restrict__ v, int64_t n, uint32_t key, int64_t* __restrict__ eq) {
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) eq[i] = float_key(v[i]) == key ? 1 : 0;
+}
+
+__global__ void outlier_divide_kernel(float* __restrict__ v, int64_t n, uint32_t key, const int64_t* __restrict__ rank,
+                                      int64_t take_equal) {
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t k = float_key(v[i]);
+    if (k > key || (k == key && rank[i] < take_equal)) v[i] /= 1000.0f;
+}
+
 void outlier_filter(dwg_ctx* ctx, float* area, int64_t n, float threshold) {
-    const int64_t m = int64_t(std::ceil(float(n) * threshold));
+    const int64_t m = int64_t(std::ceil(float(n) * threshold));  // the reference's float product
     if (m <= 0 || n == 0) return;
-    std::vector<float> h(n);
-    DWG_HIP_CHECK(hipMemcpyAsync(h.data(), area, sizeof(float) * n, hipMemcpyDeviceToHost, ctx->stream));
-    DWG_HIP_CHECK(hipStreamSynchronize(ctx->stream));
-    std::vector<int64_t> idx(n);
-    std::iota(idx.begin(), idx.end(), 0);
-    std::stable_sort(idx.begin(), idx.end(), [&](int64_t a, int64_t b) { return h[a] > h[b]; });
-    for (int64_t j = 0; j < std::min(m, n); ++j) h[idx[j]] /= 1000.0f;
-    DWG_HIP_CHECK(hipMemcpyAsync(area, h.data(), sizeof(float) * n, hipMemcpyHostToDevice, ctx->stream));
-    DWG_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    const int64_t take = std::min(m, n);
+    auto* hist = static_cast<unsigned int*>(ctx->buf("outlier_hist", 256 * sizeof(unsigned int)));
+    const unsigned blocks = unsigned(std::min<int64_t>(grid_for(n, 256), 1024));
+    uint32_t prefix = 0, mask = 0;
+    int64_t remaining = take;  // rank of the wanted key among those matching the prefix
+    for (int shift = 24; shift >= 0; shift -= 8) {
+        fill_u32(ctx, hist, 0u, 256);
+        hipLaunchKernelGGL(select_hist_kernel, dim3(blocks), dim3(256), 0, ctx->stream, area, n, prefix, mask, shift,
+                           hist);
+        check_launch();
+        unsigned int h[256];
+        DWG_HIP_CHECK(hipMemcpyAsync(h, hist, sizeof(h), hipMemcpyDeviceToHost, ctx->stream));
+        DWG_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+        int d = 255;
+        for (; d > 0 && int64_t(h[d]) < remaining; --d) remaining -= h[d];
+        prefix |= uint32_t(d) << shift;
+        mask |= 0xFFu << shift;
+    }
+    // `remaining` of the areas equal to the selected key are taken, lowest index first
+    auto* rank = static_cast<int64_t*>(ctx->buf("outlier_rank", sizeof(int64_t) * size_t(n + 1)));
+    hipLaunchKernelGGL(equal_flags_kernel, dim3(grid_for(n, 256)), dim3(256), 0, ctx->stream, area, n, prefix, rank);
+    check_launch();
+    exclusive_scan_i64(ctx, rank, n);
+    hipLaunchKernelGGL(outlier_divide_kernel, dim3(grid_for(n, 256)), dim3(256), 0, ctx->stream, area, n, prefix, rank,
+                       remaining);
+    check_launch();
 }
 
 }  // namespace dwg

@@ -41,6 +41,9 @@ struct dwg_octree {
 namespace dwg {
 
 void octree_update(dwg_ctx* ctx, dwg_octree* T, const float* nrm, const float* area, const float* radii);
+// re-packs the point terms (A·n, A) with other areas, keeping the far-field
+// moments of the last octree_update
+void octree_set_point_areas(dwg_ctx* ctx, dwg_octree* T, const float* nrm, const float* area);
 
 void gwn_direct(dwg_ctx* ctx, const float* pts, const float* nrm, const float* area, const float* radii, int64_t n,
                 const float* q, int64_t nq, float sigma, uint32_t flags, float* out_w, float* out_g);
@@ -58,6 +61,9 @@ void gwn_tree(dwg_ctx* ctx, const dwg_octree* T, const float* q, int64_t nq, flo
 void gwn_tree_self(dwg_ctx* ctx, const dwg_octree* T, float beta, float sigma, uint32_t flags, float* out);
 void gwn_tree_grid(dwg_ctx* ctx, const dwg_octree* T, const dwg_grid* g, float beta, float sigma, uint32_t flags,
                    float* out);
+// value and gradient of the Barnes-Hut sums at query points (FP32 walk)
+void gwn_tree_grad(dwg_ctx* ctx, const dwg_octree* T, const float* q, int64_t nq, float beta, float sigma,
+                   uint32_t flags, float* out_w, float* out_g);
 void gwn_tree_terms(dwg_ctx* ctx, const dwg_octree* T, const float* q, int64_t nq, float beta, int64_t* terms);
 int64_t gwn_tree_grid_terms(dwg_ctx* ctx, const dwg_octree* T, const dwg_grid* g, float beta);
 

@@ -49,24 +49,29 @@ __device__ __forceinline__ float gwn_term_r2(float dx, float dy, float dz, float
     }
 }
 
-// the same term in two factors, t = x · inv, with ok = (r > 1e-10) for the
-// screened form: a sum can then take it as fma(ok ? x : 0, inv, sum), one
-// rounding and one instruction fewer than t = x · inv; sum += ok ? t : 0
+// max of two non-negative floats on their bit patterns (the integer order is
+// the float order there): one instruction, no NaN quieting of the operands
+__device__ __forceinline__ float max_nonneg(float a, float b) {
+    return __uint_as_float(max(__float_as_uint(a), __float_as_uint(b)));
+}
+
+// The tree walk's point and dipole terms in two factors, t = x · inv, from
+// d, r² = |d|² and s²: ρ² = max(r², s²), ρ = √ρ², ρ³ = ρ²·ρ; the screened
+// guard r > 1e-10 as r² > 1e-20
 template <bool SCREENED>
-__device__ __forceinline__ float gwn_term_split(float dx, float dy, float dz, float r2, float s, const float4 b,
-                                                float sigma, float sigma_l2e, float& inv, bool& ok) {
+__device__ __forceinline__ float tree_term_split(float dx, float dy, float dz, float r2, float s2, const float4 b,
+                                                 float sigma, float sigma_l2e, float& inv, bool& ok) {
     const float dot = __builtin_fmaf(dz, b.z, __builtin_fmaf(dy, b.y, dx * b.x));
-    const float r = fast_sqrt(r2);
     if constexpr (!SCREENED) {
+        const float r = fast_sqrt(r2);
         inv = fast_rcp(__builtin_fmaf(r2, r, kEpsPlain * kInv4Pi));
         ok = true;
         return dot;
     } else {
-        const float rho = fmaxf(r, s);
-        const float e = fast_exp2(sigma_l2e * rho);
-        const float rho3 = rho * rho * rho;
-        inv = fast_rcp(__builtin_fmaf(e, rho3, kEpsScreened * kInv4Pi));
-        ok = r > 1e-10f;
+        const float rho2 = max_nonneg(r2, s2);
+        const float rho = fast_sqrt(rho2);
+        inv = fast_rcp(__builtin_fmaf(fast_exp2(sigma_l2e * rho), rho2 * rho, kEpsScreened * kInv4Pi));
+        ok = r2 > 1e-20f;
         return __builtin_fmaf(sigma, rho, 1.0f) * dot;
     }
 }
@@ -83,20 +88,60 @@ __device__ __forceinline__ float gwn_term(float qx, float qy, float qz, const fl
     return gwn_term_d<SCREENED>(a.x - qx, a.y - qy, a.z - qz, a.w, b, sigma, sigma_l2e);
 }
 
-// plain term and its gradient w.r.t. q (b pre-scaled by 1/4π):
-//   w = (d·b)/D, D = r³ + ε/4π;  ∇w = −b/D + 3 r (d·b) d / D²
-__device__ __forceinline__ void gwn_term_grad(float qx, float qy, float qz, const float4 a, const float4 b,
-                                              float& w, float& gx, float& gy, float& gz) {
-    const float dx = a.x - qx, dy = a.y - qy, dz = a.z - qz;
+// DWG_NEAR_FP64: a screened point term inside its source's screening radius
+// (r < s, so ρ = s) in FP64, from the float source a = (p, s), b = A·n/4π
+// and query q; 0 unless r > 1e-10 (TurboLibiGWN.cu:544-585)
+__device__ __forceinline__ double near_term64(const float4 a, const float4 b, float qx, float qy, float qz,
+                                              float sigma) {
+    const double dx = double(a.x) - double(qx), dy = double(a.y) - double(qy), dz = double(a.z) - double(qz);
+    const double r2 = dx * dx + dy * dy + dz * dz;
+    if (!(r2 > 1e-20)) return 0.0;
+    const double rho = a.w, sg = sigma;
+    const double dot = dx * double(b.x) + dy * double(b.y) + dz * double(b.z);
+    const double den = exp(sg * rho) * (rho * rho * rho) + 1e-7 / (4.0 * 3.14159265358979323846);
+    return (sg * rho + 1.0) * dot / den;
+}
+
+// the split of a screened point term between the FP32 sum and the FP64 one:
+// near (r < s, r > 1e-10) goes to FP64
+__device__ __forceinline__ bool near_field(float r2, float s) { return r2 < s * s && r2 > 1e-20f; }
+
+// a term and its gradient w.r.t. q from the offset d = p − q (b pre-scaled
+// by 1/4π, ε' = ε/4π):
+//   plain     w = (d·b)/D, D = r³ + ε';  ∇w = −b/D + 3 r (d·b) d / D²
+//   screened  w = g(ρ)(d·b), g = (1 + σρ)/D, D = e^{σρ}ρ³ + ε', ρ = max(r, s),
+//             0 unless r > 1e-10;  ∇w = −g b − (d·b) g'(ρ) d / r for r ≥ s
+//             (∂ρ/∂q = −d/r), −g b for r < s;
+//             g' = (σε' − e^{σρ}ρ²(σ²ρ² + 3σρ + 3)) / D²  (no cancellation)
+template <bool SCREENED>
+__device__ __forceinline__ void gwn_term_grad_d(float dx, float dy, float dz, float s, const float4 b, float sigma,
+                                                float sigma_l2e, float& w, float& gx, float& gy, float& gz) {
     const float r2 = dist2(dx, dy, dz);
     const float dot = __builtin_fmaf(dz, b.z, __builtin_fmaf(dy, b.y, dx * b.x));
     const float r = fast_sqrt(r2);
-    const float inv = fast_rcp(__builtin_fmaf(r2, r, kEpsPlain * kInv4Pi));
-    w = dot * inv;
-    const float c = 3.0f * r * w * inv;
-    gx = __builtin_fmaf(c, dx, -b.x * inv);
-    gy = __builtin_fmaf(c, dy, -b.y * inv);
-    gz = __builtin_fmaf(c, dz, -b.z * inv);
+    if constexpr (!SCREENED) {
+        const float inv = fast_rcp(__builtin_fmaf(r2, r, kEpsPlain * kInv4Pi));
+        w = dot * inv;
+        const float c = 3.0f * r * w * inv;
+        gx = __builtin_fmaf(c, dx, -b.x * inv);
+        gy = __builtin_fmaf(c, dy, -b.y * inv);
+        gz = __builtin_fmaf(c, dz, -b.z * inv);
+    } else {
+        const bool ok = r > 1e-10f;
+        const bool inside = s > r;  // ρ = s: no dependence on q through ρ
+        const float rho = inside ? s : r;
+        const float er2 = fast_exp2(sigma_l2e * rho) * (rho * rho);
+        const float eps = kEpsScreened * kInv4Pi;
+        const float inv = fast_rcp(__builtin_fmaf(er2, rho, eps));
+        const float sr = sigma * rho;
+        const float g = __builtin_fmaf(sigma, rho, 1.0f) * inv;
+        const float gp = (sigma * eps - er2 * __builtin_fmaf(sr, sr + 3.0f, 3.0f)) * inv * inv;
+        const float c = (inside || !ok) ? 0.0f : -dot * gp * fast_rcp(r);
+        w = ok ? g * dot : 0.0f;
+        gx = ok ? __builtin_fmaf(c, dx, -g * b.x) : 0.0f;
+        gy = ok ? __builtin_fmaf(c, dy, -g * b.y) : 0.0f;
+        gz = ok ? __builtin_fmaf(c, dz, -g * b.z) : 0.0f;
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -124,7 +169,7 @@ __global__ void pack_sources_kernel(const float* __restrict__ pts, const float* 
 constexpr int kDirectThreads = 256;
 constexpr int kDirectTile = 64;
 
-template <bool SCREENED, bool KAHAN>
+template <bool SCREENED, bool KAHAN, bool NEAR64 = false>
 __global__ __launch_bounds__(kDirectThreads) void gwn_direct_kernel(
     const float4* __restrict__ sa, const float4* __restrict__ sb, int64_t n, const float* __restrict__ q,
     int64_t nq, float sigma, float* __restrict__ out) {
@@ -133,33 +178,47 @@ __global__ __launch_bounds__(kDirectThreads) void gwn_direct_kernel(
     const float qx = valid ? q[3 * i] : 0.f, qy = valid ? q[3 * i + 1] : 0.f, qz = valid ? q[3 * i + 2] : 0.f;
     const float sl2e = sigma * kLog2e;
     float sum = 0.f, comp = 0.f;
+    double near = 0.0;  // DWG_NEAR_FP64: the terms inside the screening radius, in FP64
+    auto term = [&](const float4 a, const float4 b) {
+        if constexpr (NEAR64) {
+            const float dx = a.x - qx, dy = a.y - qy, dz = a.z - qz;
+            const bool nr = near_field(dist2(dx, dy, dz), a.w);
+            if (__builtin_amdgcn_ballot_w64(nr)) {
+                if (nr) near += near_term64(a, b, qx, qy, qz, sigma);
+            }
+            return nr ? 0.f : gwn_term<SCREENED>(qx, qy, qz, a, b, sigma, sl2e);
+        }
+        return gwn_term<SCREENED>(qx, qy, qz, a, b, sigma, sl2e);
+    };
     int64_t j0 = 0;
     for (; j0 + kDirectTile <= n; j0 += kDirectTile) {
         float part = 0.f;
 #pragma unroll 8
-        for (int j = 0; j < kDirectTile; ++j) part += gwn_term<SCREENED>(qx, qy, qz, sa[j0 + j], sb[j0 + j], sigma, sl2e);
+        for (int j = 0; j < kDirectTile; ++j) part += term(sa[j0 + j], sb[j0 + j]);
         if (KAHAN) kahan_add(sum, comp, part); else sum += part;
     }
     float part = 0.f;
-    for (int64_t j = j0; j < n; ++j) part += gwn_term<SCREENED>(qx, qy, qz, sa[j], sb[j], sigma, sl2e);
+    for (int64_t j = j0; j < n; ++j) part += term(sa[j], sb[j]);
     if (KAHAN) kahan_add(sum, comp, part); else sum += part;
-    if (valid) out[i] = sum - comp;
+    if (valid) out[i] = NEAR64 ? float(double(sum - comp) + near) : sum - comp;
 }
 
-template <bool KAHAN>
+template <bool SCREENED, bool KAHAN>
 __global__ __launch_bounds__(kDirectThreads) void gwn_direct_grad_kernel(
     const float4* __restrict__ sa, const float4* __restrict__ sb, int64_t n, const float* __restrict__ q,
-    int64_t nq, float* __restrict__ out_w, float* __restrict__ out_g) {
+    int64_t nq, float sigma, float* __restrict__ out_w, float* __restrict__ out_g) {
     const int64_t i = int64_t(blockIdx.x) * kDirectThreads + threadIdx.x;
     const bool valid = i < nq;
     const float qx = valid ? q[3 * i] : 0.f, qy = valid ? q[3 * i + 1] : 0.f, qz = valid ? q[3 * i + 2] : 0.f;
+    const float sl2e = sigma * kLog2e;
     float s[4] = {0, 0, 0, 0}, c[4] = {0, 0, 0, 0};
     for (int64_t j0 = 0; j0 < n; j0 += kDirectTile) {
         const int64_t j1 = j0 + kDirectTile < n ? j0 + kDirectTile : n;
         float p[4] = {0, 0, 0, 0};
         for (int64_t j = j0; j < j1; ++j) {
             float w, gx, gy, gz;
-            gwn_term_grad(qx, qy, qz, sa[j], sb[j], w, gx, gy, gz);
+            const float4 a = sa[j];
+            gwn_term_grad_d<SCREENED>(a.x - qx, a.y - qy, a.z - qz, a.w, sb[j], sigma, sl2e, w, gx, gy, gz);
             p[0] += w;
             p[1] += gx;
             p[2] += gy;
@@ -180,10 +239,9 @@ __global__ __launch_bounds__(kDirectThreads) void gwn_direct_grad_kernel(
     }
 }
 
-static void check_direct_args(int64_t n, int64_t nq, const float* radii, uint32_t flags, const float* out_g) {
+static void check_direct_args(int64_t n, int64_t nq, const float* radii, uint32_t flags, const float*) {
     DWG_REQUIRE(n >= 0 && nq >= 0, "negative size");
     DWG_REQUIRE(!(flags & DWG_USE_RADII) || radii, "DWG_USE_RADII needs radii");
-    DWG_REQUIRE(!(out_g && (flags & DWG_SCREENED)), "gradient is defined for the plain kernel only");
 }
 
 // packs the n sources into sa/sb (room for n + 1 each) and runs the direct
@@ -202,8 +260,17 @@ static void direct_sums(dwg_ctx* ctx, float4* sa, float4* sb, const float* pts, 
     ProfileScope ps(profile ? ctx : nullptr,
                     out_g ? "gwn_direct_grad" : ((flags & DWG_SCREENED) ? "gwn_direct_screened" : "gwn_direct_plain"));
     if (out_g) {
-        if (kahan) hipLaunchKernelGGL(gwn_direct_grad_kernel<true>, grid, block, 0, ctx->stream, sa, sb, n, q, nq, out_w, out_g);
-        else hipLaunchKernelGGL(gwn_direct_grad_kernel<false>, grid, block, 0, ctx->stream, sa, sb, n, q, nq, out_w, out_g);
+        const bool scr = flags & DWG_SCREENED;
+#define DWG_GRAD_LAUNCH(S, K) \
+    hipLaunchKernelGGL((gwn_direct_grad_kernel<S, K>), grid, block, 0, ctx->stream, sa, sb, n, q, nq, sigma, out_w, out_g)
+        if (scr && kahan) DWG_GRAD_LAUNCH(true, true);
+        else if (scr) DWG_GRAD_LAUNCH(true, false);
+        else if (kahan) DWG_GRAD_LAUNCH(false, true);
+        else DWG_GRAD_LAUNCH(false, false);
+#undef DWG_GRAD_LAUNCH
+    } else if ((flags & DWG_SCREENED) && (flags & DWG_NEAR_FP64)) {
+        if (kahan) hipLaunchKernelGGL((gwn_direct_kernel<true, true, true>), grid, block, 0, ctx->stream, sa, sb, n, q, nq, sigma, out_w);
+        else hipLaunchKernelGGL((gwn_direct_kernel<true, false, true>), grid, block, 0, ctx->stream, sa, sb, n, q, nq, sigma, out_w);
     } else if (flags & DWG_SCREENED) {
         if (kahan) hipLaunchKernelGGL((gwn_direct_kernel<true, true>), grid, block, 0, ctx->stream, sa, sb, n, q, nq, sigma, out_w);
         else hipLaunchKernelGGL((gwn_direct_kernel<true, false>), grid, block, 0, ctx->stream, sa, sb, n, q, nq, sigma, out_w);
@@ -333,6 +400,30 @@ void octree_update(dwg_ctx* ctx, dwg_octree* T, const float* nrm, const float* a
     T->updated = true;
 }
 
+// the point terms of one-point leaves live in their payload too
+__global__ void leaf_payload_kernel(const int4* __restrict__ info, int64_t m, const float4* __restrict__ sa,
+                                    const float4* __restrict__ sb, float4* __restrict__ payload) {
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= m) return;
+    const int4 nf = info[i];
+    if (nf.x < 0 && nf.z - nf.y == 1) {
+        payload[2 * i] = sb[nf.y];
+        payload[2 * i + 1] = sa[nf.y];
+    }
+}
+
+void octree_set_point_areas(dwg_ctx* ctx, dwg_octree* T, const float* nrm, const float* area) {
+    DWG_REQUIRE(T->updated, "octree has no moments yet: call dwg_octree_update first");
+    const int64_t n = T->n_points, m = T->n_nodes;
+    ProfileScope ps(ctx, "octree_pack");
+    hipLaunchKernelGGL(pack_sources_kernel, dim3(grid_for(n, 256)), dim3(256), 0, ctx->stream, nullptr, nrm, area,
+                       nullptr, T->perm, n, 0, nullptr, T->sb, nullptr);
+    check_launch();
+    hipLaunchKernelGGL(leaf_payload_kernel, dim3(grid_for(m, 256)), dim3(256), 0, ctx->stream, T->info, m, T->sa,
+                       T->sb, T->payload);
+    check_launch();
+}
+
 // ---------------------------------------------------------------------------
 // Barnes-Hut traversal, one wave per 64 queries
 // ---------------------------------------------------------------------------
@@ -360,6 +451,12 @@ typedef float NativeFloat4 __attribute__((ext_vector_type(4)));
 __device__ __forceinline__ int2 scalar_load(const int2* p, int64_t i) {
     const NativeInt2 v = ((const __attribute__((address_space(4))) NativeInt2*)p)[i];
     return make_int2(v.x, v.y);
+}
+
+typedef int NativeInt4 __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ int4 scalar_load(const int4* p, int64_t i) {
+    const NativeInt4 v = ((const __attribute__((address_space(4))) NativeInt4*)p)[i];
+    return make_int4(v.x, v.y, v.z, v.w);
 }
 
 __device__ __forceinline__ float4 scalar_load(const float4* p, int64_t i) {
@@ -393,13 +490,14 @@ __device__ __forceinline__ int64_t xcd_block(int64_t b, int64_t nb) {
     return (b % 8) * per + b / 8;
 }
 
-template <bool SCREENED, bool COUNT, bool BF16, bool GRID>
+template <bool SCREENED, bool COUNT, bool BF16, bool GRID, bool NEAR64 = false>
 __global__ __launch_bounds__(kTreeThreads) void gwn_tree_kernel(TreeArgs T, QueryArgs Q, float beta, float sigma,
                                                                  uint32_t flags, float* __restrict__ out,
                                                                  int64_t* __restrict__ terms_out) {
     __shared__ int s_node[kTreeWaves][kStack];
     __shared__ uint64_t s_mask[kTreeWaves][kStack];
     __shared__ float4 s_pay[kTreeWaves][16];  // the 8 children's payloads of the node being opened
+    __shared__ float4 s_cmr[kTreeWaves][16];  // and their (CM, (βR)²) records (twice)
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int64_t blk = xcd_block(blockIdx.x, gridDim.x);
     const int64_t wid = blk * kTreeWaves + wave;
@@ -437,47 +535,63 @@ __global__ __launch_bounds__(kTreeThreads) void gwn_tree_kernel(TreeArgs T, Quer
     int64_t nterms = 0;
     const bool kahan = flags & DWG_KAHAN;
 
+    constexpr float kS2 = kScreenDefault * kScreenDefault;
+    double near = 0.0;  // DWG_NEAR_FP64: point terms inside the screening radius, in FP64
+    auto point_term = [&](const float4 a, const float4 b, bool on, float part) {
+        const float dx = a.x - qx, dy = a.y - qy, dz = a.z - qz;
+        const float r2 = dist2(dx, dy, dz);
+        if constexpr (NEAR64) {
+            const bool nr = on && near_field(r2, a.w);
+            if (__builtin_amdgcn_ballot_w64(nr)) {  // rare: lanes next to a source
+                if (nr) near += near_term64(a, b, qx, qy, qz, sigma);
+            }
+            on = on && !nr;
+        }
+        float inv;
+        bool ok;
+        const float x = tree_term_split<SCREENED>(dx, dy, dz, r2, a.w * a.w, b, sigma, sl2e, inv, ok);
+        if (BF16) return __builtin_fmaf((on && ok) ? x : 0.f, inv, part);  // last product fused into the sum
+        return part + ((on && ok) ? x * inv : 0.f);
+    };
     auto leaf_sum = [&](int b, int e, bool act) {
         float part = 0.f;
-        for (int j = b; j < e; ++j) {
-            const float t = gwn_term<SCREENED>(qx, qy, qz, T.sa[j], T.sb[j], sigma, sl2e);
-            part += act ? t : 0.f;
-        }
+        for (int j = b; j < e; ++j) part = point_term(T.sa[j], T.sb[j], act, part);
         if (COUNT && act) nterms += e - b;
         if (kahan) kahan_add(sum, comp, part); else sum += part;
     };
 
     // the stack holds (first child of a node to open, lanes that open it)
     int sp = 0;
-    const int4 root = T.info[0];
+    const int4 root = scalar_load(T.info, 0);
     if (root.x < 0) {
         leaf_sum(root.y, root.z, valid);
     } else {
-        if (lane == 0) {
-            s_node[wave][0] = root.x;
-            s_mask[wave][0] = all;
-        }
+        s_node[wave][0] = root.x;
+        s_mask[wave][0] = all;
         sp = 1;
     }
     while (sp > 0) {
         --sp;
         const int fc = uniform(s_node[wave][sp]);
         const uint64_t mask = uniform64(s_mask[wave][sp]);
-        const bool act = (mask >> lane) & 1;
+        const bool act = __builtin_amdgcn_inverse_ballot_w64(mask);
         float part = 0.f;
         // the 8 children are contiguous: their records come in one batch of
         // scalar loads, their payloads in one coalesced vector load (word
         // 8c + w of child c in lane 8c + w), so a node costs one memory
         // round trip however many of its children are far or leaves
         int2 nks[8];
-        float4 crs[8];
 #pragma unroll
         for (int c = 0; c < 8; ++c) nks[c] = scalar_load(T.kid, fc + c);
-#pragma unroll
-        for (int c = 0; c < 8; ++c) crs[c] = scalar_load(T.cm_r, fc + c);
-        // payload word 8c + w of child c: loaded by lane 8c + w, then read
-        // back by the whole wave from LDS at a uniform address (a broadcast)
-        reinterpret_cast<float*>(s_pay[wave])[lane] = T.payload[int64_t(fc) * 8 + lane];
+        // payload word 8c + w of child c: loaded by lane 8c + w, and (CM, (βR)²)
+        // word 4c + w by lanes 4c + w and 32 + 4c + w (a second copy keeps the
+        // store unconditional); both read back by the whole wave from LDS at
+        // uniform addresses (broadcasts).  Records in LDS instead of SGPRs
+        // leave the stack pointer room in the scalar registers
+        const float pw = T.payload[int64_t(fc) * 8 + lane];
+        const float cw = reinterpret_cast<const float*>(T.cm_r)[int64_t(fc) * 4 + (lane & 31)];
+        reinterpret_cast<float*>(s_pay[wave])[lane] = pw;
+        reinterpret_cast<float*>(s_cmr[wave])[lane] = cw;
         __builtin_amdgcn_wave_barrier();
 #pragma unroll
         for (int c = 0; c < 8; ++c) {
@@ -485,30 +599,16 @@ __global__ __launch_bounds__(kTreeThreads) void gwn_tree_kernel(TreeArgs T, Quer
             if (nk.x == 0) continue;  // empty
             if (nk.x < 0) {  // leaf child: summed point by point whether near or far
                 if (nk.x == -1) {
-                    const float4 b = s_pay[wave][2 * c];
-                    const float4 a = s_pay[wave][2 * c + 1];
-                    if (BF16) {  // mixed precision: the term's last product fused into the sum
-                        const float dx = a.x - qx, dy = a.y - qy, dz = a.z - qz;
-                        float inv;
-                        bool ok;
-                        const float x = gwn_term_split<SCREENED>(dx, dy, dz, dist2(dx, dy, dz), a.w, b, sigma, sl2e,
-                                                                 inv, ok);
-                        part = __builtin_fmaf((act && ok) ? x : 0.f, inv, part);
-                    } else {
-                        const float t = gwn_term<SCREENED>(qx, qy, qz, a, b, sigma, sl2e);
-                        part += act ? t : 0.f;
-                    }
+                    part = point_term(s_pay[wave][2 * c + 1], s_pay[wave][2 * c], act, part);
                 } else {
-                    for (int j = nk.y; j < nk.y - nk.x; ++j) {
-                        const float t = gwn_term<SCREENED>(qx, qy, qz, scalar_load(T.sa, j), scalar_load(T.sb, j), sigma, sl2e);
-                        part += act ? t : 0.f;
-                    }
+                    for (int j = nk.y; j < nk.y - nk.x; ++j)
+                        part = point_term(scalar_load(T.sa, j), scalar_load(T.sb, j), act, part);
                 }
                 if (COUNT && act) nterms += -nk.x;
                 continue;
             }
             // cm_r.w holds (β·R)², rounded as float(float(β·R)²), for this β
-            const float4 cr = crs[c];
+            const float4 cr = s_cmr[wave][c];
             const float dx = cr.x - qx, dy = cr.y - qy, dz = cr.z - qz;
             const float r2 = dist2(dx, dy, dz);
             const bool far = r2 > cr.w;
@@ -526,19 +626,116 @@ __global__ __launch_bounds__(kTreeThreads) void gwn_tree_kernel(TreeArgs T, Quer
                     // for every far node here (far radius floor)
                     float inv;
                     bool ok;
-                    const float x = gwn_term_split<SCREENED>(dx, dy, dz, r2, kScreenDefault, e4, sigma, sl2e, inv, ok);
+                    const float x = tree_term_split<SCREENED>(dx, dy, dz, r2, kS2, e4, sigma, sl2e, inv, ok);
                     part = __builtin_fmaf((act && far) ? x : 0.f, inv, part);
                 } else {
                     // d = (CM_f − q) + δ: the first difference is exact for
                     // nearby values, so d carries a single rounding
                     const float4 lo = s_pay[wave][2 * c + 1];
-                    const float t =
-                        gwn_term_d<SCREENED>(dx + lo.x, dy + lo.y, dz + lo.z, kScreenDefault, e4, sigma, sl2e);
+                    const float ex = dx + lo.x, ey = dy + lo.y, ez = dz + lo.z;
+                    float inv;
+                    bool ok;
+                    const float x = tree_term_split<SCREENED>(ex, ey, ez, dist2(ex, ey, ez), kS2, e4, sigma, sl2e, inv, ok);
                     // the plain dipole has no r > 1e-10 guard in the reference either
-                    part += (act && far) ? t : 0.f;
+                    part += ((act && far && ok) ? x * inv : 0.f);
                 }
                 if (COUNT && act && far) nterms += 1;
             }
+            if (nm) {
+                // every lane writes the same entry: no divergent branch, so
+                // the stack pointer stays in a scalar register
+                s_node[wave][sp] = nk.x;
+                s_mask[wave][sp] = nm;
+                ++sp;
+            }
+        }
+        if (kahan) kahan_add(sum, comp, part); else sum += part;
+    }
+    if (valid) {
+        if (out) out[qi] = NEAR64 ? float(double(sum - comp) + near) : sum - comp;
+        if (COUNT) terms_out[qi] = nterms;
+    }
+}
+
+// Barnes-Hut value and gradient at query points (new; the reference has no
+// gradient): the walk and far test of gwn_tree_kernel (FP32 path: far
+// dipoles at (CM_f − q) + residual), each lane accumulating w and ∇w of its
+// own term set.
+template <bool SCREENED>
+__global__ __launch_bounds__(kTreeThreads) void gwn_tree_grad_kernel(TreeArgs T, const float* __restrict__ q,
+                                                                      int64_t nq, float sigma,
+                                                                      float* __restrict__ out_w,
+                                                                      float* __restrict__ out_g) {
+    __shared__ int s_node[kTreeWaves][kStack];
+    __shared__ uint64_t s_mask[kTreeWaves][kStack];
+    __shared__ float4 s_pay[kTreeWaves][16];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int64_t qi = (int64_t(blockIdx.x) * kTreeWaves + wave) * 64 + lane;
+    const bool valid = qi < nq;
+    const float qx = valid ? q[3 * qi] : 0.f, qy = valid ? q[3 * qi + 1] : 0.f, qz = valid ? q[3 * qi + 2] : 0.f;
+    const uint64_t all = __builtin_amdgcn_ballot_w64(valid);
+    if (all == 0) return;
+    const float sl2e = sigma * kLog2e;
+    // the terms of one opened node sum into `part`, which folds into `acc`
+    // with Kahan compensation: the gradient's near terms are large and
+    // cancel, so plain FP32 accumulation over many nodes would lose digits
+    float acc[4] = {0.f, 0.f, 0.f, 0.f}, comp[4] = {0.f, 0.f, 0.f, 0.f}, part[4] = {0.f, 0.f, 0.f, 0.f};
+    auto add = [&](float dx, float dy, float dz, float sr, const float4 b, bool on) {
+        float w, gx, gy, gz;
+        gwn_term_grad_d<SCREENED>(dx, dy, dz, sr, b, sigma, sl2e, w, gx, gy, gz);
+        part[0] += on ? w : 0.f;
+        part[1] += on ? gx : 0.f;
+        part[2] += on ? gy : 0.f;
+        part[3] += on ? gz : 0.f;
+    };
+    auto fold = [&]() {
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            kahan_add(acc[k], comp[k], part[k]);
+            part[k] = 0.f;
+        }
+    };
+    auto leaf = [&](int b, int e, bool on) {
+        for (int j = b; j < e; ++j) {
+            const float4 a = scalar_load(T.sa, j);
+            add(a.x - qx, a.y - qy, a.z - qz, a.w, scalar_load(T.sb, j), on);
+        }
+    };
+    int sp = 0;
+    const int4 root = T.info[0];
+    if (root.x < 0) {
+        leaf(root.y, root.z, valid);
+        fold();
+    } else {
+        if (lane == 0) {
+            s_node[wave][0] = root.x;
+            s_mask[wave][0] = all;
+        }
+        sp = 1;
+    }
+    while (sp > 0) {
+        --sp;
+        const int fc = uniform(s_node[wave][sp]);
+        const uint64_t mask = uniform64(s_mask[wave][sp]);
+        const bool act = (mask >> lane) & 1;
+        reinterpret_cast<float*>(s_pay[wave])[lane] = T.payload[int64_t(fc) * 8 + lane];
+        __builtin_amdgcn_wave_barrier();
+        for (int c = 0; c < 8; ++c) {
+            const int2 nk = scalar_load(T.kid, fc + c);
+            if (nk.x == 0) continue;
+            if (nk.x < 0) {
+                leaf(nk.y, nk.y - nk.x, act);
+                continue;
+            }
+            const float4 cr = scalar_load(T.cm_r, fc + c);
+            const float dx = cr.x - qx, dy = cr.y - qy, dz = cr.z - qz;
+            const bool far = dist2(dx, dy, dz) > cr.w;
+            const uint64_t farm = __builtin_amdgcn_ballot_w64(far);
+            if (mask & farm) {
+                const float4 lo = s_pay[wave][2 * c + 1];
+                add(dx + lo.x, dy + lo.y, dz + lo.z, kScreenDefault, s_pay[wave][2 * c], act && far);
+            }
+            const uint64_t nm = mask & ~farm;
             if (nm) {
                 if (lane == 0) {
                     s_node[wave][sp] = nk.x;
@@ -547,11 +744,16 @@ __global__ __launch_bounds__(kTreeThreads) void gwn_tree_kernel(TreeArgs T, Quer
                 ++sp;
             }
         }
-        if (kahan) kahan_add(sum, comp, part); else sum += part;
+        fold();
+        __builtin_amdgcn_wave_barrier();
     }
     if (valid) {
-        if (out) out[qi] = sum - comp;
-        if (COUNT) terms_out[qi] = nterms;
+        out_w[qi] = acc[0] - comp[0];
+        if (out_g) {
+            out_g[3 * qi] = acc[1] - comp[1];
+            out_g[3 * qi + 1] = acc[2] - comp[2];
+            out_g[3 * qi + 2] = acc[3] - comp[3];
+        }
     }
 }
 
@@ -644,20 +846,23 @@ static void launch_tree(dwg_ctx* ctx, const dwg_octree* Tc, const QueryArgs& Q, 
     const char* name = count ? "gwn_tree_terms" : (!Q.grid ? (scr ? "gwn_tree_screened" : "gwn_tree_plain")
                                                            : (scr ? "gwn_tree_grid_screened" : "gwn_tree_grid_plain"));
     ProfileScope ps(ctx, name);
-#define DWG_TREE_LAUNCH(S, C, B, G) \
-    hipLaunchKernelGGL((gwn_tree_kernel<S, C, B, G>), grid, block, 0, ctx->stream, A, Q, beta, sigma, flags, out, terms)
-#define DWG_TREE_LAUNCH_Q(S, C, B)       \
-    do {                                 \
-        if (!Q.grid)                     \
-            DWG_TREE_LAUNCH(S, C, B, false); \
-        else                             \
-            DWG_TREE_LAUNCH(S, C, B, true);  \
+    const bool near64 = scr && (flags & DWG_NEAR_FP64);
+#define DWG_TREE_LAUNCH(S, C, B, G, N) \
+    hipLaunchKernelGGL((gwn_tree_kernel<S, C, B, G, N>), grid, block, 0, ctx->stream, A, Q, beta, sigma, flags, out, terms)
+#define DWG_TREE_LAUNCH_Q(S, C, B, N)       \
+    do {                                    \
+        if (!Q.grid)                        \
+            DWG_TREE_LAUNCH(S, C, B, false, N); \
+        else                                \
+            DWG_TREE_LAUNCH(S, C, B, true, N);  \
     } while (0)
-    if (count) DWG_TREE_LAUNCH_Q(false, true, false);
-    else if (scr && bf) DWG_TREE_LAUNCH_Q(true, false, true);
-    else if (scr) DWG_TREE_LAUNCH_Q(true, false, false);
-    else if (bf) DWG_TREE_LAUNCH_Q(false, false, true);
-    else DWG_TREE_LAUNCH_Q(false, false, false);
+    if (count) DWG_TREE_LAUNCH_Q(false, true, false, false);
+    else if (near64 && bf) DWG_TREE_LAUNCH_Q(true, false, true, true);
+    else if (near64) DWG_TREE_LAUNCH_Q(true, false, false, true);
+    else if (scr && bf) DWG_TREE_LAUNCH_Q(true, false, true, false);
+    else if (scr) DWG_TREE_LAUNCH_Q(true, false, false, false);
+    else if (bf) DWG_TREE_LAUNCH_Q(false, false, true, false);
+    else DWG_TREE_LAUNCH_Q(false, false, false, false);
 #undef DWG_TREE_LAUNCH_Q
 #undef DWG_TREE_LAUNCH
     check_launch();
@@ -678,6 +883,36 @@ void gwn_tree_grid(dwg_ctx* ctx, const dwg_octree* T, const dwg_grid* g, float b
                    float* out) {
     DWG_REQUIRE(g && g->res[0] > 0 && g->res[1] > 0 && g->res[2] > 0 && g->h > 0, "bad grid");
     launch_tree(ctx, T, make_grid(g), beta, sigma, flags, out, nullptr, false);
+}
+
+void gwn_tree_grad(dwg_ctx* ctx, const dwg_octree* Tc, const float* q, int64_t nq, float beta, float sigma,
+                   uint32_t flags, float* out_w, float* out_g) {
+    DWG_REQUIRE(Tc && Tc->updated, "octree has no moments yet: call dwg_octree_update first");
+    DWG_REQUIRE(Tc->max_level <= 21, "octree deeper than the traversal stack");
+    DWG_REQUIRE(nq >= 0 && (q || nq == 0) && (out_w || nq == 0), "bad query or output array");
+    auto* T = const_cast<dwg_octree*>(Tc);  // far radii follow β and the payload precision; the tree is unchanged
+    if (!(T->far_beta == beta) || T->far_floor != 0.f) {
+        hipLaunchKernelGGL(far_radius_kernel, dim3(grid_for(T->n_nodes, 256)), dim3(256), 0, ctx->stream, T->radius,
+                           T->n_nodes, beta, 0.f, T->cm_r);
+        check_launch();
+        T->far_beta = beta;
+        T->far_floor = 0.f;
+    }
+    if (T->payload_bf16) {
+        hipLaunchKernelGGL(payload_precision_kernel, dim3(grid_for(T->n_nodes, 256)), dim3(256), 0, ctx->stream, T->kid,
+                           T->ec, T->n_nodes, 0, T->payload);
+        check_launch();
+        T->payload_bf16 = false;
+    }
+    if (nq == 0) return;
+    TreeArgs A{T->info, T->kid, T->cm_r, reinterpret_cast<const float*>(T->payload), T->sa, T->sb};
+    dim3 grid(grid_for((nq + 63) / 64, kTreeWaves)), block(kTreeThreads);
+    ProfileScope ps(ctx, "gwn_tree_grad");
+    if (flags & DWG_SCREENED)
+        hipLaunchKernelGGL(gwn_tree_grad_kernel<true>, grid, block, 0, ctx->stream, A, q, nq, sigma, out_w, out_g);
+    else
+        hipLaunchKernelGGL(gwn_tree_grad_kernel<false>, grid, block, 0, ctx->stream, A, q, nq, sigma, out_w, out_g);
+    check_launch();
 }
 
 void gwn_tree_terms(dwg_ctx* ctx, const dwg_octree* T, const float* q, int64_t nq, float beta, int64_t* terms) {

@@ -34,7 +34,9 @@ static void usage() {
                  "--beta   Barnes-Hut accuracy (default 2.3; <= 0 direct sums)\n"
                  "--kahan   1: compensated accumulation\n"
                  "--bf16_far   1: bf16 far-field moments\n"
+                 "--near_fp64   1: point terms inside the screening radius in FP64\n"
                  "--seed   seed of the random initial normals\n"
+                 "--residual_tol   > 0: stop after an even iteration count whose convergence residual is below it\n"
                  "--mlp   learned-initialisation weights (.bin) for --in_normal 4\n"
                  "--verbose   1: per-iteration log\n";
 }
@@ -51,38 +53,49 @@ int main(int argc, char* argv[]) {
     std::string in_path = " ", in_name = " ", out_path = " ", mlp_path;
     dwg_options opt;
     dwg_options_default(&opt);
+    if ((argc - 1) % 2 != 0) {
+        std::cerr << "option " << argv[argc - 1] << " has no value\n";
+        return 1;
+    }
     for (int i = 1; i + 1 < argc; i += 2) {
         const std::string k = argv[i];
         const char* v = argv[i + 1];
-        if (k == "--in_path") in_path = v;
-        else if (k == "--in_name") in_name = v;
-        else if (k == "--out_path") out_path = v;
-        else if (k == "--in_normal") opt.input_normal = std::stoi(v);
-        else if (k == "--grid_reso") opt.grid_reso = std::stoi(v);
-        else if (k == "--final_reso") opt.final_reso = std::stoi(v);
-        else if (k == "--max_depth") opt.max_depth = std::stoi(v);
-        else if (k == "--kw") opt.kw = std::stoi(v);
-        else if (k == "--final_kw") opt.final_kw = std::stoi(v);
-        else if (k == "--normal_kw") opt.normal_kw = std::stoi(v);
-        else if (k == "--lambda") opt.sigma = std::stof(v);
-        else if (k == "--min_num") opt.min_num = std::stoi(v);
-        else if (k == "--total_iter") opt.total_iter = std::stoi(v);
-        else if (k == "--output_mode") opt.output_mode = std::stoi(v);
-        else if (k == "--use_leaves_generate") opt.use_leaves_generate = std::stoi(v);
-        else if (k == "--use_radii") opt.use_radii = std::stoi(v);
-        else if (k == "--com_As") opt.com_As = std::stoi(v);
-        else if (k == "--As_thres") opt.As_threshold = std::stof(v);
-        else if (k == "--offset_1") opt.offset_1 = std::stof(v);
-        else if (k == "--offset_2") opt.offset_2 = std::stof(v);
-        else if (k == "--test_iter") opt.test_iter = std::stoi(v);
-        else if (k == "--source_mode") opt.source_mode = std::stoi(v);
-        else if (k == "--leaf_size") opt.leaf_size = std::stoi(v);
-        else if (k == "--beta") opt.beta = std::stof(v);
-        else if (k == "--kahan") opt.precision = std::stoi(v) ? (opt.precision | DWG_KAHAN) : (opt.precision & ~DWG_KAHAN);
-        else if (k == "--bf16_far") opt.precision = std::stoi(v) ? (opt.precision | DWG_FAR_BF16) : (opt.precision & ~DWG_FAR_BF16);
-        else if (k == "--seed") opt.seed = std::stoull(v);
-        else if (k == "--mlp") mlp_path = v;
-        else if (k == "--verbose") opt.verbose = std::stoi(v);
+        try {
+            if (k == "--in_path") in_path = v;
+            else if (k == "--in_name") in_name = v;
+            else if (k == "--out_path") out_path = v;
+            else if (k == "--in_normal") opt.input_normal = std::stoi(v);
+            else if (k == "--grid_reso") opt.grid_reso = std::stoi(v);
+            else if (k == "--final_reso") opt.final_reso = std::stoi(v);
+            else if (k == "--max_depth") opt.max_depth = std::stoi(v);
+            else if (k == "--kw") opt.kw = std::stoi(v);
+            else if (k == "--final_kw") opt.final_kw = std::stoi(v);
+            else if (k == "--normal_kw") opt.normal_kw = std::stoi(v);
+            else if (k == "--lambda") opt.sigma = std::stof(v);
+            else if (k == "--min_num") opt.min_num = std::stoi(v);
+            else if (k == "--total_iter") opt.total_iter = std::stoi(v);
+            else if (k == "--output_mode") opt.output_mode = std::stoi(v);
+            else if (k == "--use_leaves_generate") opt.use_leaves_generate = std::stoi(v);
+            else if (k == "--use_radii") opt.use_radii = std::stoi(v);
+            else if (k == "--com_As") opt.com_As = std::stoi(v);
+            else if (k == "--As_thres") opt.As_threshold = std::stof(v);
+            else if (k == "--offset_1") opt.offset_1 = std::stof(v);
+            else if (k == "--offset_2") opt.offset_2 = std::stof(v);
+            else if (k == "--test_iter") opt.test_iter = std::stoi(v);
+            else if (k == "--source_mode") opt.source_mode = std::stoi(v);
+            else if (k == "--leaf_size") opt.leaf_size = std::stoi(v);
+            else if (k == "--beta") opt.beta = std::stof(v);
+            else if (k == "--kahan") opt.precision = std::stoi(v) ? (opt.precision | DWG_KAHAN) : (opt.precision & ~DWG_KAHAN);
+            else if (k == "--bf16_far") opt.precision = std::stoi(v) ? (opt.precision | DWG_FAR_BF16) : (opt.precision & ~DWG_FAR_BF16);
+            else if (k == "--near_fp64") opt.precision = std::stoi(v) ? (opt.precision | DWG_NEAR_FP64) : (opt.precision & ~DWG_NEAR_FP64);
+            else if (k == "--seed") opt.seed = std::stoull(v);
+            else if (k == "--residual_tol") opt.residual_tol = std::stof(v);
+            else if (k == "--mlp") mlp_path = v;
+            else if (k == "--verbose") opt.verbose = std::stoi(v);
+        } catch (const std::exception&) {
+            std::cerr << "bad value '" << v << "' for option " << k << "\n";
+            return 1;
+        }
     }
     // output naming (src/main.cpp:151-156)
     const std::string full_in = in_path + "/" + in_name;

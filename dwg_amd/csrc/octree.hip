@@ -127,7 +127,9 @@ __global__ void children_kernel(int4* __restrict__ info, double4* __restrict__ c
 }
 
 // Bottom-up ΣA·p and ΣA of the nodes of one level; cm = ΣA·p/ΣA (0 for an
-// empty node).
+// empty node; NaN for a node whose points all have zero area, as the
+// reference's 0/0 gives, TurboLibiGWN.cu:986-988: such a node is never far,
+// its radius comes out NaN too).
 __global__ void node_sums_kernel(const int4* __restrict__ info, int64_t ls, int64_t le, const int* __restrict__ perm,
                                  const double* __restrict__ P, const double* __restrict__ A,
                                  double4* __restrict__ sums, double4* __restrict__ cm) {
@@ -154,7 +156,9 @@ __global__ void node_sums_kernel(const int4* __restrict__ info, int64_t ls, int6
         }
     }
     sums[i] = make_double4(sx, sy, sz, sa);
-    cm[i] = sa > 0 ? make_double4(sx / sa, sy / sa, sz / sa, sa) : make_double4(0, 0, 0, 0);
+    const double nan = __longlong_as_double(0x7ff8000000000000ll);
+    cm[i] = sa > 0 ? make_double4(sx / sa, sy / sa, sz / sa, sa)
+                   : (nf.z > nf.y ? make_double4(nan, nan, nan, 0) : make_double4(0, 0, 0, 0));
 }
 
 __device__ __forceinline__ uint64_t wave_max_u64(uint64_t v) {

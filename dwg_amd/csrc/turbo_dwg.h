@@ -92,7 +92,7 @@ public:
                     dwg_octree* prebuilt = nullptr, const float* d_area = nullptr);
     // per-source areas (device, n floats) for the far-field moments EC = Σ A·n
     // (TurboDWG::UpdateAs, TurboDWG.cu:693-701); CM and R keep the areas the
-    // octree was built with
+    // octree was built with (SetSources with d_area builds it area-weighted)
     void SetAreas(const float* d_area);
     // Radii of the sources from their k nearest neighbours (GetRadiiandAs /
     // TurboDWG::UpdateRadii).
@@ -101,7 +101,11 @@ public:
     // learned initialisation: MLP normals from the k-NN patches of the sources
     void MlpInit(const dwg_mlp* mlp);
     // TurboScreenedDWGLaunch (TurboDWG.cu:456-672); returns wall seconds.
-    double TurboScreenedDWGLaunch(int total_iters, double* w_mean, int64_t* iter_tris, int64_t* iter_zeros);
+    // residual (nullable): per-iteration convergence residual; with
+    // residual_tol > 0 the loop stops after the first even iteration count
+    // whose residual is below it; *iters_run receives the count run.
+    double TurboScreenedDWGLaunch(int total_iters, double* w_mean, int64_t* iter_tris, int64_t* iter_zeros,
+                                  double* residual = nullptr, int* iters_run = nullptr);
     // GenerateMesh (TurboDWG.cu:808-924): transfer the source normals to the
     // points, one screened evaluation at the points and on the final grid, MC.
     // d_points: device FP64 (np×3); points_tree (nullable, ownership taken):
@@ -126,7 +130,9 @@ public:
 
 private:
     void check(int rc, const char* what) const;
-    void mesh_pass(const Grid& g, float iso_scale, int64_t* tri_count, HostMesh* tris);
+    // filter_areas: apply the outlier filter to the point terms after the
+    // moment refresh (GenerateMesh over the points)
+    void mesh_pass(const Grid& g, float iso_scale, int64_t* tri_count, HostMesh* tris, bool filter_areas);
     void dump_mesh(const dwg_grid& g, float iso, int64_t ntri, int it);
 
     dwg_ctx* ctx_;

@@ -323,7 +323,8 @@ static float sigma_schedule(int it, float sigma_in, int warm_iters) {
     return s;
 }
 
-double TurboDWG::TurboScreenedDWGLaunch(int total_iters, double* w_mean, int64_t* iter_tris, int64_t* iter_zeros) {
+double TurboDWG::TurboScreenedDWGLaunch(int total_iters, double* w_mean, int64_t* iter_tris, int64_t* iter_zeros,
+                                        double* residual, int* iters_run) {
     const auto t0 = Clock::now();
     if (opt_.As_threshold > 0) check(dwg_outlier_filter(ctx_, area_.ptr, n_, opt_.As_threshold), "outlier filter");
     const uint32_t flags = DWG_SCREENED | (opt_.use_radii ? DWG_USE_RADII : 0u) | opt_.precision;
@@ -342,7 +343,13 @@ double TurboDWG::TurboScreenedDWGLaunch(int total_iters, double* w_mean, int64_t
         grid_pts.upload(gp.data(), int64_t(gp.size()), ctx_->stream);
     }
     int64_t capacity = std::max<int64_t>(cent_.count / 3, 1 << 20);
-    for (int it = 0; it < total_iters; ++it) {
+    // the convergence residual compares each iteration's normals with the
+    // previous ones (only when it is asked for or stops the loop)
+    const bool want_residual = residual || opt_.residual_tol > 0;
+    DevArray<float> prev;
+    if (want_residual) prev.resize(3 * n_);
+    int it = 0;
+    for (; it < total_iters; ++it) {
         const float sigma = sigma_schedule(it, opt_.sigma, opt_.sigma_warm_iters);
         const float* rad = opt_.use_radii ? radii_.ptr : nullptr;
         check(dwg_octree_update(ctx_, tree_, nrm_.ptr, area_.ptr, rad), "moments");
@@ -375,12 +382,26 @@ double TurboDWG::TurboScreenedDWGLaunch(int total_iters, double* w_mean, int64_t
         if (!dump_dir_.empty() && opt_.test_iter != 999999 && opt_.test_iter > 0 && it % opt_.test_iter == 0)
             dump_mesh(g, iso, ntri, it);
         int64_t nz = 0;
+        if (want_residual)
+            DWG_HIP_CHECK(hipMemcpyAsync(prev.ptr, nrm_.ptr, sizeof(float) * 3 * n_, hipMemcpyDeviceToDevice,
+                                         ctx_->stream));
         check(dwg_diffuse_normals(ctx_, knn_, pts_.ptr, n_, cent_.ptr, tri_nrm_.ptr, ntri, 15, 10, 10, nrm_.ptr, &nz),
               "diffuse");
         if (iter_zeros) iter_zeros[it] = nz;
-        if (opt_.verbose) std::printf("iter = %d  W_mean = %.6g  triangles = %lld  zero normals = %lld\n", it,
-                                      double(w_mean ? w_mean[it] : iso), (long long)ntri, (long long)nz);
+        double res = -1.0;
+        if (want_residual) {
+            check(dwg_normal_change(ctx_, prev.ptr, nrm_.ptr, n_, &res), "residual");
+            if (residual) residual[it] = res;
+        }
+        if (opt_.verbose) std::printf("iter = %d  W_mean = %.6g  triangles = %lld  zero normals = %lld  residual = %.3g\n",
+                                      it, double(w_mean ? w_mean[it] : iso), (long long)ntri, (long long)nz, res);
+        // an even count keeps the start's side (the sign flips every iteration)
+        if (opt_.residual_tol > 0 && (it + 1) % 2 == 0 && res < double(opt_.residual_tol)) {
+            ++it;
+            break;
+        }
     }
+    if (iters_run) *iters_run = it;
     check(dwg_ctx_sync(ctx_), "sync");
     return seconds_since(t0);
 }
@@ -422,7 +443,7 @@ void TurboDWG::CountTerms(int64_t* src_terms, int64_t* grid_terms) {
     check(dwg_gwn_tree_grid_terms(ctx_, tree_, &g, opt_.beta, grid_terms), "grid terms");
 }
 
-void TurboDWG::mesh_pass(const Grid& grid, float iso_scale, int64_t* tri_count, HostMesh* tris) {
+void TurboDWG::mesh_pass(const Grid& grid, float iso_scale, int64_t* tri_count, HostMesh* tris, bool filter_areas) {
     // one screened evaluation at the sources and on the grid with the
     // unscheduled σ, iso = mean source value, marching cubes
     struct Tag {  // kernels of the final pass are profiled as "<kernel>:final"
@@ -439,7 +460,13 @@ void TurboDWG::mesh_pass(const Grid& grid, float iso_scale, int64_t* tri_count, 
     float* wg = static_cast<float*>(ctx_->buf("final_field", sizeof(float) * size_t(grid.size())));
     const float* rad = opt_.use_radii ? radii_.ptr : nullptr;
     check(dwg_octree_update(ctx_, tree_, nrm_.ptr, area_.ptr, rad), "moments");
-    if (opt_.As_threshold > 0) check(dwg_outlier_filter(ctx_, area_.ptr, n_, opt_.As_threshold), "outlier filter");
+    if (filter_areas && opt_.As_threshold > 0) {
+        // GenerateMesh filters the areas after the moment refresh
+        // (TurboDWG.cu:870-875): the far-field moments keep the unfiltered
+        // areas, the point terms take the filtered ones
+        check(dwg_outlier_filter(ctx_, area_.ptr, n_, opt_.As_threshold), "outlier filter");
+        check(dwg_octree_set_point_areas(ctx_, tree_, nrm_.ptr, area_.ptr), "point areas");
+    }
     const float beta = opt_.beta > 0 ? opt_.beta : 1e30f;  // β → ∞ takes every point term
     check(dwg_gwn_tree_self(ctx_, tree_, beta, opt_.sigma, flags, w_src_.ptr), "gwn points");
     check(dwg_gwn_tree_grid(ctx_, tree_, &g, beta, opt_.sigma, flags, wg), "gwn final grid");
@@ -491,11 +518,13 @@ void TurboDWG::GenerateMesh(const double* d_points, int64_t np, const Grid& fina
         dwg::point_areas(ctx_, pidx.get(), dp.ptr, np, 15, dn.ptr, pa.ptr);
     }
     // the points become the sources (TurboDWG.cu:865-876), reusing the octree
-    // over the input points (TurboDWG.cu:869-870), built with constant areas
-    SetSources(d_points, np, dn.ptr, grid_, owned.release());
-    if (opt_.com_As) SetAreas(pa.ptr);
+    // over the input points (TurboDWG.cu:869-870), built with constant areas.
+    // With --com_As 1, MeshGeneration refreshes CM, R and EC with the computed
+    // areas (UpdateGWNInfo, TurboLibiGWN.cu:954-1017, called at
+    // TurboDWG.cu:931): the octree is built again with area-weighted moments
+    SetSources(d_points, np, dn.ptr, grid_, owned.release(), opt_.com_As ? pa.ptr : nullptr);
     UpdateRadii(opt_.final_kw);
-    mesh_pass(final_grid, 1.0f, tri_count, tris);
+    mesh_pass(final_grid, 1.0f, tri_count, tris, true);
     if (point_normals) {
         point_normals->resize(3 * np);
         dn.download(point_normals->data(), 3 * np, ctx_->stream);
@@ -503,8 +532,10 @@ void TurboDWG::GenerateMesh(const double* d_points, int64_t np, const Grid& fina
 }
 
 void TurboDWG::GenerateMeshFromSources(const Grid& final_grid, int64_t* tri_count, HostMesh* tris) {
+    // GenerateMesh(grid) (TurboDWG.cu:768-806): the areas were filtered once
+    // before the iterations and are not filtered again
     UpdateRadii(opt_.final_kw);
-    mesh_pass(final_grid, 1.0f, tri_count, tris);
+    mesh_pass(final_grid, 1.0f, tri_count, tris, false);
 }
 
 // ---------------------------------------------------------------------------
@@ -666,8 +697,16 @@ void DWGLaunch(dwg_ctx* ctx, const float* pts, const float* nrm, int64_t n, cons
         dwg.set_normals(keep);
     }
     const auto t1 = Clock::now();
+    int iters_run = 0;
     const double t_iter = dwg.TurboScreenedDWGLaunch(opt.total_iter, res ? res->w_mean : nullptr,
-                                                     res ? res->iter_tris : nullptr, res ? res->iter_zeros : nullptr);
+                                                     res ? res->iter_tris : nullptr, res ? res->iter_zeros : nullptr,
+                                                     res ? res->residual : nullptr, &iters_run);
+    if (res) res->iterations = iters_run;
+    if (res && res->source_normals) {  // the sources' normals after the iterations
+        std::vector<float> sn;
+        dwg.download_normals(sn);
+        std::copy(sn.begin(), sn.end(), res->source_normals);
+    }
     const auto t2 = Clock::now();
     int64_t ntri = 0;
     HostMesh tris;
@@ -701,8 +740,10 @@ void DWGLaunch(dwg_ctx* ctx, const float* pts, const float* nrm, int64_t n, cons
         }
     }
     if (res) {
-        if (res->normals && opt.use_leaves_generate != 1)
-            std::copy(point_nrm.begin(), point_nrm.begin() + std::min<size_t>(point_nrm.size(), size_t(3 * n)), res->normals);
+        // the points' normals (with --use_leaves_generate 1 the final pass
+        // keeps the sources' normals: returned when the sources are the points)
+        if (res->normals && point_nrm.size() == size_t(3 * n))
+            std::copy(point_nrm.begin(), point_nrm.end(), res->normals);
         res->tri_count = ntri;
         res->mesh = (res->want_mesh && ntri > 0) ? tris.release() : nullptr;
         res->n_sources = ns;
@@ -751,6 +792,7 @@ void dwg_options_default(dwg_options* o) {
     o->count_terms = 0;
     o->mlp = nullptr;
     o->sigma_warm_iters = 4;
+    o->residual_tol = 0.0f;
 }
 
 void dwg_free(void* p) { dwg_host::free_host(p); }

@@ -14,7 +14,7 @@ import numpy as np
 import torch
 
 from . import _lib
-from ._lib import DWG_FAR_BF16, DWG_KAHAN, DWG_SCREENED, DWG_USE_RADII, DwgGrid
+from ._lib import DWG_FAR_BF16, DWG_KAHAN, DWG_NEAR_FP64, DWG_SCREENED, DWG_USE_RADII, DwgGrid
 
 
 class DwgError(RuntimeError):
@@ -87,11 +87,12 @@ class Context:
         return _f32(x, self.device)
 
     # ---- sums ---------------------------------------------------------------
-    def gwn_direct(self, pts, nrm, area, q, sigma=10.0, screened=False, radii=None, kahan=True, grad=False):
+    def gwn_direct(self, pts, nrm, area, q, sigma=10.0, screened=False, radii=None, kahan=True, grad=False,
+                   near64=False):
         pts, nrm, area, q = self.f32(pts), self.f32(nrm), self.f32(area), self.f32(q)
         rad = None if radii is None else self.f32(radii)
         flags = (DWG_SCREENED if screened else 0) | (DWG_USE_RADII if rad is not None else 0) | \
-                (DWG_KAHAN if kahan else 0)
+                (DWG_KAHAN if kahan else 0) | (DWG_NEAR_FP64 if near64 else 0)
         nq = q.numel() // 3
         w = torch.empty(nq, device=self.device, dtype=torch.float32)
         g = torch.empty((nq, 3), device=self.device, dtype=torch.float32) if grad else None
@@ -156,31 +157,65 @@ class Octree:
         c.check(c.L.dwg_octree_leaves(c.h, self.h, int(min_num), _ptr(cen), _ptr(wid), ctypes.byref(cnt)))
         return cen[:cnt.value].cpu().numpy(), wid[:cnt.value].cpu().numpy()
 
+    def leaf_normals(self, pts, nrm, min_num=0):
+        """SmoothNormalstoOctreeLeaves + whole-matrix normalisation of the
+        tree's points' normals onto its leaves (dwg_octree_leaf_normals)."""
+        c = self.ctx
+        P = torch.as_tensor(np.ascontiguousarray(np.asarray(pts, np.float64).reshape(-1, 3))).to(c.device)
+        N = torch.as_tensor(np.ascontiguousarray(np.asarray(nrm, np.float64).reshape(-1, 3))).to(c.device)
+        nl = self.info()["leaves"]
+        out = torch.empty((max(nl, 1), 3), device=c.device, dtype=torch.float32)
+        cnt = ctypes.c_int64()
+        c.check(c.L.dwg_octree_leaf_normals(c.h, self.h, int(min_num), _ptr(P), _ptr(N), _ptr(out), ctypes.byref(cnt)))
+        return out[:cnt.value].cpu().numpy()
+
     def update(self, nrm, area, radii=None):
         c = self.ctx
         self._nrm, self._area = c.f32(nrm), c.f32(area)
         self._radii = None if radii is None else c.f32(radii)
         c.check(c.L.dwg_octree_update(c.h, self.h, _ptr(self._nrm), _ptr(self._area), _ptr(self._radii)))
 
-    def _flags(self, screened, use_radii, kahan, bf16):
-        return (DWG_SCREENED if screened else 0) | (DWG_USE_RADII if use_radii else 0) | \
-               (DWG_KAHAN if kahan else 0) | (DWG_FAR_BF16 if bf16 else 0)
+    def set_point_areas(self, area):
+        """Point terms with other areas; the far-field moments stay those of
+        the last update (dwg_octree_set_point_areas)."""
+        c = self.ctx
+        self._point_area = c.f32(area)
+        c.check(c.L.dwg_octree_set_point_areas(c.h, self.h, _ptr(self._nrm), _ptr(self._point_area)))
 
-    def eval(self, q, beta=2.3, sigma=10.0, screened=True, kahan=False, bf16=False, out=None):
+    def _flags(self, screened, use_radii, kahan, bf16, near64=False):
+        if use_radii is None:  # per-source radii whenever update() was given them
+            use_radii = getattr(self, "_radii", None) is not None
+        return (DWG_SCREENED if screened else 0) | (DWG_USE_RADII if use_radii else 0) | \
+               (DWG_KAHAN if kahan else 0) | (DWG_FAR_BF16 if bf16 else 0) | (DWG_NEAR_FP64 if near64 else 0)
+
+    def eval(self, q, beta=2.3, sigma=10.0, screened=True, kahan=False, bf16=False, out=None, use_radii=None,
+             near64=False):
         c = self.ctx
         q = c.f32(q)
         nq = q.numel() // 3
         out = torch.empty(nq, device=c.device, dtype=torch.float32) if out is None else out
         c.check(c.L.dwg_gwn_tree(c.h, self.h, _ptr(q), nq, float(beta), float(sigma),
-                                 self._flags(screened, False, kahan, bf16), _ptr(out)))
+                                 self._flags(screened, use_radii, kahan, bf16, near64), _ptr(out)))
         return out
 
-    def eval_grid(self, grid: DwgGrid, beta=2.3, sigma=10.0, screened=True, kahan=False, bf16=False, out=None):
+    def eval_grad(self, q, beta=2.3, sigma=10.0, screened=True):
+        """Value and gradient of the Barnes-Hut sums at q (dwg_gwn_tree_grad)."""
+        c = self.ctx
+        q = c.f32(q)
+        nq = q.numel() // 3
+        w = torch.empty(nq, device=c.device, dtype=torch.float32)
+        g = torch.empty((nq, 3), device=c.device, dtype=torch.float32)
+        c.check(c.L.dwg_gwn_tree_grad(c.h, self.h, _ptr(q), nq, float(beta), float(sigma),
+                                      DWG_SCREENED if screened else 0, _ptr(w), _ptr(g)))
+        return w, g
+
+    def eval_grid(self, grid: DwgGrid, beta=2.3, sigma=10.0, screened=True, kahan=False, bf16=False, out=None,
+                  use_radii=None):
         c = self.ctx
         G = grid.res[0] * grid.res[1] * grid.res[2]
         out = torch.empty(G, device=c.device, dtype=torch.float32) if out is None else out
         c.check(c.L.dwg_gwn_tree_grid(c.h, self.h, ctypes.byref(grid), float(beta), float(sigma),
-                                      self._flags(screened, False, kahan, bf16), _ptr(out)))
+                                      self._flags(screened, use_radii, kahan, bf16), _ptr(out)))
         return out
 
     def terms(self, q, beta=2.3):

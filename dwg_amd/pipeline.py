@@ -3,7 +3,8 @@
 ``reconstruct(points, **options)`` mirrors the DWG_CUDA command line
 (src/main.cpp:17-166): every option keeps the reference flag's name and
 default; the MI355X-only options are ``source_mode``, ``leaf_size``,
-``beta``, ``kahan``, ``bf16_far`` and ``seed``.
+``beta``, ``kahan``, ``bf16_far``, ``near_fp64``, ``residual_tol`` and
+``seed``.
 """
 from __future__ import annotations
 
@@ -14,7 +15,7 @@ from dataclasses import dataclass, field
 import numpy as np
 
 from . import _lib
-from ._lib import DWG_FAR_BF16, DWG_KAHAN, DwgOptions, DwgResult
+from ._lib import DWG_FAR_BF16, DWG_KAHAN, DWG_NEAR_FP64, DwgOptions, DwgResult
 
 
 @dataclass
@@ -34,6 +35,9 @@ class Reconstruction:
     terms_src: int = 0
     terms_grid: int = 0
     grid_points: int = 0
+    source_normals: np.ndarray | None = None  # the sources' normals after the iterations (on request)
+    residual: np.ndarray = field(default_factory=lambda: np.zeros(0))  # convergence residual per iteration
+    iterations: int = 0                      # iterations run
 
     def tris_input_frame(self):
         return None if self.tris is None else self.tris * self.scale + self.center
@@ -45,6 +49,7 @@ def options(**kw) -> DwgOptions:
     L.dwg_options_default(ctypes.byref(o))
     kahan = kw.pop("kahan", None)
     bf16 = kw.pop("bf16_far", None)
+    near64 = kw.pop("near_fp64", None)
     lam = kw.pop("lam", None)
     if lam is not None:
         kw["sigma"] = lam
@@ -56,6 +61,8 @@ def options(**kw) -> DwgOptions:
         o.precision = (o.precision | DWG_KAHAN) if kahan else (o.precision & ~DWG_KAHAN)
     if bf16 is not None:
         o.precision = (o.precision | DWG_FAR_BF16) if bf16 else (o.precision & ~DWG_FAR_BF16)
+    if near64 is not None:
+        o.precision = (o.precision | DWG_NEAR_FP64) if near64 else (o.precision & ~DWG_NEAR_FP64)
     return o
 
 
@@ -71,7 +78,8 @@ def _host_array(L, ptr, ntri):
     return np.frombuffer(raw, np.float32).reshape(ntri, 3, 3)
 
 
-def reconstruct(ctx, points, normals=None, want_mesh=True, out_base=None, mlp=None, **kw) -> Reconstruction:
+def reconstruct(ctx, points, normals=None, want_mesh=True, out_base=None, mlp=None, want_source_normals=False,
+                want_residual=False, **kw) -> Reconstruction:
     """Run DWG on host points (N×3); ``ctx`` is a dwg_amd.gpu.Context."""
     L = _lib.load()
     P = np.ascontiguousarray(np.asarray(points, np.float32).reshape(-1, 3))
@@ -89,11 +97,18 @@ def reconstruct(ctx, points, normals=None, want_mesh=True, out_base=None, mlp=No
     it_z = np.zeros(iters, np.int64)
     r.w_mean, r.iter_tris, r.iter_zeros = wm.ctypes.data, it_t.ctypes.data, it_z.ctypes.data
     r.want_mesh = 1 if want_mesh else 0
+    src_n = np.zeros((n, 3), np.float32) if want_source_normals else None
+    r.source_normals = None if src_n is None else src_n.ctypes.data
+    resid = np.full(iters, np.nan) if want_residual else None
+    r.residual = None if resid is None else resid.ctypes.data
     ctx.check(L.dwg_launch(ctx.h, P.ctypes.data, None if nin is None else nin.ctypes.data, n, ctypes.byref(o),
                            None if out_base is None else out_base.encode(), ctypes.byref(r)))
     tris = None
     if want_mesh:
         tris = _host_array(L, r.mesh, int(r.tri_count))
+    k = int(r.iterations)
     return Reconstruction(out_n, tris, int(r.n_sources), np.array(r.center[:]), float(r.scale), r.time_setup_s,
-                          r.time_iter_s, r.time_final_s, r.time_total_s, wm[: o.total_iter], it_t[: o.total_iter],
-                          it_z[: o.total_iter], int(r.terms_src), int(r.terms_grid), int(r.grid_points))
+                          r.time_iter_s, r.time_final_s, r.time_total_s, wm[:k], it_t[:k], it_z[:k], int(r.terms_src),
+                          int(r.terms_grid), int(r.grid_points),
+                          None if src_n is None else src_n[: int(r.n_sources)],
+                          np.zeros(0) if resid is None else resid[: int(r.iterations)], int(r.iterations))

@@ -7,7 +7,10 @@ reconstruction itself receives positions only, as the reference does with
 unoriented input (--in_normal 3, DWGProcess.h:474-510).
 
 Randomness: numpy's PCG64 generator seeded with the workload seed; all
-sampling is area-uniform on the clean surface.
+sampling is area-uniform on the clean surface.  Rotations, dot products
+and norms are written out as elementwise products and sums in a fixed
+order (no BLAS matrix product, no einsum), so every inside / accept
+decision, and with it the sample, is the same on every host.
 
 Workloads:
   sphere(n=10_000, seed=1)            unit sphere, no noise
@@ -26,8 +29,22 @@ from concurrent.futures import ThreadPoolExecutor
 import numpy as np
 
 
+def _dot3(a, b):
+    """Row-wise a·b of n×3 arrays, summed x, y, z in that order."""
+    return a[..., 0] * b[..., 0] + a[..., 1] * b[..., 1] + a[..., 2] * b[..., 2]
+
+
+def _mat3(x, m):
+    """Rows of x times the 3×3 matrix m (x @ m), each entry a fixed-order sum."""
+    return np.stack([x[:, 0] * m[0, j] + x[:, 1] * m[1, j] + x[:, 2] * m[2, j] for j in range(3)], -1)
+
+
 def _unit(v):
-    return v / np.linalg.norm(v, axis=-1, keepdims=True)
+    """v over its length along the last axis (summed in index order)."""
+    ss = v[..., 0] * v[..., 0]
+    for k in range(1, v.shape[-1]):
+        ss = ss + v[..., k] * v[..., k]
+    return v / np.sqrt(ss)[..., None]
 
 
 def sphere(n=10_000, seed=1, radius=1.0, center=(0.0, 0.0, 0.0)):
@@ -57,7 +74,8 @@ def torus(n=100_000, seed=2, R=1.0, r=0.35, noise=0.002):
         have += int(acc.sum())
     p = np.concatenate(out_p)[:n]
     nn = np.concatenate(out_n)[:n]
-    diag = np.linalg.norm(p.max(0) - p.min(0))
+    e = p.max(0) - p.min(0)
+    diag = float(np.sqrt(_dot3(e, e)))
     p = p + rng.normal(0.0, noise * diag, p.shape)
     return p.astype(np.float32), nn.astype(np.float32)
 
@@ -90,21 +108,22 @@ class _Prim:
         return float(p[0])
 
     def to_local(self, x):
-        return (x - self.ctr) @ self.rot  # rot columns are the local axes
+        return _mat3(x - self.ctr, self.rot)  # rot columns are the local axes
 
     def inside(self, x):
         """Strictly inside the solid (points on or near the boundary are not)."""
         p = self.params
         if self.kind in (0, 3):  # radial: no rotation needed
             d = x - self.ctr
-            d2 = np.einsum("ij,ij->i", d, d)
+            d2 = _dot3(d, d)
             if self.kind == 0:
                 return d2 < (p[0] * (1 - 1e-6)) ** 2
             ro, t = p
             return (d2 < (ro * (1 - 1e-6)) ** 2) & (d2 > ((ro - t) * (1 + 1e-6)) ** 2)
         y = self.to_local(x)
         if self.kind == 1:
-            return ((y / p) ** 2).sum(1) < 1 - 1e-6
+            z = y / p
+            return _dot3(z, z) < 1 - 1e-6
         if self.kind == 2:
             R, r = p
             q = np.sqrt(y[:, 0] ** 2 + y[:, 1] ** 2) - R
@@ -149,7 +168,7 @@ class _Prim:
                 ns.append(np.stack([np.cos(th) * np.cos(ph), np.cos(th) * np.sin(ph), np.sin(th)], -1))
                 have += len(th)
             y, n = np.concatenate(ys)[:m], np.concatenate(ns)[:m]
-        return y @ self.rot.T + self.ctr, n @ self.rot.T
+        return _mat3(y, self.rot.T) + self.ctr, _mat3(n, self.rot.T)
 
 
 def _random_rotation(rng):
@@ -199,7 +218,8 @@ def union_of(prims, n, rng, noise, adaptive=False, progress=None, clean=False):
     # bounding balls overlap
     ctr = np.array([p.ctr for p in prims])
     rad = np.array([p.bound() for p in prims])
-    gap = np.linalg.norm(ctr[:, None] - ctr[None], axis=2) - (rad[:, None] + rad[None])
+    dc = ctr[:, None] - ctr[None]
+    gap = np.sqrt(_dot3(dc, dc)) - (rad[:, None] + rad[None])
     near = [[j for j in np.nonzero(gap[i] < 1e-9)[0] if j != i] for i in range(len(prims))]
     pts, nrms, have, drawn = [], [], 0, 0
     while have < n:
@@ -236,7 +256,8 @@ def union_of(prims, n, rng, noise, adaptive=False, progress=None, clean=False):
     nn = np.concatenate(nrms)
     perm = rng.permutation(len(p))[:n]
     p, nn = p[perm], nn[perm]
-    diag = np.linalg.norm(p.max(0) - p.min(0))
+    e = p.max(0) - p.min(0)
+    diag = float(np.sqrt(_dot3(e, e)))
     p0 = p
     p = p + rng.normal(0.0, noise * diag, p.shape)
     if clean:
@@ -262,7 +283,8 @@ def large(n=5_000_000, seed=5, m=128, shells=16, hole_frac=0.05, hole_radius=0.0
     keep = np.ones(len(p), bool)
     while keep.sum() > n:
         c = p[rng.integers(0, len(p))]
-        keep &= np.einsum("ij,ij->i", p - c, p - c) > hole_radius ** 2
+        d = p - c
+        keep &= _dot3(d, d) > hole_radius ** 2
     idx = np.nonzero(keep)[0][:n]
     if clean:
         return p[idx], nn[idx], p0[idx]

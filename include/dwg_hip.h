@@ -35,6 +35,8 @@ extern "C" {
 #define DWG_USE_RADII 2u  /* screening radius per source; else 0.0015 for every source */
 #define DWG_KAHAN 4u      /* compensated (Kahan) accumulation of per-tile partial sums */
 #define DWG_FAR_BF16 8u   /* far-field dipole moments stored and read as bf16 (tree only) */
+#define DWG_NEAR_FP64 16u /* screened point terms inside the source's screening radius (r < s)
+                             evaluated and summed in FP64 (new; direct and tree sums) */
 
 typedef struct dwg_ctx dwg_ctx;
 typedef struct dwg_octree dwg_octree;
@@ -70,7 +72,9 @@ int dwg_ctx_counter_get(dwg_ctx* ctx, const char* name, int64_t* value);
  * (fast_winding_number.cpp:313-323) and, with DWG_SCREENED, the per-point
  * term of ScreenedGWNLibiPStoQ (TurboLibiGWN.cu:544-585).
  * d_radii may be NULL unless DWG_USE_RADII.  d_out_grad (nullable, nq×3)
- * receives ∇_q w of the plain kernel (new; plain kernel only). */
+ * receives ∇_q w (new; the reference has no gradient), of the plain or the
+ * screened term; a screened term inside its screening radius (ρ = s) has
+ * ∇w = −g(s)·A·n. */
 int dwg_gwn_direct(dwg_ctx* ctx, const float* d_pts, const float* d_nrm, const float* d_area,
                    const float* d_radii, int64_t n, const float* d_q, int64_t nq, float sigma,
                    uint32_t flags, float* d_out_w, float* d_out_grad);
@@ -107,16 +111,39 @@ void dwg_octree_destroy(dwg_octree* tree);
  * than min_num ≥ 0 points, in breadth-first order; the count in *out_count. */
 int dwg_octree_leaves(dwg_ctx* ctx, const dwg_octree* tree, int min_num, double* d_centres, double* d_widths,
                       int64_t* out_count);
+/* SmoothNormalstoOctreeLeaves + .normalized() (DWGProcess.h:430-472, 537):
+ * for each leaf of dwg_octree_leaves' selection (same order), component k
+ * of its normal is Σ_j exp(−((P_jk − C_k)/W)²/4)·N_jk over the leaf's
+ * points j (C its centre, W its width); the whole n_leaves×3 matrix is then
+ * divided by its Frobenius norm (rows are not unit length).  d_pts, d_nrm:
+ * the tree's points and their normals, device FP64 n×3, caller order;
+ * d_out: device floats, room for n_leaves×3. */
+int dwg_octree_leaf_normals(dwg_ctx* ctx, const dwg_octree* tree, int min_num, const double* d_pts,
+                            const double* d_nrm, float* d_out, int64_t* out_count);
 int dwg_octree_info(const dwg_octree* tree, int64_t* n_nodes, int64_t* n_leaves, int* max_level);
 /* Per-iteration refresh of the far-field moment EC = Σ A·n per node and of
  * the packed sources (TurboLibiGWN::UpdateMassNormals, TurboLibiGWN.cu:1051-1059).
  * d_radii may be NULL. */
 int dwg_octree_update(dwg_ctx* ctx, dwg_octree* tree, const float* d_nrm, const float* d_area,
                       const float* d_radii);
+/* Re-packs the per-point terms (A·n, A) with other areas (d_area NULL: 2e-6)
+ * and keeps the far-field moments EC of the last dwg_octree_update.  The
+ * reference's sums take the point areas apart from the node moments
+ * (TurboLibiScreenedGWNLaunch's A argument, TurboLibiGWN.cu:912-922);
+ * GenerateMesh filters the areas after the moments (TurboDWG.cu:870-875). */
+int dwg_octree_set_point_areas(dwg_ctx* ctx, dwg_octree* tree, const float* d_nrm, const float* d_area);
 /* Barnes-Hut sums at nq query points (TurboLibiGWN::TurboLibiScreenedGWNLaunch,
  * TurboLibiGWN.cu:912-922; plain: TurboLibiGWNLaunch :900-910). */
 int dwg_gwn_tree(dwg_ctx* ctx, const dwg_octree* tree, const float* d_q, int64_t nq, float beta,
                  float sigma, uint32_t flags, float* d_out);
+/* Value and gradient ∇_q w of the Barnes-Hut sums at nq query points (new;
+ * the reference approximates the gradient direction by marching-cubes face
+ * normals): the term set of dwg_gwn_tree's FP32 walk, each term with its
+ * exact gradient (far nodes: dipole EC at CM, s = 0.0015).  Flags:
+ * DWG_SCREENED; the radii are those of the last dwg_octree_update.
+ * d_out_grad (nq×3) may be NULL. */
+int dwg_gwn_tree_grad(dwg_ctx* ctx, const dwg_octree* tree, const float* d_q, int64_t nq, float beta,
+                      float sigma, uint32_t flags, float* d_out_w, float* d_out_grad);
 /* The same sums at the octree's own points, evaluated in tree order (one
  * wave per 64 consecutive tree slots) and written in the caller's point order
  * (the source self-evaluation of TurboScreenedDWGLaunch, TurboDWG.cu:536). */
@@ -191,6 +218,11 @@ int dwg_transfer_normals(dwg_ctx* ctx, const dwg_knn_index* src_index, const flo
 /* Divide the largest ceil(n·threshold) areas by 1000 (OutlierFilter,
  * TurboDWG.cu:991-1009). */
 int dwg_outlier_filter(dwg_ctx* ctx, float* d_area, int64_t n, float threshold);
+/* Convergence residual of a diffusion step (new; the reference runs a fixed
+ * iteration count, TurboDWG.cu:516): mean over the n sources of
+ * 1 − |cos ∠(old_i, new_i)|, reduced in FP64 on the GPU; the absolute value
+ * ignores the global sign flip of each iteration; a zero normal counts 1. */
+int dwg_normal_change(dwg_ctx* ctx, const float* d_old, const float* d_new, int64_t n, double* h_residual);
 /* FP64 sum of a float array (the reference's thrust::reduce, TurboDWG.cu:541). */
 int dwg_reduce_sum(dwg_ctx* ctx, const float* d_vals, int64_t n, double* h_sum);
 /* Axis-aligned bounds of a point array (cukd computeBounds, builder_common.h:76-150). */
@@ -267,19 +299,22 @@ typedef struct dwg_options {
     int32_t source_mode;         /* 0: octree-leaf centres as the reference, 1: the input points */
     int32_t leaf_size;           /* octree leaf capacity for the sums (1 = octreeDWG) */
     float beta;                  /* Barnes-Hut accuracy, 2.3 (DWGProcess.h:689); ≤ 0: direct sums */
-    uint32_t precision;          /* DWG_KAHAN | DWG_FAR_BF16 */
+    uint32_t precision;          /* DWG_KAHAN | DWG_FAR_BF16 | DWG_NEAR_FP64 */
     uint64_t seed;               /* random initial normals */
     int32_t normalize_input;     /* 1: centre and scale as ReadPointandNormals (DWGProcess.h:53-97) */
     int32_t verbose;             /* progress lines on stdout */
     int32_t count_terms;         /* 1: count the Barnes-Hut terms of one iteration (untimed) */
     const dwg_mlp* mlp;          /* weights for input_normal 4 */
     int32_t sigma_warm_iters;    /* first iterations at σ = 20 (TurboDWG.cu:526-528), 4 */
+    float residual_tol;          /* > 0: stop after an even iteration count whose convergence residual
+                                    (dwg_normal_change of that iteration) is below it; 0: total_iter */
 } dwg_options;
 
 void dwg_options_default(dwg_options* opt);
 
 typedef struct dwg_result {
-    float* normals;        /* in: n×3 buffer or NULL; out: final point normals (normalised frame) */
+    float* normals;        /* in: n×3 buffer or NULL; out: final point normals (normalised frame);
+                              with use_leaves_generate 1 only when the sources are the points */
     int32_t want_mesh;     /* in: 1 to receive the final mesh in `mesh` */
     float* mesh;           /* out: tri_count×9 floats (normalised frame) in pinned host
                               memory allocated by the library; release with dwg_free() */
@@ -297,6 +332,10 @@ typedef struct dwg_result {
     int64_t terms_src;     /* out (count_terms): terms of one source evaluation */
     int64_t terms_grid;    /* out (count_terms): terms of one grid evaluation */
     int64_t grid_points;   /* out: points of the iteration grid */
+    float* source_normals; /* in: n×3 buffer or NULL; out: the n_sources×3 source normals after the
+                              iterations, before the final pass (normalised frame) */
+    double* residual;      /* in: total_iter buffer or NULL; out: convergence residual per iteration */
+    int32_t iterations;    /* out: iterations run (total_iter unless residual_tol stopped earlier) */
 } dwg_result;
 
 /* Full reconstruction of n host points (DWGLaunch → DataprePointCloud →

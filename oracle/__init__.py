@@ -61,7 +61,8 @@ def lib():
     L.orc_tree_eval.argtypes = [_vp, _f64p, _f64p, _vp, _f64p, _i64, _dbl, _int, _dbl, _int, _int,
                                 _f64p, _vp]
     L.orc_direct.argtypes = [_f64p, _f64p, _f64p, _vp, _i64, _f64p, _i64, _int, _dbl, _int, _f64p]
-    L.orc_direct_grad.argtypes = [_f64p, _f64p, _f64p, _i64, _f64p, _i64, _f64p, _f64p]
+    L.orc_direct_grad.argtypes = [_f64p, _f64p, _f64p, _vp, _i64, _f64p, _i64, _int, _dbl, _int, _f64p, _f64p]
+    L.orc_tree_eval_grad.argtypes = [_vp, _f64p, _f64p, _vp, _f64p, _i64, _dbl, _int, _dbl, _int, _f64p, _f64p]
     L.orc_voxel_grid.argtypes = [_f64p, _i64, _dbl, _int, _f64p, ctypes.POINTER(_dbl), _i32p]
     L.orc_mc.argtypes = [_f64p, _i32p, _f64p, _dbl, _dbl, _i64, _vp, _vp, _vp]
     L.orc_mc.restype = _i64
@@ -143,15 +144,32 @@ class Octree:
         return cm, r, ec
 
     def eval(self, Q, beta, kind="screened", sigma=10.0, radii=None, float_decisions=True,
-             count_terms=False):
+             count_terms=False, point_areas=None):
+        """Barnes-Hut sums at Q.  point_areas (optional) replaces the areas of
+        the point terms only; far nodes keep the moments of the last
+        precompute/refresh, as the reference's launch takes the areas apart
+        from the node moments (TurboLibiGWN.cu:912-922, TurboDWG.cu:870-875)."""
         Q = _c(Q, np.float64).reshape(-1, 3)
         out = np.zeros(len(Q))
         terms = np.zeros(len(Q), np.int64) if count_terms else None
         rad = None if radii is None else _c(radii, np.float64)
-        lib().orc_tree_eval(self.h, self.N, self.A, _ptr(rad), Q, len(Q), float(beta),
+        A = self.A if point_areas is None else _c(point_areas, np.float64).reshape(-1)
+        lib().orc_tree_eval(self.h, self.N, A, _ptr(rad), Q, len(Q), float(beta),
                             0 if kind == "plain" else 1, float(sigma), 1 if rad is not None else 0,
                             1 if float_decisions else 0, out, _ptr(terms))
         return (out, terms) if count_terms else out
+
+    def eval_grad(self, Q, beta, kind="screened", sigma=10.0, radii=None):
+        """Barnes-Hut value and gradient: the term set of eval (float far
+        decisions), each term with its exact gradient (no reference
+        counterpart)."""
+        Q = _c(Q, np.float64).reshape(-1, 3)
+        w = np.zeros(len(Q))
+        g = np.zeros((len(Q), 3))
+        rad = None if radii is None else _c(radii, np.float64)
+        lib().orc_tree_eval_grad(self.h, self.N, self.A, _ptr(rad), Q, len(Q), float(beta),
+                                 0 if kind == "plain" else 1, float(sigma), 1 if rad is not None else 0, w, g)
+        return w, g
 
 
 def direct(P, N, A, Q, kind="plain", sigma=10.0, radii=None):
@@ -167,15 +185,18 @@ def direct(P, N, A, Q, kind="plain", sigma=10.0, radii=None):
     return out
 
 
-def direct_grad(P, N, A, Q):
-    """FP64 plain sum and its exact gradient with respect to the query point."""
+def direct_grad(P, N, A, Q, kind="plain", sigma=10.0, radii=None):
+    """FP64 all-pairs sum and its exact gradient with respect to the query
+    point, plain or screened term (no reference counterpart)."""
     P = _c(P, np.float64).reshape(-1, 3)
     N = _c(N, np.float64).reshape(-1, 3)
     A = _c(A, np.float64).reshape(-1)
     Q = _c(Q, np.float64).reshape(-1, 3)
+    rad = None if radii is None else _c(radii, np.float64)
     w = np.zeros(len(Q))
     g = np.zeros((len(Q), 3))
-    lib().orc_direct_grad(P, N, A, len(P), Q, len(Q), w, g)
+    lib().orc_direct_grad(P, N, A, _ptr(rad), len(P), Q, len(Q), 0 if kind == "plain" else 1, float(sigma),
+                          1 if rad is not None else 0, w, g)
     return w, g
 
 
@@ -323,6 +344,33 @@ def point_areas(P, I, N=None):
     return A
 
 
+def outlier_filter(A, threshold):
+    """OutlierFilter (TurboDWG.cu:991-1009): sort the areas in descending order
+    (thrust::sort_by_key with thrust::greater, a stable radix sort: equal
+    areas keep ascending index order) and divide the first
+    ceil(float(n)·threshold) by 1000 in float (AsFiltertoZero, :442-454)."""
+    A = np.array(A, dtype=np.float32, copy=True).reshape(-1)
+    m = int(np.ceil(np.float32(len(A)) * np.float32(threshold)))
+    if m <= 0:
+        return A
+    order = np.argsort(-A, kind="stable")
+    A[order[:m]] = A[order[:m]] / np.float32(1000.0)
+    return A
+
+
+def normal_change(old, new):
+    """Convergence residual of a diffusion step (new; no reference
+    counterpart): mean over the sources of 1 − |cos ∠(old_i, new_i)|, a zero
+    normal counting 1, in FP64."""
+    a = np.asarray(old, np.float64).reshape(-1, 3)
+    b = np.asarray(new, np.float64).reshape(-1, 3)
+    la, lb = (a * a).sum(1), (b * b).sum(1)
+    ok = (la > 0) & (lb > 0)
+    c = np.ones(len(a))
+    c[ok] = 1.0 - np.abs((a[ok] * b[ok]).sum(1)) / np.sqrt(la[ok] * lb[ok])
+    return float(c.mean()) if len(a) else 0.0
+
+
 def sigma_schedule(it, sigma_in):
     return float(lib().orc_sigma_schedule(int(it), float(sigma_in)))
 
@@ -345,10 +393,10 @@ def dwg_iterate(P, N0, A, res, origin, h, iters, sigma=10.0, beta=2.3, max_depth
     return N, wm, nt, nz
 
 
-def octree_leaves(P, max_depth=10, min_num=0):
+def octree_leaves(P, max_depth=10, min_num=0, widths=False):
     """GetOctreeLeaves (DWGProcess.h:383-428): breadth-first over octreeDWG,
     the centres of leaf cells holding more than min_num points.
-    Returns (centres [m,3], point index lists)."""
+    Returns (centres [m,3], point index lists[, cell widths [m]])."""
     from collections import deque
     P = _c(P, np.float64).reshape(-1, 3)
     t = Octree(P, max_depth, 1)
@@ -356,7 +404,7 @@ def octree_leaves(P, max_depth=10, min_num=0):
     lo, hi = P.min(0), P.max(0)
     centres = {0: ((lo + hi) / 2, (hi - lo).max())}
     q = deque([0])
-    out, pts = [], []
+    out, pts, wids = [], [], []
     while q:
         i = q.popleft()
         c, w = centres[i]
@@ -364,6 +412,7 @@ def octree_leaves(P, max_depth=10, min_num=0):
             if cnt[i] > min_num:
                 out.append(c)
                 pts.append(t.node_points(i))
+                wids.append(w)
             continue
         for k in range(8):
             kid = int(ch[i, k])
@@ -371,4 +420,23 @@ def octree_leaves(P, max_depth=10, min_num=0):
             centres[kid] = (c + sign * w / 4, w / 2)
             if cnt[kid] > 0:
                 q.append(kid)
+    if widths:
+        return np.array(out), pts, np.array(wids)
     return np.array(out), pts
+
+
+def leaf_normals(P, N, max_depth=10, min_num=0):
+    """SmoothNormalstoOctreeLeaves (DWGProcess.h:430-472): component k of a
+    leaf's normal is Σ_j exp(−((P_jk − C_k)/W)²/4)·N_jk over the leaf's
+    points in octree order (C the leaf centre, W its width, octree.cpp
+    CN/W); then .normalized() (DWGProcess.h:537) divides the whole matrix by
+    its Frobenius norm."""
+    P = _c(P, np.float64).reshape(-1, 3)
+    N = _c(N, np.float64).reshape(-1, 3)
+    C, pts, W = octree_leaves(P, max_depth, min_num, widths=True)
+    out = np.zeros((len(C), 3))
+    for i, (c, idx, w) in enumerate(zip(C, pts, W)):
+        x = (P[idx] - c) / w
+        out[i] = (np.exp(-x * x / 4.0) * N[idx]).sum(0)
+    f = np.sqrt((out * out).sum())
+    return out / f if f > 0 else out

@@ -15,7 +15,8 @@
 //                         src/TurboLibiGWN.cu:513-812 (screened kernel semantics)
 //   orc_direct            fast_winding_number.cpp:313-323 (β≤0, double accumulator)
 //                         src/TurboLibiGWN.cu:544-585 (screened term)
-//   orc_direct_grad       NEW (no reference counterpart): exact ∇_q of the plain term
+//   orc_direct_grad       NEW (no reference counterpart): exact ∇_q of the plain and screened terms
+//   orc_tree_eval_grad    NEW: the Barnes-Hut term set of orc_tree_eval with each term's gradient
 //   orc_voxel_grid        deps/libigl/include/igl/voxel_grid.cpp:15-87, grid.cpp:14-45
 //   orc_mc_*              src/TurboMC.cu:373-478 (corners, lerp, table, negated normal)
 //   orc_knn_brute         cudaKDTree/cukd/knn.h:277-292, 403-414; math.h:210-211
@@ -369,22 +370,122 @@ void orc_direct(const double* P, const double* N, const double* A, const double*
     });
 }
 
-// ∇_q of the plain regularised sum w(q) = Σ A(d·n)/(4π r³ + ε), d = p − q:
-//   ∂(d·n)/∂q = −n,  ∂r³/∂q = −3 r d,  so
-//   ∇w = Σ A [ −n / D + 12π r (d·n) d / D² ],  D = 4π r³ + ε.
-void orc_direct_grad(const double* P, const double* N, const double* A, int64_t n,
-                     const double* Q, int64_t nq, double* out_w, double* out_g) {
+}  // extern "C"
+
+namespace {
+
+// Value and ∇_q of one term (new; the reference has no gradient), FP64,
+// d = p − q, an = A·n:
+//   plain     w = (d·an)/D, D = 4π r³ + 1e-8:  ∇w = −an/D + 12π r (d·an) d / D²
+//             (∂(d·an)/∂q = −an, ∂r³/∂q = −3 r d)
+//   screened  w = g(ρ)(d·an), g = (σρ + 1)/D, D = e^{σρ} 4π ρ³ + 1e-7,
+//             ρ = max(s, r), skipped when r ≤ 1e-10:
+//             ∇w = −g an − (d·an) g'(ρ) d / r for r ≥ s (∂ρ/∂q = −d/r), −g an for r < s;
+//             g' = (σ D − (σρ + 1) e^{σρ} 4π (σρ³ + 3ρ²)) / D²
+inline void term_grad(int kind, const double d[3], const double an[3], double s, double sigma, double& w,
+                      double g[3]) {
+    const double r = std::sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+    const double dot = d[0] * an[0] + d[1] * an[1] + d[2] * an[2];
+    if (kind == 0) {
+        const double D = 4.0 * kPi * r * r * r + 1e-8;
+        w = dot / D;
+        for (int k = 0; k < 3; ++k) g[k] = -an[k] / D + 12.0 * kPi * r * dot * d[k] / (D * D);
+        return;
+    }
+    w = 0;
+    g[0] = g[1] = g[2] = 0;
+    if (!(r > 1e-10)) return;
+    const double rho = s > r ? s : r;
+    const double E = std::exp(sigma * rho);
+    const double D = E * 4.0 * kPi * rho * rho * rho + 1e-7;
+    const double num = sigma * rho + 1.0;
+    const double gv = num / D;
+    w = gv * dot;
+    const double gp = (sigma * D - num * E * 4.0 * kPi * (sigma * rho * rho * rho + 3.0 * rho * rho)) / (D * D);
+    const double c = (s > r) ? 0.0 : -dot * gp / r;
+    for (int k = 0; k < 3; ++k) g[k] = -gv * an[k] + c * d[k];
+}
+
+}  // namespace
+
+extern "C" {
+
+// FP64 all-pairs value and gradient; kind 0 plain, 1 screened (radii[j] when
+// use_radii, else 0.0015).
+void orc_direct_grad(const double* P, const double* N, const double* A, const double* radii, int64_t n,
+                     const double* Q, int64_t nq, int kind, double sigma, int use_radii, double* out_w,
+                     double* out_g) {
     parallel_for(nq, [&](int64_t b, int64_t e) {
         for (int64_t qi = b; qi < e; ++qi) {
             double wn = 0, g[3] = {0, 0, 0};
             for (int64_t j = 0; j < n; ++j) {
-                double d[3] = {P[3 * j] - Q[3 * qi], P[3 * j + 1] - Q[3 * qi + 1], P[3 * j + 2] - Q[3 * qi + 2]};
-                double an[3] = {A[j] * N[3 * j], A[j] * N[3 * j + 1], A[j] * N[3 * j + 2]};
-                double r = std::sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
-                double D = 4.0 * kPi * r * r * r + 1e-8;
-                double dot = d[0] * an[0] + d[1] * an[1] + d[2] * an[2];
-                wn += dot / D;
-                for (int k = 0; k < 3; ++k) g[k] += -an[k] / D + 12.0 * kPi * r * dot * d[k] / (D * D);
+                const double d[3] = {P[3 * j] - Q[3 * qi], P[3 * j + 1] - Q[3 * qi + 1], P[3 * j + 2] - Q[3 * qi + 2]};
+                const double an[3] = {A[j] * N[3 * j], A[j] * N[3 * j + 1], A[j] * N[3 * j + 2]};
+                double w, gj[3];
+                term_grad(kind, d, an, use_radii ? radii[j] : 0.0015, sigma, w, gj);
+                wn += w;
+                for (int k = 0; k < 3; ++k) g[k] += gj[k];
+            }
+            out_w[qi] = wn;
+            for (int k = 0; k < 3; ++k) out_g[3 * qi + k] = g[k];
+        }
+    });
+}
+
+// Barnes-Hut value and gradient: the term set of orc_tree_eval (float far
+// decisions), each term with its gradient; far nodes are dipoles at CM with
+// moment EC and s = 0.0015.
+void orc_tree_eval_grad(void* h, const double* N, const double* A, const double* radii, const double* Q,
+                        int64_t nq, double beta, int kind, double sigma, int use_radii, double* out_w,
+                        double* out_g) {
+    auto* t = static_cast<Tree*>(h);
+    const double* P = t->P;
+    const float beta_f = float(beta);
+    parallel_for(nq, [&](int64_t b, int64_t e) {
+        std::vector<int> near, next;
+        for (int64_t qi = b; qi < e; ++qi) {
+            const double q[3] = {Q[3 * qi], Q[3 * qi + 1], Q[3 * qi + 2]};
+            const float qf[3] = {float(q[0]), float(q[1]), float(q[2])};
+            double wn = 0, g[3] = {0, 0, 0};
+            auto add = [&](const double d[3], const double an[3], double s) {
+                double w, gj[3];
+                term_grad(kind, d, an, s, sigma, w, gj);
+                wn += w;
+                for (int k = 0; k < 3; ++k) g[k] += gj[k];
+            };
+            auto point_sum = [&](int node) {
+                for (int p : t->pi[node]) {
+                    const double d[3] = {P[3 * p] - q[0], P[3 * p + 1] - q[1], P[3 * p + 2] - q[2]};
+                    const double an[3] = {A[p] * N[3 * p], A[p] * N[3 * p + 1], A[p] * N[3 * p + 2]};
+                    add(d, an, use_radii ? radii[p] : 0.0015);
+                }
+            };
+            near.assign(1, 0);
+            while (!near.empty()) {
+                next.clear();
+                for (int node : near) {
+                    if (t->ch[node][0] == -1) {
+                        point_sum(node);
+                        continue;
+                    }
+                    for (int k = 0; k < 8; ++k) {
+                        const int c = t->ch[node][k];
+                        if (t->pi[c].empty()) continue;
+                        const float dx = t->cm_f[c][0] - qf[0], dy = t->cm_f[c][1] - qf[1], dz = t->cm_f[c][2] - qf[2];
+                        const float br = beta_f * t->r_f[c];
+                        if (fdist2(dx, dy, dz) > br * br) {
+                            if (t->ch[c][0] == -1) {
+                                point_sum(c);
+                            } else {
+                                const double d[3] = {t->cm[c][0] - q[0], t->cm[c][1] - q[1], t->cm[c][2] - q[2]};
+                                add(d, t->ec[c].data(), 0.0015);
+                            }
+                        } else {
+                            next.push_back(c);
+                        }
+                    }
+                }
+                std::swap(near, next);
             }
             out_w[qi] = wn;
             for (int k = 0; k < 3; ++k) out_g[3 * qi + k] = g[k];

@@ -92,3 +92,53 @@ def test_periodic_dumps(tmp_path):
         (count,) = struct.unpack("<I", b[80:84])
         assert count > 100 and len(b) == 84 + 50 * count
     assert (tmp_path / "s_md_10_gr_48_iter_5.stl").exists()
+
+
+def _xyz_normals(path):
+    X = np.loadtxt(path)
+    return X[:, :3], X[:, 3:] / np.linalg.norm(X[:, 3:], axis=1, keepdims=True)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("flip", [False, True])
+def test_input_normals_start(tmp_path, flip):
+    """--in_normal 1: the input normals start the iterations (read from the
+    .xyz file's last three columns, DWGProcess.h:53-97, 494-509); after an
+    even number of iterations the result keeps the input's side, so flipped
+    input normals give inward normals (the Gauss-map start would give outward
+    ones).  The input points are the sources here; the leaf smoothing of
+    octree-leaf sources has its own parity test against the oracle."""
+    P, N = shapes.sphere(40_000, 43)
+    _write_xyz(tmp_path / "in.xyz", 0.5 * P, -N if flip else N)
+    r = _run(["--in_path", str(tmp_path), "--in_name", "in.xyz", "--out_path", str(tmp_path), "--output_mode", "2",
+              "--in_normal", "1", "--grid_reso", "96", "--final_reso", "96", "--total_iter", "6",
+              "--source_mode", "1"])
+    assert r.returncode == 0, r.stderr
+    X, Nout = _xyz_normals(tmp_path / "in_md_10_gr_96_iter_6.xyz")
+    cos = np.einsum("ij,ij->i", Nout, N)
+    print(f"input normals (flip={flip}): median cos to the true normals {np.median(cos):.4f}")
+    assert np.median(-cos if flip else cos) > 0.99
+
+
+@pytest.mark.gpu
+def test_random_start_is_seeded_and_converges(tmp_path):
+    """--in_normal 0: seeded random normals (GetRandomNs, DWGProcess.h:264-281;
+    the reference seeds from std::random_device) start the iterations; the
+    diffusion turns them into a consistent field, and the same seed gives the
+    same output."""
+    P, N = shapes.sphere(40_000, 44)
+    _write_xyz(tmp_path / "r.xyz", 0.5 * P, N)
+    outs = []
+    for run in range(2):
+        d = tmp_path / f"out{run}"
+        d.mkdir()
+        r = _run(["--in_path", str(tmp_path), "--in_name", "r.xyz", "--out_path", str(d), "--output_mode", "2",
+                  "--in_normal", "0", "--grid_reso", "96", "--final_reso", "96", "--total_iter", "30",
+                  "--source_mode", "1"])
+        assert r.returncode == 0, r.stderr
+        outs.append(_xyz_normals(d / "r_md_10_gr_96_iter_30.xyz")[1])
+    assert np.array_equal(outs[0], outs[1])
+    cos = np.einsum("ij,ij->i", outs[0], N)
+    print(f"random start: median |cos| {np.median(np.abs(cos)):.4f}, outward share {np.mean(cos > 0):.3f}")
+    assert np.median(np.abs(cos)) > 0.95
+    assert max(np.mean(cos > 0), np.mean(cos < 0)) > 0.9  # one consistent side

@@ -105,6 +105,45 @@ def test_gradient_matches_finite_differences():
     assert np.allclose(g, fd, rtol=1e-5, atol=1e-6)
 
 
+def test_screened_gradient_matches_finite_differences():
+    # ρ = r for every pair (queries away from the sphere), and a single
+    # dipole seen from inside its screening radius (ρ = s: only −g·A·n)
+    P, N, A = _sphere(500, 3)
+    Q = np.array([[0.1, 0.2, -0.3], [1.4, 0.2, 0.1], [0.0, 0.97, 0.0]])
+    R = np.random.default_rng(2).uniform(0.0015, 0.015, len(P))
+    h = 1e-6
+    for sigma in [0.0, 10.0, 20.0]:
+        for radii in [None, R]:
+            kw = dict(kind="screened", sigma=sigma, radii=radii)
+            w, g = O.direct_grad(P, N, A, Q, **kw)
+            assert np.allclose(w, O.direct(P, N, A, Q, **kw), rtol=1e-13)
+            fd = np.stack([(O.direct(P, N, A, Q + h * e, **kw) - O.direct(P, N, A, Q - h * e, **kw)) / (2 * h)
+                           for e in np.eye(3)], -1)
+            assert np.allclose(g, fd, rtol=1e-5, atol=1e-6)
+    p, n, a = np.array([[0.3, 0.1, 0.2]]), np.array([[0.0, 0.6, 0.8]]), np.array([1e-3])
+    q = p + np.array([[4e-4, -3e-4, 5e-4]])  # r ≈ 7.1e-4 < s = 0.0015
+    w, g = O.direct_grad(p, n, a, q, kind="screened", sigma=10.0)
+    fd = np.stack([(O.direct(p, n, a, q + 1e-7 * e, kind="screened") - O.direct(p, n, a, q - 1e-7 * e,
+                                                                                  kind="screened")) / 2e-7
+                   for e in np.eye(3)], -1)
+    assert np.allclose(g, fd, rtol=1e-6)
+    D = np.exp(10 * 0.0015) * 4 * np.pi * 0.0015 ** 3 + 1e-7
+    assert np.allclose(g, -(1 + 10 * 0.0015) / D * a[0] * n, rtol=1e-12)
+
+
+def test_tree_gradient_infinite_beta_equals_direct():
+    P, N, A = _sphere(1500, 4)
+    Q = np.random.default_rng(5).uniform(-1.3, 1.3, (40, 3))
+    t = O.Octree(P, 10, 1)
+    t.precompute(N, A)
+    for kind in ["plain", "screened"]:
+        w, g = t.eval_grad(Q, 1e30, kind=kind)
+        wd, gd = O.direct_grad(P, N, A, Q, kind=kind)
+        assert np.allclose(w, wd, rtol=1e-12, atol=1e-13) and np.allclose(g, gd, rtol=1e-11, atol=1e-12)
+        # at a finite β the values are eval's: the same term set
+        assert np.allclose(t.eval_grad(Q, 2.3, kind=kind)[0], t.eval(Q, 2.3, kind=kind), rtol=1e-13, atol=1e-15)
+
+
 def test_tree_infinite_beta_equals_direct():
     P, N, A = _sphere(3000, 4)
     Q = np.random.default_rng(0).uniform(-1.3, 1.3, (50, 3))
@@ -420,3 +459,114 @@ def test_point_areas_random_plane():
     A = O.point_areas(P, idx[inner])
     assert np.allclose(A[full], G[inner][full], rtol=1e-9)
     assert 0.75 < full.mean() < 0.95
+
+
+def _mc_case_field(c):
+    # corner i of a cube sits at (x + (b0 != b1), y + b1, z + (b2 ? 0 : 1))
+    # (TurboMC.cu:401-406); bit i of the case set <=> corner i above the iso
+    f = np.zeros(8)
+    for i in range(8):
+        x, y, z = int((i & 1) != ((i >> 1) & 1)), (i >> 1) & 1, 0 if i & 4 else 1
+        f[x + 2 * (y + 2 * z)] = 1.0 if (c >> i) & 1 else -1.0
+    return f
+
+
+def _edge_orientation(f, tri, n):
+    # every vertex of a ±1 cube lies at an edge midpoint: Σ over the three
+    # vertices of (above corner − below corner)·n, positive iff the face
+    # normal points towards increasing values
+    s = 0.0
+    for v in tri:
+        fr = np.abs(v - np.round(v))
+        a = int(np.argmax(fr))
+        assert abs(fr[a] - 0.5) < 1e-9 and np.sum(fr > 1e-9) == 1
+        lo, hi = v.copy(), v.copy()
+        lo[a] -= 0.5
+        hi[a] += 0.5
+        val = lambda p: f[int(round(p[0])) + 2 * (int(round(p[1])) + 2 * int(round(p[2])))]
+        above, below = (hi, lo) if val(hi) > val(lo) else (lo, hi)
+        s += float(np.dot(above - below, n))
+    return s
+
+
+def test_mc_every_case_oriented():
+    # All 256 cube configurations one by one, independently of the shared
+    # table: cases 0 and 255 give nothing, every other case gives triangles
+    # whose face normals point towards increasing values (TurboMC.cu:57-314,
+    # 401-470), so a row with a reversed winding fails here.
+    res = np.array([2, 2, 2], np.int32)
+    total = 0
+    for c in range(256):
+        f = _mc_case_field(c)
+        tris, cent, nrm = O.marching_cubes(f, res, np.zeros(3), 1.0, 0.0)
+        assert (len(tris) == 0) == (c in (0, 255)), c
+        for t, n in zip(tris, nrm):
+            assert _edge_orientation(f, t, n) > 0, c
+        total += len(tris)
+    assert total == 820  # the triangle count of the classic table over all cases
+
+
+def test_outlier_filter_known_answers():
+    # ceil(n·threshold) largest areas divided by 1000; equal areas taken in
+    # ascending index order (TurboDWG.cu:991-1009)
+    A = np.array([3, 1, 5, 5, 2, 5, 0, 4], np.float32)
+    out = O.outlier_filter(A, 0.25)  # ceil(2.0) = 2: the first two 5s
+    assert np.array_equal(out, np.array([3, 1, 5e-3, 5e-3, 2, 5, 0, 4], np.float32))
+    out = O.outlier_filter(A, 0.3)  # ceil(2.4) = 3: all three 5s
+    assert np.array_equal(out, np.array([3, 1, 5e-3, 5e-3, 2, 5e-3, 0, 4], np.float32))
+    out = O.outlier_filter(A, 0.5)  # 4: the 5s and the 4
+    assert np.array_equal(out, np.array([3, 1, 5e-3, 5e-3, 2, 5e-3, 0, 4e-3], np.float32))
+    assert np.array_equal(O.outlier_filter(A, 0.0), A)
+    assert np.array_equal(O.outlier_filter(A, 1.0), A / np.float32(1000))
+
+
+def test_tree_point_areas_override():
+    # point terms take the override, far nodes the precomputed moments: with
+    # β → ∞ every term is a point term, so the sum is the direct sum with the
+    # override; with β = 2.3 and the override equal to the areas, nothing moves
+    P, N, A = _sphere(2000, 8)
+    Q = np.random.default_rng(3).uniform(-1.3, 1.3, (40, 3))
+    t = O.Octree(P, 10, 1)
+    t.precompute(N, A)
+    A2 = A * np.random.default_rng(4).uniform(0.5, 2.0, len(A))
+    assert np.allclose(t.eval(Q, 1e30, point_areas=A2), O.direct(P, N, A2, Q, kind="screened"), rtol=1e-12,
+                       atol=1e-13)
+    assert np.array_equal(t.eval(Q, 2.3, point_areas=A), t.eval(Q, 2.3))
+    assert not np.allclose(t.eval(Q, 2.3, point_areas=A2), t.eval(Q, 2.3))
+
+
+def test_leaf_normals_closed_form():
+    # two points far apart: the root splits, each point alone in a child cell
+    # of width W/2 centred a quarter width off the root centre
+    P = np.array([[0.0, 0.0, 0.0], [1.0, 0.5, 0.25]])
+    N = np.array([[0.6, 0.0, 0.8], [0.0, 1.0, 0.0]])
+    L = O.leaf_normals(P, N)
+    W = 1.0  # the largest extent
+    c0 = np.array([0.5, 0.25, 0.125]) - W / 4
+    c1 = np.array([0.5, 0.25, 0.125]) + np.array([W / 4, W / 4, W / 4])
+    raw = np.stack([np.exp(-((P[0] - c0) / (W / 2)) ** 2 / 4) * N[0], np.exp(-((P[1] - c1) / (W / 2)) ** 2 / 4) * N[1]])
+    assert np.allclose(L, raw / np.linalg.norm(raw), rtol=1e-14)
+    # whole-matrix normalisation: unit Frobenius norm, invariant to scaling
+    Ps, Ns = shapes.sphere(3000, 9)
+    L = O.leaf_normals(Ps, Ns)
+    assert abs(np.linalg.norm(L) - 1) < 1e-12
+    assert np.allclose(O.leaf_normals(Ps, 3.0 * Ns), L, rtol=1e-12)
+    # the Gaussian weight lies in (e^{-1/16}, 1] inside a leaf (|P − C| ≤ W/2
+    # per axis), so each leaf row stays within that factor of Σ N
+    cen, pts, wid = O.octree_leaves(Ps.astype(np.float64), 10, 0, widths=True)
+    f = np.linalg.norm(np.stack([(np.exp(-((Ps[i] - c) / w) ** 2 / 4) * Ns[i]).sum(0)
+                                 for c, i, w in zip(cen, pts, wid)]))
+    one = np.array([len(i) == 1 for i in pts])
+    sums = np.stack([Ns[i].sum(0) for i in pts])
+    ratio = (L * f)[one] / sums[one]
+    assert np.all((ratio > np.exp(-1 / 16) - 1e-12) & (ratio <= 1 + 1e-12))
+
+
+def test_normal_change_closed_forms():
+    rng = np.random.default_rng(12)
+    n = rng.standard_normal((1000, 3))
+    assert O.normal_change(n, n) == pytest.approx(0, abs=1e-15)
+    assert O.normal_change(n, -2.5 * n) == pytest.approx(0, abs=1e-15)  # sign and scale blind
+    e = np.tile([1.0, 0, 0], (4, 1))
+    f = np.array([[0, 1.0, 0], [0, 0, 1.0], [np.cos(0.3), np.sin(0.3), 0], [0, 0, 0]])
+    assert O.normal_change(e, f) == pytest.approx((1 + 1 + (1 - np.cos(0.3)) + 1) / 4, rel=1e-14)
