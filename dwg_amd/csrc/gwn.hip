@@ -427,7 +427,7 @@ void octree_set_point_areas(dwg_ctx* ctx, dwg_octree* T, const float* nrm, const
 // ---------------------------------------------------------------------------
 // Barnes-Hut traversal, one wave per 64 queries
 // ---------------------------------------------------------------------------
-constexpr int kTreeThreads = 256;
+constexpr int kTreeThreads = 64;
 constexpr int kTreeWaves = kTreeThreads / 64;
 constexpr int kStack = 7 * 21 + 8;
 

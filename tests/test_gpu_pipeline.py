@@ -196,3 +196,47 @@ def test_direct_sums_loop_matches_oracle(ctx):
     assert np.allclose(r.w_mean[:3], wm[:3], rtol=1e-3), (r.w_mean, wm)
     assert np.allclose(r.w_mean, wm, rtol=2e-2), (r.w_mean, wm)
     assert np.all(np.abs(r.iter_tris - nt) <= 0.03 * nt), (r.iter_tris, nt)
+
+
+def test_small_sphere_direct_config(ctx):
+    # The small sphere configuration at full size with its own method:
+    # 10,000 area-uniform sphere points (seed 1), direct O(N²) sums (β ≤ 0)
+    # for the sources and the 128³ grid, 30 iterations.  Checked: the
+    # direct sums with the final normals at 2,000 sampled grid points against
+    # the FP64 all-pairs sum (fast_winding_number.cpp:313-323); Gauss's law
+    # with sphere-area weights (w = 1 inside, 0 outside); normals against the
+    # analytic sphere.  The reconstructed normals carry the method's own
+    # error (median about 5°), so the Gauss sums deviate from 1 and 0 by up to
+    # a few percent at single queries (measured: inside mean 0.996, min
+    # 0.972; outside max 0.020) and are checked to 5% per query, 1% on the
+    # mean.
+    P, N = shapes.sphere(10_000, 1)
+    r = pipeline.reconstruct(ctx, P, beta=0.0, grid_reso=128, final_reso=128, total_iter=30, source_mode=1)
+    assert r.iterations == 30 and len(r.w_mean) == 30
+    Pn = ((P.astype(np.float64) - r.center) / r.scale).astype(np.float32)
+    n = r.normals / np.linalg.norm(r.normals, axis=1, keepdims=True)
+    origin, h, res = O.voxel_grid(Pn.astype(np.float64), 0.05, 128)
+    rng = np.random.default_rng(5)
+    gi = rng.integers(0, int(res[0]) * int(res[1]) * int(res[2]), 2000)
+    G = np.stack([origin[0] + (gi % res[0]) * h, origin[1] + (gi // res[0] % res[1]) * h,
+                  origin[2] + gi // (res[0] * res[1]) * h], -1).astype(np.float32)
+    A = np.full(len(Pn), 2e-6, np.float32)
+    w = ctx.gwn_direct(Pn, n, A, G, sigma=10.0, screened=True, kahan=True).cpu().numpy()
+    ref = O.direct(Pn.astype(np.float64), n.astype(np.float64), A.astype(np.float64), G.astype(np.float64),
+                   kind="screened", sigma=10.0)
+    err = np.abs(w - ref).max() / np.abs(ref).max()
+    # Gauss's law with the final normals: plain kernel, areas 4πR²/n
+    R = np.linalg.norm(Pn, axis=1).mean()
+    Ag = np.full(len(Pn), 4 * np.pi * R * R / len(Pn), np.float32)
+    dirs = rng.standard_normal((200, 3))
+    dirs /= np.linalg.norm(dirs, axis=1, keepdims=True)
+    Qin, Qout = (0.5 * R * dirs).astype(np.float32), (1.5 * R * dirs).astype(np.float32)
+    win = ctx.gwn_direct(Pn, n, Ag, Qin).cpu().numpy()
+    wout = ctx.gwn_direct(Pn, n, Ag, Qout).cpu().numpy()
+    ang = metrics.normal_angle_deg(r.normals, N)
+    print(f"small sphere, direct sums: grid max |w - w64| / max|w| {err:.2e}; Gauss inside {win.mean():.4f} "
+          f"(min {win.min():.4f}), outside {np.abs(wout).max():.2e}; normals median {np.median(ang):.2f} mean "
+          f"{ang.mean():.2f} deg")
+    assert err < 1e-5
+    assert abs(win.mean() - 1) < 0.01 and np.all(np.abs(win - 1) < 0.05) and np.all(np.abs(wout) < 0.05)
+    assert np.median(ang) < 6.0 and np.mean(ang < 90) > 0.99
