@@ -17,10 +17,11 @@ struct dwg_octree {
     std::vector<int64_t> level_start;  // nodes of depth d: [level_start[d], level_start[d+1])
     int4* info = nullptr;      // (first_child | −1, begin, end, depth)
     int2* kid = nullptr;       // traversal record: (first_child, 0) internal, (−count, begin) leaf
-    float4* cm_r = nullptr;    // (float(CM), (β·R)² for the β of the last evaluation)
+    float4* cm_r = nullptr;    // (float(CM), (β·R)² for the β of the last evaluation); one-point leaf: (point, guard)
     float* radius = nullptr;   // float(R)
     float far_beta = -1.f;     // β that cm_r.w was computed for
     float far_floor = -1.f;    // lower bound cm_r.w was clamped to (0 for the exact far test)
+    float far_guard = 0.f;     // cm_r.w of one-point leaves: the point term's guard (1e-20 screened, −1 plain)
     float4* cm_lo = nullptr;   // (CM − float(CM), 0): residual of the rounded centre
     float4* ec = nullptr;      // (EC/4π, 0), refreshed per iteration
 
@@ -31,8 +32,9 @@ struct dwg_octree {
     double4* cell = nullptr;       // cell centre and width (breadth-first node order)
     double4* cm64 = nullptr;       // (CM, ΣA) in FP64
     // 8 words per node, read by the traversal for all 8 children of a node
-    // in one vector load: internal node (EC/4π, 0, CM − float(CM), 0);
-    // leaf with one point: that point's (A·n/4π, A, x, y, z, s)
+    // in one vector load: internal node (EC/4π, s², CM − float(CM), 0) with
+    // s the default screening radius; leaf with one point: that point's
+    // (A·n/4π, s², 0, 0, 0, 0), its position in cm_r
     float4* payload = nullptr;
     bool payload_bf16 = false;  // internal nodes' EC/4π in the payload rounded to BF16
     ~dwg_octree();

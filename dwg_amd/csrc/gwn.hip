@@ -352,11 +352,27 @@ dwg_octree::~dwg_octree() {
 namespace dwg {
 
 // EC/4π per node (FP64 prefix-sum difference, one rounding), into ec and
-// the node payload: words 0-3 of an internal node; a leaf with one point
-// carries that point's (A·n/4π, A) and (x, y, z, s) instead.
+// A leaf with one point is walked like a far dipole: its traversal record
+// holds the point (x, y, z), its payload the point's (A·n/4π, s²) and a zero
+// residual, so the walk reads the same words for both and evaluates the
+// same term, ρ² = max(r², s²) (an internal node's s² is the default
+// screening radius's)
+constexpr float kDipoleS2 = kScreenDefault * kScreenDefault;
+
+__device__ __forceinline__ void write_point_leaf(int64_t i, const float4 a, const float4 b, float4* __restrict__ cm_r,
+                                                 float4* __restrict__ payload) {
+    payload[2 * i] = make_float4(b.x, b.y, b.z, a.w * a.w);
+    payload[2 * i + 1] = make_float4(0.f, 0.f, 0.f, 0.f);
+    cm_r[i].x = a.x;
+    cm_r[i].y = a.y;
+    cm_r[i].z = a.z;
+}
+
+// the node payload: words 0-3 of an internal node (EC/4π, s²); a leaf with
+// one point carries that point's term instead (write_point_leaf)
 __global__ void node_ec_kernel(const int4* __restrict__ info, const Double3* __restrict__ scan, int64_t m,
                                const float4* __restrict__ sa, const float4* __restrict__ sb,
-                               float4* __restrict__ ec, float4* __restrict__ payload) {
+                               float4* __restrict__ ec, float4* __restrict__ payload, float4* __restrict__ cm_r) {
     int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i >= m) return;
     const int4 nf = info[i];
@@ -364,10 +380,9 @@ __global__ void node_ec_kernel(const int4* __restrict__ info, const Double3* __r
     const float x = float(s.x * double(kInv4Pi)), y = float(s.y * double(kInv4Pi)), z = float(s.z * double(kInv4Pi));
     ec[i] = make_float4(x, y, z, 0.f);
     if (nf.x >= 0) {
-        payload[2 * i] = make_float4(x, y, z, 0.f);
+        payload[2 * i] = make_float4(x, y, z, kDipoleS2);
     } else if (nf.z - nf.y == 1) {
-        payload[2 * i] = sb[nf.y];
-        payload[2 * i + 1] = sa[nf.y];
+        write_point_leaf(i, sa[nf.y], sb[nf.y], cm_r, payload);
     }
 }
 
@@ -394,7 +409,7 @@ void octree_update(dwg_ctx* ctx, dwg_octree* T, const float* nrm, const float* a
     }
     ProfileScope ps(ctx, "octree_ec");
     hipLaunchKernelGGL(node_ec_kernel, dim3(grid_for(m, 256)), dim3(256), 0, ctx->stream, T->info, T->an64, m, T->sa,
-                       T->sb, T->ec, T->payload);
+                       T->sb, T->ec, T->payload, T->cm_r);
     check_launch();
     T->payload_bf16 = false;
     T->updated = true;
@@ -402,14 +417,12 @@ void octree_update(dwg_ctx* ctx, dwg_octree* T, const float* nrm, const float* a
 
 // the point terms of one-point leaves live in their payload too
 __global__ void leaf_payload_kernel(const int4* __restrict__ info, int64_t m, const float4* __restrict__ sa,
-                                    const float4* __restrict__ sb, float4* __restrict__ payload) {
+                                    const float4* __restrict__ sb, float4* __restrict__ payload,
+                                    float4* __restrict__ cm_r) {
     const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i >= m) return;
     const int4 nf = info[i];
-    if (nf.x < 0 && nf.z - nf.y == 1) {
-        payload[2 * i] = sb[nf.y];
-        payload[2 * i + 1] = sa[nf.y];
-    }
+    if (nf.x < 0 && nf.z - nf.y == 1) write_point_leaf(i, sa[nf.y], sb[nf.y], cm_r, payload);
 }
 
 void octree_set_point_areas(dwg_ctx* ctx, dwg_octree* T, const float* nrm, const float* area) {
@@ -420,7 +433,7 @@ void octree_set_point_areas(dwg_ctx* ctx, dwg_octree* T, const float* nrm, const
                        nullptr, T->perm, n, 0, nullptr, T->sb, nullptr);
     check_launch();
     hipLaunchKernelGGL(leaf_payload_kernel, dim3(grid_for(m, 256)), dim3(256), 0, ctx->stream, T->info, m, T->sa,
-                       T->sb, T->payload);
+                       T->sb, T->payload, T->cm_r);
     check_launch();
 }
 
@@ -494,10 +507,10 @@ template <bool SCREENED, bool COUNT, bool BF16, bool GRID, bool NEAR64 = false>
 __global__ __launch_bounds__(kTreeThreads) void gwn_tree_kernel(TreeArgs T, QueryArgs Q, float beta, float sigma,
                                                                  uint32_t flags, float* __restrict__ out,
                                                                  int64_t* __restrict__ terms_out) {
-    __shared__ int s_node[kTreeWaves][kStack];
+    __shared__ int2 s_node[kTreeWaves][kStack];
     __shared__ uint64_t s_mask[kTreeWaves][kStack];
     __shared__ float4 s_pay[kTreeWaves][16];  // the 8 children's payloads of the node being opened
-    __shared__ float4 s_cmr[kTreeWaves][16];  // and their (CM, (βR)²) records (twice)
+    __shared__ float4 s_cmr[kTreeWaves][16];  // and their (CM, w) records (twice)
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int64_t blk = xcd_block(blockIdx.x, gridDim.x);
     const int64_t wid = blk * kTreeWaves + wave;
@@ -535,7 +548,6 @@ __global__ __launch_bounds__(kTreeThreads) void gwn_tree_kernel(TreeArgs T, Quer
     int64_t nterms = 0;
     const bool kahan = flags & DWG_KAHAN;
 
-    constexpr float kS2 = kScreenDefault * kScreenDefault;
     double near = 0.0;  // DWG_NEAR_FP64: point terms inside the screening radius, in FP64
     auto point_term = [&](const float4 a, const float4 b, bool on, float part) {
         const float dx = a.x - qx, dy = a.y - qy, dz = a.z - qz;
@@ -555,26 +567,32 @@ __global__ __launch_bounds__(kTreeThreads) void gwn_tree_kernel(TreeArgs T, Quer
     };
     auto leaf_sum = [&](int b, int e, bool act) {
         float part = 0.f;
-        for (int j = b; j < e; ++j) part = point_term(T.sa[j], T.sb[j], act, part);
+        for (int j = b; j < e; ++j) part = point_term(scalar_load(T.sa, j), scalar_load(T.sb, j), act, part);
         if (COUNT && act) nterms += e - b;
         if (kahan) kahan_add(sum, comp, part); else sum += part;
     };
 
-    // the stack holds (first child of a node to open, lanes that open it)
+    // the stack holds (traversal record of a node to open, lanes that open
+    // it): an internal node's first child, or a leaf's (−count, begin)
     int sp = 0;
     const int4 root = scalar_load(T.info, 0);
     if (root.x < 0) {
         leaf_sum(root.y, root.z, valid);
     } else {
-        s_node[wave][0] = root.x;
+        s_node[wave][0] = make_int2(root.x, 0);
         s_mask[wave][0] = all;
         sp = 1;
     }
     while (sp > 0) {
         --sp;
-        const int fc = uniform(s_node[wave][sp]);
+        const int2 top = s_node[wave][sp];
+        const int fc = uniform(top.x);
         const uint64_t mask = uniform64(s_mask[wave][sp]);
         const bool act = __builtin_amdgcn_inverse_ballot_w64(mask);
+        if (fc < 0) {  // a leaf of several points: summed point by point
+            leaf_sum(uniform(top.y), uniform(top.y) - fc, act);
+            continue;
+        }
         float part = 0.f;
         // the 8 children are contiguous: their records come in one batch of
         // scalar loads, their payloads in one coalesced vector load (word
@@ -583,7 +601,7 @@ __global__ __launch_bounds__(kTreeThreads) void gwn_tree_kernel(TreeArgs T, Quer
         int2 nks[8];
 #pragma unroll
         for (int c = 0; c < 8; ++c) nks[c] = scalar_load(T.kid, fc + c);
-        // payload word 8c + w of child c: loaded by lane 8c + w, and (CM, (βR)²)
+        // payload word 8c + w of child c: loaded by lane 8c + w, and (CM, w)
         // word 4c + w by lanes 4c + w and 32 + 4c + w (a second copy keeps the
         // store unconditional); both read back by the whole wave from LDS at
         // uniform addresses (broadcasts).  Records in LDS instead of SGPRs
@@ -597,25 +615,27 @@ __global__ __launch_bounds__(kTreeThreads) void gwn_tree_kernel(TreeArgs T, Quer
         for (int c = 0; c < 8; ++c) {
             const int2 nk = nks[c];
             if (nk.x == 0) continue;  // empty
-            if (nk.x < 0) {  // leaf child: summed point by point whether near or far
-                if (nk.x == -1) {
-                    part = point_term(s_pay[wave][2 * c + 1], s_pay[wave][2 * c], act, part);
-                } else {
-                    for (int j = nk.y; j < nk.y - nk.x; ++j)
-                        part = point_term(scalar_load(T.sa, j), scalar_load(T.sb, j), act, part);
-                }
-                if (COUNT && act) nterms += -nk.x;
-                continue;
-            }
-            // cm_r.w holds (β·R)², rounded as float(float(β·R)²), for this β
+            // one test and one term for every kind of child: an internal
+            // node is far when d² > (β·R)² (cm_r.w), a one-point leaf's
+            // record makes the test its term's guard, a leaf of several
+            // points is never far (∞) and is opened like an internal node
             const float4 cr = s_cmr[wave][c];
             const float dx = cr.x - qx, dy = cr.y - qy, dz = cr.z - qz;
             const float r2 = dist2(dx, dy, dz);
-            const bool far = r2 > cr.w;
+            bool far = r2 > cr.w;
+            if constexpr (NEAR64) {
+                // point terms inside the screening radius go to the FP64 sum
+                if (nk.x == -1) {
+                    const bool nr = act && far && r2 < s_pay[wave][2 * c].w;
+                    if (__builtin_amdgcn_ballot_w64(nr)) {  // rare: lanes next to a source
+                        if (nr) near += near_term64(scalar_load(T.sa, nk.y), scalar_load(T.sb, nk.y), qx, qy, qz, sigma);
+                    }
+                    far = far && !nr;
+                }
+            }
             // lane masks straight from the compare: far for the lanes that
             // opened this node, near ones open the child
             const uint64_t farm = __builtin_amdgcn_ballot_w64(far);
-            const uint64_t nm = mask & ~farm;
             if (mask & farm) {
                 const float4 e4 = s_pay[wave][2 * c];
                 if (BF16) {
@@ -623,28 +643,35 @@ __global__ __launch_bounds__(kTreeThreads) void gwn_tree_kernel(TreeArgs T, Quer
                     // the rounding of the centre: d = CM_f − q and its |d|²
                     // from the far test serve as they are, and the term's
                     // last product is fused into the sum.  r > 1e-10 holds
-                    // for every far node here (far radius floor)
-                    float inv;
-                    bool ok;
-                    const float x = tree_term_split<SCREENED>(dx, dy, dz, r2, kS2, e4, sigma, sl2e, inv, ok);
-                    part = __builtin_fmaf((act && far) ? x : 0.f, inv, part);
+                    // for every far node here (far radius floor) and is the
+                    // far test of a point.  The lanes that take the term are
+                    // selected by the exec mask (mask & farm), not by a
+                    // select on the result: one VALU instruction less per term
+                    if (__builtin_amdgcn_inverse_ballot_w64(mask & farm)) {
+                        float inv;
+                        bool ok;
+                        const float x = tree_term_split<SCREENED>(dx, dy, dz, r2, e4.w, e4, sigma, sl2e, inv, ok);
+                        part = __builtin_fmaf(x, inv, part);
+                    }
                 } else {
                     // d = (CM_f − q) + δ: the first difference is exact for
-                    // nearby values, so d carries a single rounding
+                    // nearby values, so d carries a single rounding (δ = 0
+                    // for a point)
                     const float4 lo = s_pay[wave][2 * c + 1];
                     const float ex = dx + lo.x, ey = dy + lo.y, ez = dz + lo.z;
                     float inv;
                     bool ok;
-                    const float x = tree_term_split<SCREENED>(ex, ey, ez, dist2(ex, ey, ez), kS2, e4, sigma, sl2e, inv, ok);
+                    const float x = tree_term_split<SCREENED>(ex, ey, ez, dist2(ex, ey, ez), e4.w, e4, sigma, sl2e, inv, ok);
                     // the plain dipole has no r > 1e-10 guard in the reference either
                     part += ((act && far && ok) ? x * inv : 0.f);
                 }
-                if (COUNT && act && far) nterms += 1;
             }
-            if (nm) {
+            if (COUNT && act && (far || nk.x == -1)) nterms += 1;
+            const uint64_t nm = mask & ~farm;
+            if (nm && nk.x != -1) {
                 // every lane writes the same entry: no divergent branch, so
                 // the stack pointer stays in a scalar register
-                s_node[wave][sp] = nk.x;
+                s_node[wave][sp] = nk;
                 s_mask[wave][sp] = nm;
                 ++sp;
             }
@@ -791,13 +818,17 @@ static QueryArgs make_grid(const dwg_grid* g) {
 
 // cm_r.w = (β·R)² for the traversal's far test d² > (β·R)², with the
 // rounding sequence float(β·R) then its square (the order the oracle uses),
-// clamped below by `floor`
-__global__ void far_radius_kernel(const float* __restrict__ r, int64_t m, float beta, float floor,
-                                  float4* __restrict__ cm_r) {
+// clamped below by `floor`.  A leaf with one point gets `point_guard`, so
+// that the far test is the point term's own guard (r² > 1e-20 screened,
+// none plain: −1); a leaf with several points ∞, so that it is always
+// opened and its points summed one by one
+__global__ void far_radius_kernel(const int2* __restrict__ kid, const float* __restrict__ r, int64_t m, float beta,
+                                  float floor, float point_guard, float4* __restrict__ cm_r) {
     const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i >= m) return;
+    const int k = kid[i].x;
     const float br = beta * r[i];
-    cm_r[i].w = fmaxf(br * br, floor);
+    cm_r[i].w = k == -1 ? point_guard : k < -1 ? INFINITY : fmaxf(br * br, floor);
 }
 
 // Far-radius floor of the mixed-precision walk: with d² > 4e-20 every far
@@ -815,7 +846,8 @@ __global__ void payload_precision_kernel(const int2* __restrict__ kid, const flo
     const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i >= m || kid[i].x <= 0) return;
     const float4 e = ec[i];
-    payload[2 * i] = bf16 ? make_float4(bf16_round(e.x), bf16_round(e.y), bf16_round(e.z), 0.f) : e;
+    payload[2 * i] = bf16 ? make_float4(bf16_round(e.x), bf16_round(e.y), bf16_round(e.z), kDipoleS2)
+                          : make_float4(e.x, e.y, e.z, kDipoleS2);
 }
 
 static void launch_tree(dwg_ctx* ctx, const dwg_octree* Tc, const QueryArgs& Q, float beta, float sigma,
@@ -825,12 +857,14 @@ static void launch_tree(dwg_ctx* ctx, const dwg_octree* Tc, const QueryArgs& Q, 
     auto* T = const_cast<dwg_octree*>(Tc);  // the far radii follow β; the tree itself is unchanged
     const bool want_bf16 = (flags & DWG_FAR_BF16) && !count;
     const float floor = want_bf16 ? kFarFloorBf16 : 0.f;
-    if (!(T->far_beta == beta) || T->far_floor != floor) {
-        hipLaunchKernelGGL(far_radius_kernel, dim3(grid_for(T->n_nodes, 256)), dim3(256), 0, ctx->stream, T->radius,
-                           T->n_nodes, beta, floor, T->cm_r);
+    const float guard = (flags & DWG_SCREENED) ? 1e-20f : -1.f;
+    if (!(T->far_beta == beta) || T->far_floor != floor || T->far_guard != guard) {
+        hipLaunchKernelGGL(far_radius_kernel, dim3(grid_for(T->n_nodes, 256)), dim3(256), 0, ctx->stream, T->kid,
+                           T->radius, T->n_nodes, beta, floor, guard, T->cm_r);
         check_launch();
         T->far_beta = beta;
         T->far_floor = floor;
+        T->far_guard = guard;
     }
     if (T->payload_bf16 != want_bf16) {
         hipLaunchKernelGGL(payload_precision_kernel, dim3(grid_for(T->n_nodes, 256)), dim3(256), 0, ctx->stream, T->kid,
@@ -891,12 +925,14 @@ void gwn_tree_grad(dwg_ctx* ctx, const dwg_octree* Tc, const float* q, int64_t n
     DWG_REQUIRE(Tc->max_level <= 21, "octree deeper than the traversal stack");
     DWG_REQUIRE(nq >= 0 && (q || nq == 0) && (out_w || nq == 0), "bad query or output array");
     auto* T = const_cast<dwg_octree*>(Tc);  // far radii follow β and the payload precision; the tree is unchanged
-    if (!(T->far_beta == beta) || T->far_floor != 0.f) {
-        hipLaunchKernelGGL(far_radius_kernel, dim3(grid_for(T->n_nodes, 256)), dim3(256), 0, ctx->stream, T->radius,
-                           T->n_nodes, beta, 0.f, T->cm_r);
+    const float guard = (flags & DWG_SCREENED) ? 1e-20f : -1.f;
+    if (!(T->far_beta == beta) || T->far_floor != 0.f || T->far_guard != guard) {
+        hipLaunchKernelGGL(far_radius_kernel, dim3(grid_for(T->n_nodes, 256)), dim3(256), 0, ctx->stream, T->kid,
+                           T->radius, T->n_nodes, beta, 0.f, guard, T->cm_r);
         check_launch();
         T->far_beta = beta;
         T->far_floor = 0.f;
+        T->far_guard = guard;
     }
     if (T->payload_bf16) {
         hipLaunchKernelGGL(payload_precision_kernel, dim3(grid_for(T->n_nodes, 256)), dim3(256), 0, ctx->stream, T->kid,

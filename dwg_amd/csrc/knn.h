@@ -41,6 +41,9 @@ struct dwg_knn_index {
         int64_t pyr_off_h[dwg::kMaxPyr];
         int pyr_levels = 0;             // 0: no pyramid (grid too elongated)
         float pyr_margin = 0.f;         // rounding allowance of cell boundaries
+        // bounding box of the points of every cell (at 2·cell) and pyramid
+        // block (at 2·(ncells + byte offset)): (lo, 0), (hi, 0); (∞, −∞) empty
+        float4* pbox = nullptr;
     } lvl[2];
     ~dwg_knn_index();
 };

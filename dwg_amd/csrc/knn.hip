@@ -23,6 +23,7 @@ dwg_knn_index::~dwg_knn_index() {
         if (L.pyr) (void)hipFree(L.pyr);
         if (L.pyr_dim) (void)hipFree(L.pyr_dim);
         if (L.pyr_off) (void)hipFree(L.pyr_off);
+        if (L.pbox) (void)hipFree(L.pbox);
     }
 }
 
@@ -263,6 +264,52 @@ __global__ void pyr_up_kernel(const uint8_t* __restrict__ lower, int dx, int dy,
     out[b] = uint8_t(m);
 }
 
+// Bounding boxes of the points themselves, per cell and per pyramid block:
+// a query far from the cloud is as close to a box of a surface patch as to
+// the patch, where the cell's own box can be a cell width nearer.  Exact
+// bounds of float points need no rounding allowance: the box distance,
+// computed like a point's d², never exceeds the d² of a point inside.
+__global__ void cell_box_kernel(const int64_t* __restrict__ cell_start, const float4* __restrict__ pts, int64_t ncells,
+                                float4* __restrict__ box) {
+    const int64_t c = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (c >= ncells) return;
+    float4 lo = make_float4(INFINITY, INFINITY, INFINITY, 0.f), hi = make_float4(-INFINITY, -INFINITY, -INFINITY, 0.f);
+    for (int64_t s = cell_start[c], e = cell_start[c + 1]; s < e; ++s) {
+        const float4 p = pts[s];
+        lo.x = fminf(lo.x, p.x);
+        lo.y = fminf(lo.y, p.y);
+        lo.z = fminf(lo.z, p.z);
+        hi.x = fmaxf(hi.x, p.x);
+        hi.y = fmaxf(hi.y, p.y);
+        hi.z = fmaxf(hi.z, p.z);
+    }
+    box[2 * c] = lo;
+    box[2 * c + 1] = hi;
+}
+
+__global__ void box_up_kernel(const float4* __restrict__ lower, int dx, int dy, int dz, int ux, int uy, int uz,
+                              float4* __restrict__ out) {
+    const int64_t b = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (b >= int64_t(ux) * uy * uz) return;
+    const int x = int(b % ux), y = int((b / ux) % uy), z = int(b / (int64_t(ux) * uy));
+    float4 lo = make_float4(INFINITY, INFINITY, INFINITY, 0.f), hi = make_float4(-INFINITY, -INFINITY, -INFINITY, 0.f);
+    for (int c = 0; c < 8; ++c) {
+        const int X = 2 * x + (c & 1), Y = 2 * y + ((c >> 1) & 1), Z = 2 * z + (c >> 2);
+        if (X < dx && Y < dy && Z < dz) {
+            const int64_t i = X + int64_t(dx) * (Y + int64_t(dy) * Z);
+            const float4 a = lower[2 * i], h = lower[2 * i + 1];
+            lo.x = fminf(lo.x, a.x);
+            lo.y = fminf(lo.y, a.y);
+            lo.z = fminf(lo.z, a.z);
+            hi.x = fmaxf(hi.x, h.x);
+            hi.y = fmaxf(hi.y, h.y);
+            hi.z = fmaxf(hi.z, h.z);
+        }
+    }
+    out[2 * b] = lo;
+    out[2 * b + 1] = hi;
+}
+
 static void pyramid(dwg_ctx* ctx, dwg_knn_index::Level& L) {
     int4* dims = L.pyr_dim_h;
     int64_t* off = L.pyr_off_h;
@@ -294,6 +341,18 @@ static void pyramid(dwg_ctx* ctx, dwg_knn_index::Level& L) {
         else
             hipLaunchKernelGGL(pyr_up_kernel, dim3(grid_for(nb, 256)), dim3(256), 0, ctx->stream, L.pyr + off[l - 1],
                                lo.x, lo.y, lo.z, up.x, up.y, up.z, L.pyr + off[l]);
+        check_launch();
+    }
+    DWG_HIP_CHECK(hipMalloc(&L.pbox, sizeof(float4) * 2 * size_t(L.ncells + total)));
+    hipLaunchKernelGGL(cell_box_kernel, dim3(grid_for(L.ncells, 256)), dim3(256), 0, ctx->stream, L.cell_start, L.pts,
+                       L.ncells, L.pbox);
+    check_launch();
+    for (int l = 1; l < levels; ++l) {
+        const int4 lo = dims[l - 1], up = dims[l];
+        const int64_t nb = int64_t(up.x) * up.y * up.z;
+        const float4* lower = L.pbox + 2 * (l == 1 ? 0 : L.ncells + off[l - 1]);
+        hipLaunchKernelGGL(box_up_kernel, dim3(grid_for(nb, 256)), dim3(256), 0, ctx->stream, lower, lo.x, lo.y, lo.z,
+                           up.x, up.y, up.z, L.pbox + 2 * (L.ncells + off[l]));
         check_launch();
     }
     float ext = 0.f;
@@ -997,7 +1056,27 @@ struct PyrArgs {
     const int64_t* off;
     int levels;
     float margin;
+    const float4* box;  // the points' bounding boxes (dwg_knn_index::Level::pbox)
+    int64_t ncells;
 };
+
+// squared distance from q to a box (lo, hi), computed like a point's d²
+// (dist2): every step is monotone, so it never exceeds the d² of a point
+// inside the box
+__device__ __forceinline__ float tight_box_d2(const float4 lo, const float4 hi, float qx, float qy, float qz) {
+    const float gx = fmaxf(fmaxf(lo.x - qx, qx - hi.x), 0.f);
+    const float gy = fmaxf(fmaxf(lo.y - qy, qy - hi.y), 0.f);
+    const float gz = fmaxf(fmaxf(lo.z - qz, qz - hi.z), 0.f);
+    return dist2(gx, gy, gz);
+}
+
+typedef float KnnFloat4 __attribute__((ext_vector_type(4)));
+// a wave-uniform box through the scalar data cache
+__device__ __forceinline__ float tight_box_d2_uniform(const float4* box, int64_t i, float qx, float qy, float qz) {
+    const auto* b = (const __attribute__((address_space(4))) KnnFloat4*)box;
+    const KnnFloat4 lo = b[2 * i], hi = b[2 * i + 1];
+    return tight_box_d2(make_float4(lo.x, lo.y, lo.z, 0.f), make_float4(hi.x, hi.y, hi.z, 0.f), qx, qy, qz);
+}
 
 __device__ __forceinline__ uint64_t pyr_entry(int x, int y, int z, uint32_t m) {
     return uint64_t(uint32_t(x)) | (uint64_t(uint32_t(y)) << 16) | (uint64_t(uint32_t(z)) << 32) | (uint64_t(m) << 48);
@@ -1101,7 +1180,10 @@ __global__ __launch_bounds__(kPyrThreads) void knn_pyramid_kernel(KnnGridArgs G,
 // point is then one uniform load for the wave instead of 64 scattered ones,
 // and the lanes stay in step.  Children are taken nearest-first for the
 // wave's first active lane; each lane still prunes with its own k-th key,
-// so its result is exactly its own k nearest points.
+// so its result is exactly its own k nearest points.  A child's distance
+// is that of the bounding box of its points, not of its cells: a far query
+// over a surface patch then opens the few cells near its foot point instead
+// of every cell whose box comes within a cell width of its k-th distance.
 constexpr int kPwWaves = 2;  // small blocks free their slots sooner: the walks end at very different times
 constexpr int kPwStack = 128;  // ≥ 7 · kMaxPyr + 1 entries
 
@@ -1136,7 +1218,7 @@ __global__ __launch_bounds__(kPwWaves * 64) void knn_pyramid_wave_kernel(KnnGrid
     auto kth_of = [&]() { return best[0] == ~0ull ? cap : fminf(cap, __uint_as_float(uint32_t(best[0] >> 32))); };
     int4* stk = s_stk[wave];
     const int top = Y.levels - 1;
-    if (lane == 0) stk[0] = make_int4(0, top << 16, int(uint32_t(all)), int(uint32_t(all >> 32)));
+    stk[0] = make_int4(0, top << 16, int(uint32_t(all)), int(uint32_t(all >> 32)));
     __builtin_amdgcn_wave_barrier();
     int sp = 1;
     while (sp > 0) {
@@ -1145,10 +1227,16 @@ __global__ __launch_bounds__(kPwWaves * 64) void knn_pyramid_wave_kernel(KnnGrid
         const int bx = uniform(e.x & 0xFFFF), by = uniform(int(uint32_t(e.x) >> 16)), bz = uniform(e.y & 0xFFFF),
                   l = uniform(int(uint32_t(e.y) >> 16));
         const uint64_t mask = (uint64_t(uint32_t(uniform(e.w))) << 32) | uint32_t(uniform(e.z));
-        const bool act = (mask >> lane) & 1;
+        const bool act = __builtin_amdgcn_inverse_ballot_w64(mask);
         const int4 dm = s_dim[l];
         const uint32_t occ = uniform(int(Y.mask[s_off[l] + bx + int64_t(dm.x) * (by + int64_t(dm.y) * bz)]));
-        const float w = G.cell * float(1 << (l - 1));
+        // children: blocks of level l − 1 (cells at l = 1), their distance
+        // from the bounding box of their points
+        const int4 cdm = s_dim[l - 1];
+        const int64_t cbase = l == 1 ? 0 : Y.ncells + s_off[l - 1];
+        auto child_d2 = [&](int x, int y, int z) {
+            return tight_box_d2_uniform(Y.box, cbase + x + int64_t(cdm.x) * (y + int64_t(cdm.y) * z), qx, qy, qz);
+        };
         const float kth = kth_of();
         // the children that some lane still needs, keyed by the first active
         // lane's box distance (float bits, positive) with the child number in
@@ -1161,8 +1249,7 @@ __global__ __launch_bounds__(kPwWaves * 64) void knn_pyramid_wave_kernel(KnnGrid
             need[c] = 0;
             key[c] = 0xFFFFFFFFu;
             if ((occ >> c) & 1u) {
-                const float d = pyr_box_d2(G, w, Y.margin, qx, qy, qz, 2 * bx + (c & 1), 2 * by + ((c >> 1) & 1),
-                                           2 * bz + (c >> 2));
+                const float d = child_d2(2 * bx + (c & 1), 2 * by + ((c >> 1) & 1), 2 * bz + (c >> 2));
                 need[c] = __builtin_amdgcn_ballot_w64(act && d <= kth);
                 if (need[c]) {
                     const uint32_t dr = uint32_t(__builtin_amdgcn_readlane(int(__float_as_uint(d)), ref));
@@ -1190,7 +1277,7 @@ __global__ __launch_bounds__(kPwWaves * 64) void knn_pyramid_wave_kernel(KnnGrid
                 const int c = int(key[j] & 7u);
                 const int x = 2 * bx + (c & 1), y = 2 * by + ((c >> 1) & 1), z = 2 * bz + (c >> 2);
                 // the lanes that still need this cell with their k-th key now
-                const float d = pyr_box_d2(G, w, Y.margin, qx, qy, qz, x, y, z);
+                const float d = child_d2(x, y, z);
                 const bool use = act && d <= kth_of();
                 if (!__builtin_amdgcn_ballot_w64(use)) continue;
                 const int64_t cid = x + int64_t(G.dim[0]) * (y + int64_t(G.dim[1]) * z);
@@ -1213,8 +1300,9 @@ __global__ __launch_bounds__(kPwWaves * 64) void knn_pyramid_wave_kernel(KnnGrid
 #pragma unroll
                 for (int cc = 0; cc < 8; ++cc) m = cc == c ? need[cc] : m;
                 const int x = 2 * bx + (c & 1), y = 2 * by + ((c >> 1) & 1), z = 2 * bz + (c >> 2);
-                if (lane == 0)
-                    stk[sp] = make_int4(x | (y << 16), z | ((l - 1) << 16), int(uint32_t(m)), int(uint32_t(m >> 32)));
+                // every lane writes the same entry (no divergent branch: the
+                // stack pointer stays in a scalar register)
+                stk[sp] = make_int4(x | (y << 16), z | ((l - 1) << 16), int(uint32_t(m)), int(uint32_t(m >> 32)));
                 ++sp;
             }
         }
@@ -1239,7 +1327,7 @@ static void launch_far(dwg_ctx* ctx, const dwg_knn_index* I, const float* q, int
         launch_wave(ctx, I, q, nq, qlist, k, skip, out_idx, out_d2);
         return;
     }
-    const PyrArgs Y{L.pyr, L.pyr_dim, L.pyr_off, L.pyr_levels, L.pyr_margin};
+    const PyrArgs Y{L.pyr, L.pyr_dim, L.pyr_off, L.pyr_levels, L.pyr_margin, L.pbox, L.ncells};
     if (shared) {
         DWG_REQUIRE(L.pyr_levels <= kMaxPyr && 7 * L.pyr_levels + 1 <= kPwStack, "pyramid deeper than the wave stack");
         dim3 grid(grid_for(nq, kPwWaves * 64)), block(kPwWaves * 64);
