@@ -83,6 +83,10 @@ def load():
     if not os.path.exists(LIB_PATH):
         raise RuntimeError(f"{LIB_PATH} is missing: build it with __graft_entry__.build() "
                            "(there is no CPU fallback)")
+    # torch first: the library then binds to the HIP runtime torch loads.
+    # Loaded the other way round, the process holds two HIP runtimes and the
+    # library's dwg_ctx_create fails on a GPU
+    import torch  # noqa: F401
     L = ctypes.CDLL(LIB_PATH)
     P = ctypes.POINTER
     L.dwg_ctx_create.argtypes = [_int, _vp, P(_vp)]

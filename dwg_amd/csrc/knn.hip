@@ -569,7 +569,7 @@ __global__ __launch_bounds__(256) void knn_query_kernel(KnnGridArgs G0, const in
 // ---------------------------------------------------------------------------
 constexpr double kScatterFix = 1099511627776.0;  // 2^40
 constexpr int kTile = 4;
-constexpr int kStage = 2048;
+constexpr int kStage = 1024;
 constexpr int kTileThreads = 256;
 // halos of the tiled passes: most queries close within 2 cells; the ones
 // that do not (off-surface centroids) get passes with 5 and 9 cells
@@ -645,7 +645,7 @@ __global__ __launch_bounds__(kTileThreads) void knn_tile_kernel(
     const int* __restrict__ tiles, const int64_t* __restrict__ tile_start, const int* __restrict__ order,
     const float* __restrict__ q, int k, const uint8_t* __restrict__ skip, int32_t* __restrict__ out_idx,
     float* __restrict__ out_d2, int* __restrict__ redo, float* __restrict__ redo_kth, unsigned int* __restrict__ n_redo,
-    KnnScatter sc) {
+    int64_t* __restrict__ redo_flag, KnnScatter sc) {
     constexpr int kRegion = kTile + 2 * HALO;
     constexpr int kRows = kRegion * kRegion;
     constexpr int kHash = SCATTER ? 512 : 1;
@@ -762,6 +762,14 @@ __global__ __launch_bounds__(kTileThreads) void knn_tile_kernel(
         if (!exact && best[0] != ~0ull) {
             const float safe = dmin * (1.0f - 1e-5f) - 1e-6f * G.cell;
             exact = safe > 0.f && kd2 < safe * safe;
+        }
+        if (!exact && redo_flag) {
+            // ordered: the query keeps its place in tile order (compacted
+            // afterwards), so the far walk's waves get spatially coherent runs
+            redo_flag[t] = 1;
+            redo[t] = qi;
+            if (redo_kth) redo_kth[t] = kd2;
+            continue;
         }
         if (!exact) {
             // the wave's lanes that give up here take consecutive slots from
@@ -1176,14 +1184,20 @@ __global__ __launch_bounds__(kPyrThreads) void knn_pyramid_kernel(KnnGridArgs G,
 // The same pyramid walk shared by a wave: 64 neighbouring queries (the far
 // queries arrive in tile order) walk the union of their searches with one
 // wave-uniform stack of (block, lanes that need it) in LDS, the way the
-// Barnes-Hut walk shares its tree.  Every block's occupancy byte and every
-// point is then one uniform load for the wave instead of 64 scattered ones,
-// and the lanes stay in step.  Children are taken nearest-first for the
-// wave's first active lane; each lane still prunes with its own k-th key,
-// so its result is exactly its own k nearest points.  A child's distance
-// is that of the bounding box of its points, not of its cells: a far query
-// over a surface patch then opens the few cells near its foot point instead
-// of every cell whose box comes within a cell width of its k-th distance.
+// Barnes-Hut walk shares its tree.  Every block and every point is then one
+// load for the wave instead of 64 scattered ones, and the lanes stay in
+// step.  Children are taken nearest-first for the wave's first active lane;
+// each lane still prunes with its own k-th key, so its result is exactly
+// its own k nearest points.  A child's distance is that of the bounding box
+// of its points, not of its cells: a far query over a surface patch then
+// opens the few cells near its foot point instead of every cell whose box
+// comes within a cell width of its k-th distance.
+//
+// The walk is a chain of dependent loads, so each step issues its loads
+// together: opening a block fetches its 8 children's boxes and their own
+// occupancy bytes (or, for cells, their point ranges) in one vector load
+// over a few lanes; a stack entry carries its block's occupancy byte; a
+// cell's points come in one vector load (64 at a time) staged in LDS.
 constexpr int kPwWaves = 2;  // small blocks free their slots sooner: the walks end at very different times
 constexpr int kPwStack = 128;  // ≥ 7 · kMaxPyr + 1 entries
 
@@ -1197,7 +1211,12 @@ __global__ __launch_bounds__(kPwWaves * 64) void knn_pyramid_wave_kernel(KnnGrid
                                                                         const float* __restrict__ init_kth,
                                                                         int32_t* __restrict__ out_idx,
                                                                         float* __restrict__ out_d2) {
-    __shared__ int4 s_stk[kPwWaves][kPwStack];  // (x | y << 16, z | level << 16, lanes lo, lanes hi)
+    // stack entry: (x | y << 16, z | level << 16 | occupancy << 20, lanes lo, lanes hi)
+    __shared__ int4 s_stk[kPwWaves][kPwStack];
+    __shared__ float4 s_box[kPwWaves][16];    // box of child c: lo at 2c, hi at 2c + 1
+    __shared__ int64_t s_rng[kPwWaves][16];   // level 1: point range of child cell c at 2c, 2c + 1
+    __shared__ int s_cocc[kPwWaves][16];      // level ≥ 2: occupancy byte of child c at 2c
+    __shared__ float4 s_pts[kPwWaves][64];    // points of the cell being scanned
     __shared__ int4 s_dim[kMaxPyr];
     __shared__ int64_t s_off[kMaxPyr];
     if (threadIdx.x < Y.levels) {
@@ -1217,26 +1236,40 @@ __global__ __launch_bounds__(kPwWaves * 64) void knn_pyramid_wave_kernel(KnnGrid
     if (all == 0) return;
     auto kth_of = [&]() { return best[0] == ~0ull ? cap : fminf(cap, __uint_as_float(uint32_t(best[0] >> 32))); };
     int4* stk = s_stk[wave];
+    float4* box = s_box[wave];
     const int top = Y.levels - 1;
-    stk[0] = make_int4(0, top << 16, int(uint32_t(all)), int(uint32_t(all >> 32)));
+    const int occ_top = uniform(int(Y.mask[s_off[top]]));
+    // every lane writes the same entries (no divergent branch: the stack
+    // pointer stays in a scalar register)
+    stk[0] = make_int4(0, (top << 16) | (occ_top << 20), int(uint32_t(all)), int(uint32_t(all >> 32)));
     __builtin_amdgcn_wave_barrier();
     int sp = 1;
     while (sp > 0) {
         --sp;
         const int4 e = stk[sp];
-        const int bx = uniform(e.x & 0xFFFF), by = uniform(int(uint32_t(e.x) >> 16)), bz = uniform(e.y & 0xFFFF),
-                  l = uniform(int(uint32_t(e.y) >> 16));
+        const uint32_t e0 = uint32_t(uniform(e.x)), e1 = uint32_t(uniform(e.y));
+        const int bx = int(e0 & 0xFFFFu), by = int(e0 >> 16), bz = int(e1 & 0xFFFFu), l = int((e1 >> 16) & 0xFu);
+        const uint32_t occ = e1 >> 20;
         const uint64_t mask = (uint64_t(uint32_t(uniform(e.w))) << 32) | uint32_t(uniform(e.z));
         const bool act = __builtin_amdgcn_inverse_ballot_w64(mask);
-        const int4 dm = s_dim[l];
-        const uint32_t occ = uniform(int(Y.mask[s_off[l] + bx + int64_t(dm.x) * (by + int64_t(dm.y) * bz)]));
-        // children: blocks of level l − 1 (cells at l = 1), their distance
-        // from the bounding box of their points
-        const int4 cdm = s_dim[l - 1];
-        const int64_t cbase = l == 1 ? 0 : Y.ncells + s_off[l - 1];
-        auto child_d2 = [&](int x, int y, int z) {
-            return tight_box_d2_uniform(Y.box, cbase + x + int64_t(cdm.x) * (y + int64_t(cdm.y) * z), qx, qy, qz);
-        };
+        // the children's loads, together: lanes 2c and 2c + 1 fetch child
+        // c's box (lo, hi); lanes 16 + 2c and 17 + 2c its point range (a
+        // cell) or lane 16 + 2c its occupancy byte (a block)
+        {
+            const int c = (lane >> 1) & 7;
+            if ((occ >> c) & 1u) {
+                const int4 cdm = s_dim[l - 1];
+                const int x = 2 * bx + (c & 1), y = 2 * by + ((c >> 1) & 1), z = 2 * bz + (c >> 2);
+                const int64_t ci = x + int64_t(cdm.x) * (y + int64_t(cdm.y) * z);
+                if (lane < 16) {
+                    box[lane] = Y.box[2 * ((l == 1 ? 0 : Y.ncells + s_off[l - 1]) + ci) + (lane & 1)];
+                } else if (lane < 32) {
+                    if (l == 1) s_rng[wave][lane - 16] = cell_start[ci + (lane & 1)];
+                    else if (!(lane & 1)) s_cocc[wave][lane - 16] = int(Y.mask[s_off[l - 1] + ci]);
+                }
+            }
+        }
+        __builtin_amdgcn_wave_barrier();
         const float kth = kth_of();
         // the children that some lane still needs, keyed by the first active
         // lane's box distance (float bits, positive) with the child number in
@@ -1249,7 +1282,7 @@ __global__ __launch_bounds__(kPwWaves * 64) void knn_pyramid_wave_kernel(KnnGrid
             need[c] = 0;
             key[c] = 0xFFFFFFFFu;
             if ((occ >> c) & 1u) {
-                const float d = child_d2(2 * bx + (c & 1), 2 * by + ((c >> 1) & 1), 2 * bz + (c >> 2));
+                const float d = tight_box_d2(box[2 * c], box[2 * c + 1], qx, qy, qz);
                 need[c] = __builtin_amdgcn_ballot_w64(act && d <= kth);
                 if (need[c]) {
                     const uint32_t dr = uint32_t(__builtin_amdgcn_readlane(int(__float_as_uint(d)), ref));
@@ -1275,20 +1308,28 @@ __global__ __launch_bounds__(kPwWaves * 64) void knn_pyramid_wave_kernel(KnnGrid
             for (int j = 0; j < 8; ++j) {
                 if (key[j] == 0xFFFFFFFFu) break;
                 const int c = int(key[j] & 7u);
-                const int x = 2 * bx + (c & 1), y = 2 * by + ((c >> 1) & 1), z = 2 * bz + (c >> 2);
                 // the lanes that still need this cell with their k-th key now
-                const float d = child_d2(x, y, z);
+                const float d = tight_box_d2(box[2 * c], box[2 * c + 1], qx, qy, qz);
                 const bool use = act && d <= kth_of();
                 if (!__builtin_amdgcn_ballot_w64(use)) continue;
-                const int64_t cid = x + int64_t(G.dim[0]) * (y + int64_t(G.dim[1]) * z);
-                const int64_t se = cell_start[cid + 1];
-                for (int64_t s0 = cell_start[cid]; s0 < se; ++s0) {
-                    const float4 p = pts[s0];
-                    const int idx = __float_as_int(p.w);
-                    if (skip && skip[idx]) continue;
-                    const float d2 = dist2(qx - p.x, qy - p.y, qz - p.z);
-                    const uint64_t kk = (uint64_t(__float_as_uint(d2)) << 32) | uint32_t(idx);
-                    if (use && kk < best[0]) knn_insert<K>(best, kk);
+                const int64_t cb = uniform64(uint64_t(s_rng[wave][2 * c])), ce = uniform64(uint64_t(s_rng[wave][2 * c + 1]));
+                for (int64_t s0 = cb; s0 < ce; s0 += 64) {
+                    const int cnt = int(min(ce - s0, int64_t(64)));
+                    if (lane < cnt) {
+                        float4 p = pts[s0 + lane];
+                        if (skip && skip[__float_as_int(p.w)]) p.w = __int_as_float(-1);
+                        s_pts[wave][lane] = p;
+                    }
+                    __builtin_amdgcn_wave_barrier();
+                    for (int jj = 0; jj < cnt; ++jj) {
+                        const float4 p = s_pts[wave][jj];
+                        const int idx = __float_as_int(p.w);
+                        if (idx < 0) continue;
+                        const float d2 = dist2(qx - p.x, qy - p.y, qz - p.z);
+                        const uint64_t kk = (uint64_t(__float_as_uint(d2)) << 32) | uint32_t(idx);
+                        if (use && kk < best[0]) knn_insert<K>(best, kk);
+                    }
+                    __builtin_amdgcn_wave_barrier();
                 }
             }
         } else {  // push the needed children, farthest first so the nearest pops first
@@ -1297,12 +1338,15 @@ __global__ __launch_bounds__(kPwWaves * 64) void knn_pyramid_wave_kernel(KnnGrid
                 if (key[j] == 0xFFFFFFFFu) continue;
                 const int c = int(key[j] & 7u);
                 uint64_t m = 0;
+                int co = 0;
 #pragma unroll
-                for (int cc = 0; cc < 8; ++cc) m = cc == c ? need[cc] : m;
+                for (int cc = 0; cc < 8; ++cc) {
+                    m = cc == c ? need[cc] : m;
+                }
+                co = uniform(s_cocc[wave][2 * c]);
                 const int x = 2 * bx + (c & 1), y = 2 * by + ((c >> 1) & 1), z = 2 * bz + (c >> 2);
-                // every lane writes the same entry (no divergent branch: the
-                // stack pointer stays in a scalar register)
-                stk[sp] = make_int4(x | (y << 16), z | ((l - 1) << 16), int(uint32_t(m)), int(uint32_t(m >> 32)));
+                stk[sp] = make_int4(x | (y << 16), z | ((l - 1) << 16) | (co << 20), int(uint32_t(m)),
+                                    int(uint32_t(m >> 32)));
                 ++sp;
             }
         }
@@ -1375,13 +1419,25 @@ static void launch_per_lane(dwg_ctx* ctx, const dwg_knn_index* I, const float* q
     check_launch();
 }
 
+// the queries a tiled pass leaves open, gathered in tile order
+__global__ void redo_compact_kernel(const int64_t* __restrict__ pos, const int* __restrict__ tmp_q,
+                                    const float* __restrict__ tmp_kth, int64_t n, int* __restrict__ redo,
+                                    float* __restrict__ redo_kth) {
+    const int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (t >= n || pos[t + 1] == pos[t]) return;
+    redo[pos[t]] = tmp_q[t];
+    if (redo_kth) redo_kth[pos[t]] = tmp_kth[t];
+}
+
 // One tiled pass over the queries qlist[0..nq) (or 0..nq−1, masked by
-// qmask); the queries it cannot close are listed in redo.  Returns their count.
+// qmask); the queries it cannot close are listed in redo (with their k-th
+// bound in redo_kth), in tile order when `ordered`, else in the order the
+// waves end.  Returns their count.
 template <int HALO>
 static int64_t tiled_pass(dwg_ctx* ctx, const dwg_knn_index* I, const float* q, int64_t nq, const int* qlist,
                           const uint8_t* qmask, int k, const uint8_t* skip, int32_t* out_idx, float* out_d2,
                           int* redo, const std::string& name, const KnnScatter* sc = nullptr,
-                          float* redo_kth = nullptr) {
+                          float* redo_kth = nullptr, bool ordered = false) {
     const KnnGridArgs& G = I->lvl[0].G;
     TileArgs T;
     for (int d = 0; d < 3; ++d) T.tdim[d] = (G.dim[d] + kTile - 1) / kTile;
@@ -1412,6 +1468,19 @@ static int64_t tiled_pass(dwg_ctx* ctx, const dwg_knn_index* I, const float* q, 
     check_launch();
     const int64_t nlist = read_scalar<int64_t>(ctx, flags + ntiles);
     if (ctx->profile) ctx->count(name + ":queries", read_scalar<int64_t>(ctx, start + ntiles));
+    // ordered: each query's place t in tile order gets its flag (and, when
+    // open, its index and bound) in scratch arrays, compacted below
+    int64_t* rflag = nullptr;
+    int* tmp_q = nullptr;
+    float* tmp_kth = nullptr;
+    if (ordered) {
+        rflag = static_cast<int64_t*>(ctx->buf("knnt_rflag", sizeof(int64_t) * (nq + 1)));
+        tmp_q = static_cast<int*>(ctx->buf("knnt_rtmp", sizeof(int) * nq + 64));
+        if (redo_kth) tmp_kth = static_cast<float*>(ctx->buf("knnt_rkth", sizeof(float) * nq + 64));
+        fill_u32(ctx, rflag, 0, 2 * (nq + 1));
+    }
+    int* redo_out = ordered ? tmp_q : redo;
+    float* kth_out = ordered ? tmp_kth : redo_kth;
     if (nlist > 0) {
         ProfileScope pt(ctx, name.c_str());
         const KnnScatter scv = sc ? *sc : KnnScatter{nullptr, nullptr};
@@ -1420,11 +1489,11 @@ static int64_t tiled_pass(dwg_ctx* ctx, const dwg_knn_index* I, const float* q, 
         if (sc)                                                                                                        \
             hipLaunchKernelGGL((knn_tile_kernel<KK, HALO, true>), dim3(unsigned(nlist)), dim3(kTileThreads), 0,        \
                                ctx->stream, G, T, I->lvl[0].cell_start, I->lvl[0].pts, list, start, order, q, k, skip, \
-                               out_idx, out_d2, redo, redo_kth, n_redo, scv);                                                    \
+                               out_idx, out_d2, redo_out, kth_out, n_redo, rflag, scv);                                \
         else                                                                                                           \
             hipLaunchKernelGGL((knn_tile_kernel<KK, HALO, false>), dim3(unsigned(nlist)), dim3(kTileThreads), 0,       \
                                ctx->stream, G, T, I->lvl[0].cell_start, I->lvl[0].pts, list, start, order, q, k, skip, \
-                               out_idx, out_d2, redo, redo_kth, n_redo, scv);                                                    \
+                               out_idx, out_d2, redo_out, kth_out, n_redo, rflag, scv);                                \
     } while (0)
         if (k <= 8) DWG_KNNT(8);
         else if (k <= 10) DWG_KNNT(10);
@@ -1434,7 +1503,16 @@ static int64_t tiled_pass(dwg_ctx* ctx, const dwg_knn_index* I, const float* q, 
 #undef DWG_KNNT
         check_launch();
     }
-    const int64_t nredo = int64_t(read_scalar<unsigned int>(ctx, n_redo));
+    int64_t nredo;
+    if (ordered) {
+        exclusive_scan<int64_t>(ctx, rflag, nq);
+        hipLaunchKernelGGL(redo_compact_kernel, dim3(grid_for(nq, 256)), dim3(256), 0, ctx->stream, rflag, tmp_q,
+                           tmp_kth, nq, redo, redo_kth);
+        check_launch();
+        nredo = read_scalar<int64_t>(ctx, rflag + nq);
+    } else {
+        nredo = int64_t(read_scalar<unsigned int>(ctx, n_redo));
+    }
     ctx->count(name + ":redo", nredo);
     return nredo;
 }
@@ -1484,7 +1562,7 @@ void knn_query(dwg_ctx* ctx, const dwg_knn_index* I, const float* q, int64_t nq,
                               : 0;
     float* kth3 = static_cast<float*>(ctx->buf("knnt_kth3", sizeof(float) * nq + 64));
     const int64_t n3 = n2 > 0 ? tiled_pass<kHalo3>(ctx, I, q, n2, redo2, nullptr, k, skip, out_idx, out_d2, redo3,
-                                                   base + ":tiles3", sc, kth3)
+                                                   base + ":tiles3", sc, kth3, true)
                               : 0;
     {
         const std::string name = base + ":redo";
