@@ -310,6 +310,17 @@ __global__ void box_up_kernel(const float4* __restrict__ lower, int dx, int dy, 
     out[2 * b + 1] = hi;
 }
 
+// cell boundaries are computed as lo + b·w in float and points are
+// assigned by floor((v − lo)/cell): both are within a few ulp of the exact
+// boundary, so a box of cells is shrunk by 16 ulp of the largest coordinate
+// plus 1e-5 cell before it may exclude a point
+static float cell_margin(const KnnGridArgs& G) {
+    float ext = 0.f;
+    for (int a = 0; a < 3; ++a)
+        ext = std::max({ext, std::fabs(G.lo[a]), std::fabs(G.lo[a] + G.cell * float(G.dim[a]))});
+    return 16.f * 1.1920929e-7f * ext + 1e-5f * G.cell;
+}
+
 static void pyramid(dwg_ctx* ctx, dwg_knn_index::Level& L) {
     int4* dims = L.pyr_dim_h;
     int64_t* off = L.pyr_off_h;
@@ -355,13 +366,7 @@ static void pyramid(dwg_ctx* ctx, dwg_knn_index::Level& L) {
                            up.x, up.y, up.z, L.pbox + 2 * (L.ncells + off[l]));
         check_launch();
     }
-    float ext = 0.f;
-    for (int a = 0; a < 3; ++a)
-        ext = std::max({ext, std::fabs(L.G.lo[a]), std::fabs(L.G.lo[a] + L.G.cell * float(L.G.dim[a]))});
-    // cell boundaries are computed as lo + b·w in float and points are
-    // assigned by floor((v − lo)/cell): both are within a few ulp of the
-    // exact boundary; 16 ulp of the largest coordinate plus 1e-5 cell
-    L.pyr_margin = 16.f * 1.1920929e-7f * ext + 1e-5f * L.G.cell;
+    L.pyr_margin = cell_margin(L.G);
     L.pyr_levels = levels;
 }
 
@@ -578,7 +583,8 @@ constexpr int kHalo2 = 5;
 constexpr int kHalo3 = 9;
 
 struct TileArgs {
-    int tdim[3];  // tiles per axis
+    int tdim[3];   // tiles per axis
+    float margin;  // rounding allowance of cell boundaries (cell_margin)
 };
 
 __device__ __forceinline__ int tile_of_point(const KnnGridArgs& G, const TileArgs& T, float x, float y, float z) {
@@ -661,6 +667,7 @@ __global__ __launch_bounds__(kTileThreads) void knn_tile_kernel(
     __shared__ int64_t s_rb[kRows];
     __shared__ int s_cnt[kRows];
     __shared__ int s_pref[kRows + 1];
+    __shared__ int s_ryz[kRows];  // y | z << 16 of each row
     __shared__ int s_ring[HALO + 2];
     const int tile = tiles[blockIdx.x];
     const int tx = tile % T.tdim[0], ty = (tile / T.tdim[0]) % T.tdim[1], tz = tile / (T.tdim[0] * T.tdim[1]);
@@ -697,8 +704,10 @@ __global__ __launch_bounds__(kTileThreads) void knn_tile_kernel(
     for (int h = 0; h < 2; ++h) {
         if (threadIdx.x + h * kTileThreads < nrows) {
             const int pos = s_ring[ring[h]] + rank[h];
+            const int r = threadIdx.x + h * kTileThreads;
             s_rb[pos] = rb[h];
             s_cnt[pos] = rc[h];
+            s_ryz[pos] = (y0 + r % ny) | ((z0 + r / ny) << 16);
         }
     }
     __syncthreads();
@@ -708,6 +717,7 @@ __global__ __launch_bounds__(kTileThreads) void knn_tile_kernel(
     }
     __syncthreads();
     const int total = s_pref[nrows];
+    const float rxlo = __builtin_fmaf(float(x0), G.cell, G.lo[0]), rxhi = __builtin_fmaf(float(x1 + 1), G.cell, G.lo[0]);
     // faces of the staged box that are not on the grid boundary
     const float flx = x0 > 0 ? G.lo[0] + float(x0) * G.cell : -INFINITY;
     const float fhx = x1 + 1 < G.dim[0] ? G.lo[0] + float(x1 + 1) * G.cell : INFINITY;
@@ -741,18 +751,42 @@ __global__ __launch_bounds__(kTileThreads) void knn_tile_kernel(
                 s_pts[j] = p;
             }
             __syncthreads();
-            if (active) {
-                for (int j = 0; j < cnt; ++j) {
-                    const float4 p = s_pts[j];
-                    const float d2 = dist2(qx - p.x, qy - p.y, qz - p.z);
-                    if (!(d2 <= kd2)) continue;
-                    const int idx = __float_as_int(p.w);
-                    if (idx < 0) continue;
-                    const uint64_t key = (uint64_t(__float_as_uint(d2)) << 32) | uint32_t(idx);
-                    if (key < best[0]) {
-                        knn_insert<K>(best, key);
-                        kd2 = best[0] == ~0ull ? INFINITY : __uint_as_float(uint32_t(best[0] >> 32));
-                    }
+            auto candidate = [&](const float4 p) {
+                const float d2 = dist2(qx - p.x, qy - p.y, qz - p.z);
+                if (!(d2 <= kd2)) return;
+                const int idx = __float_as_int(p.w);
+                if (idx < 0) return;
+                const uint64_t key = (uint64_t(__float_as_uint(d2)) << 32) | uint32_t(idx);
+                if (key < best[0]) {
+                    knn_insert<K>(best, key);
+                    kd2 = best[0] == ~0ull ? INFINITY : __uint_as_float(uint32_t(best[0] >> 32));
+                }
+            };
+            if (active && HALO == kHalo1) {
+                // the first pass's small region is nearly all needed
+                for (int j = 0; j < cnt; ++j) candidate(s_pts[j]);
+            } else if (active) {
+                // row by row in ring order: a row whose box of cells is
+                // farther than the k-th distance of every lane of the wave
+                // is skipped whole
+                int r = 0;
+                for (int hi = nrows - 1; r < hi;) {  // the row holding point cb
+                    const int mid = (r + hi + 1) >> 1;
+                    if (s_pref[mid] <= cb) r = mid;
+                    else hi = mid - 1;
+                }
+                for (; r < nrows; ++r) {
+                    const int a = max(s_pref[r], cb), b = min(s_pref[r + 1], cb + cnt);
+                    if (a >= cb + cnt) break;
+                    if (a >= b) continue;
+                    const int yz = s_ryz[r];
+                    const float ylo = __builtin_fmaf(float(yz & 0xFFFF), G.cell, G.lo[1]);
+                    const float zlo = __builtin_fmaf(float(yz >> 16), G.cell, G.lo[2]);
+                    const float gy = fmaxf(fmaxf(ylo - qy, qy - (ylo + G.cell)) - T.margin, 0.f);
+                    const float gz = fmaxf(fmaxf(zlo - qz, qz - (zlo + G.cell)) - T.margin, 0.f);
+                    const float gx = fmaxf(fmaxf(rxlo - qx, qx - rxhi) - T.margin, 0.f);
+                    if (!__builtin_amdgcn_ballot_w64(dist2(gx, gy, gz) <= kd2)) continue;
+                    for (int j = a - cb; j < b - cb; ++j) candidate(s_pts[j]);
                 }
             }
         }
@@ -1419,6 +1453,55 @@ static void launch_per_lane(dwg_ctx* ctx, const dwg_knn_index* I, const float* q
     check_launch();
 }
 
+// The queries qlist[0..nq) (or 0..nq−1, masked by qmask) bucketed by tile
+// of kTile³ fine cells: order[start[t]..start[t+1]) are tile t's queries,
+// list[0..nlist) the tiles that have any
+struct TileBuckets {
+    TileArgs T;
+    int64_t ntiles = 0, nlist = 0;
+    int64_t* start = nullptr;
+    int* order = nullptr;
+    int* list = nullptr;
+};
+
+static TileBuckets tile_buckets(dwg_ctx* ctx, const dwg_knn_index* I, const float* q, int64_t nq, const int* qlist,
+                                const uint8_t* qmask) {
+    const KnnGridArgs& G = I->lvl[0].G;
+    TileBuckets B;
+    TileArgs& T = B.T;
+    for (int d = 0; d < 3; ++d) T.tdim[d] = (G.dim[d] + kTile - 1) / kTile;
+    T.margin = cell_margin(G);
+    const int64_t ntiles = int64_t(T.tdim[0]) * T.tdim[1] * T.tdim[2];
+    B.ntiles = ntiles;
+    int* tile_of = static_cast<int*>(ctx->buf("knnt_tile_of", sizeof(int) * nq));
+    int64_t* start = static_cast<int64_t*>(ctx->buf("knnt_start", sizeof(int64_t) * (ntiles + 1)));
+    unsigned long long* cursor = static_cast<unsigned long long*>(ctx->buf("knnt_cursor", sizeof(uint64_t) * ntiles));
+    int* order = static_cast<int*>(ctx->buf("knnt_order", sizeof(int) * nq));
+    int64_t* flags = static_cast<int64_t*>(ctx->buf("knnt_flags", sizeof(int64_t) * (ntiles + 1)));
+    int* list = static_cast<int*>(ctx->buf("knnt_list", sizeof(int) * ntiles));
+    fill_u32(ctx, start, 0, 2 * (ntiles + 1));
+    hipLaunchKernelGGL(tile_count_kernel, dim3(grid_for(nq, 256)), dim3(256), 0, ctx->stream, q, nq, qlist, qmask, G, T,
+                       tile_of, reinterpret_cast<unsigned long long*>(start));
+    check_launch();
+    exclusive_scan<int64_t>(ctx, start, ntiles);
+    hipLaunchKernelGGL(copy_i64_kernel, dim3(grid_for(ntiles, 256)), dim3(256), 0, ctx->stream, start, cursor, ntiles);
+    check_launch();
+    hipLaunchKernelGGL(tile_scatter_kernel, dim3(grid_for(nq, 256)), dim3(256), 0, ctx->stream, tile_of, nq, qlist,
+                       cursor, order);
+    check_launch();
+    hipLaunchKernelGGL(tile_flags_kernel, dim3(grid_for(ntiles + 1, 256)), dim3(256), 0, ctx->stream, start, ntiles,
+                       flags);
+    check_launch();
+    exclusive_scan<int64_t>(ctx, flags, ntiles);
+    hipLaunchKernelGGL(tile_list_kernel, dim3(grid_for(ntiles, 256)), dim3(256), 0, ctx->stream, flags, ntiles, list);
+    check_launch();
+    B.nlist = read_scalar<int64_t>(ctx, flags + ntiles);
+    B.start = start;
+    B.order = order;
+    B.list = list;
+    return B;
+}
+
 // the queries a tiled pass leaves open, gathered in tile order
 __global__ void redo_compact_kernel(const int64_t* __restrict__ pos, const int* __restrict__ tmp_q,
                                     const float* __restrict__ tmp_kth, int64_t n, int* __restrict__ redo,
@@ -1439,34 +1522,14 @@ static int64_t tiled_pass(dwg_ctx* ctx, const dwg_knn_index* I, const float* q, 
                           int* redo, const std::string& name, const KnnScatter* sc = nullptr,
                           float* redo_kth = nullptr, bool ordered = false) {
     const KnnGridArgs& G = I->lvl[0].G;
-    TileArgs T;
-    for (int d = 0; d < 3; ++d) T.tdim[d] = (G.dim[d] + kTile - 1) / kTile;
-    const int64_t ntiles = int64_t(T.tdim[0]) * T.tdim[1] * T.tdim[2];
-    int* tile_of = static_cast<int*>(ctx->buf("knnt_tile_of", sizeof(int) * nq));
-    int64_t* start = static_cast<int64_t*>(ctx->buf("knnt_start", sizeof(int64_t) * (ntiles + 1)));
-    unsigned long long* cursor = static_cast<unsigned long long*>(ctx->buf("knnt_cursor", sizeof(uint64_t) * ntiles));
-    int* order = static_cast<int*>(ctx->buf("knnt_order", sizeof(int) * nq));
-    int64_t* flags = static_cast<int64_t*>(ctx->buf("knnt_flags", sizeof(int64_t) * (ntiles + 1)));
-    int* list = static_cast<int*>(ctx->buf("knnt_list", sizeof(int) * ntiles));
+    const TileBuckets B = tile_buckets(ctx, I, q, nq, qlist, qmask);
+    const TileArgs& T = B.T;
+    const int64_t ntiles = B.ntiles, nlist = B.nlist;
+    const int64_t* start = B.start;
+    const int* order = B.order;
+    const int* list = B.list;
     unsigned int* n_redo = static_cast<unsigned int*>(ctx->buf("knnt_nredo", 64));
-    fill_u32(ctx, start, 0, 2 * (ntiles + 1));
     fill_u32(ctx, n_redo, 0, 1);
-    hipLaunchKernelGGL(tile_count_kernel, dim3(grid_for(nq, 256)), dim3(256), 0, ctx->stream, q, nq, qlist, qmask, G, T,
-                       tile_of, reinterpret_cast<unsigned long long*>(start));
-    check_launch();
-    exclusive_scan<int64_t>(ctx, start, ntiles);
-    hipLaunchKernelGGL(copy_i64_kernel, dim3(grid_for(ntiles, 256)), dim3(256), 0, ctx->stream, start, cursor, ntiles);
-    check_launch();
-    hipLaunchKernelGGL(tile_scatter_kernel, dim3(grid_for(nq, 256)), dim3(256), 0, ctx->stream, tile_of, nq, qlist,
-                       cursor, order);
-    check_launch();
-    hipLaunchKernelGGL(tile_flags_kernel, dim3(grid_for(ntiles + 1, 256)), dim3(256), 0, ctx->stream, start, ntiles,
-                       flags);
-    check_launch();
-    exclusive_scan<int64_t>(ctx, flags, ntiles);
-    hipLaunchKernelGGL(tile_list_kernel, dim3(grid_for(ntiles, 256)), dim3(256), 0, ctx->stream, flags, ntiles, list);
-    check_launch();
-    const int64_t nlist = read_scalar<int64_t>(ctx, flags + ntiles);
     if (ctx->profile) ctx->count(name + ":queries", read_scalar<int64_t>(ctx, start + ntiles));
     // ordered: each query's place t in tile order gets its flag (and, when
     // open, its index and bound) in scratch arrays, compacted below
