@@ -150,6 +150,15 @@ __device__ __forceinline__ float fast_sqrt(float x) { return __builtin_amdgcn_sq
 __device__ __forceinline__ float fast_rcp(float x) { return __builtin_amdgcn_rcpf(x); }
 __device__ __forceinline__ float fast_exp2(float x) { return __builtin_amdgcn_exp2f(x); }
 
+// Block index remap so that each XCD (blocks are dealt round-robin over
+// the 8 XCDs) gets a contiguous run of block indices: neighbouring bricks
+// or query chunks then share one L2
+__device__ __forceinline__ int64_t xcd_block(int64_t b, int64_t nb) {
+    const int64_t per = nb / 8;
+    if (b >= per * 8) return b;  // tail keeps its slot
+    return (b % 8) * per + b / 8;
+}
+
 __device__ __forceinline__ int uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
 __device__ __forceinline__ uint64_t uniform64(uint64_t v) {
     uint32_t lo = __builtin_amdgcn_readfirstlane(uint32_t(v));

@@ -495,14 +495,6 @@ struct QueryArgs {
 };
 
 
-// Block index remap so each XCD (blocks are dealt round-robin over 8 XCDs)
-// gets a contiguous run of spatially adjacent bricks / query chunks.
-__device__ __forceinline__ int64_t xcd_block(int64_t b, int64_t nb) {
-    const int64_t per = nb / 8;
-    if (b >= per * 8) return b;  // tail keeps its slot
-    return (b % 8) * per + b / 8;
-}
-
 template <bool SCREENED, bool COUNT, bool BF16, bool GRID, bool NEAR64 = false>
 __global__ __launch_bounds__(kTreeThreads) void gwn_tree_kernel(TreeArgs T, QueryArgs Q, float beta, float sigma,
                                                                  uint32_t flags, float* __restrict__ out,

@@ -36,24 +36,34 @@ struct McArgs {
     float ox, oy, oz, h, iso;
 };
 
-__device__ __forceinline__ int cube_case(const McArgs& A, int x, int y, int z, float (&v)[8]) {
+// the 4 corners of face z of the cubes in column (x, y): face corner j is
+// at (corner_dx(j), corner_dy(j)); a cube's corners 0-3 lie on its face
+// z + 1, corners 4-7 on its face z
+__device__ __forceinline__ void load_face(const McArgs& A, int x, int y, int z, float (&f)[4]) {
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+        f[j] = A.f[int64_t(x + dwg_mc::corner_dx(j)) +
+                   int64_t(A.rx) * (int64_t(y + dwg_mc::corner_dy(j)) + int64_t(A.ry) * z)];
+}
+
+__device__ __forceinline__ int cube_case(const McArgs& A, const float (&lo)[4], const float (&hi)[4], float (&v)[8]) {
     int c = 0;
 #pragma unroll
-    for (int i = 0; i < 8; ++i) {
-        const int mx = x + dwg_mc::corner_dx(i), my = y + dwg_mc::corner_dy(i), mz = z + dwg_mc::corner_dz(i);
-        v[i] = A.f[int64_t(mx) + int64_t(A.rx) * (int64_t(my) + int64_t(A.ry) * mz)];
-        c |= (v[i] > A.iso) ? (1 << i) : 0;
+    for (int j = 0; j < 4; ++j) {
+        v[j] = hi[j];
+        v[4 + j] = lo[j];
     }
+#pragma unroll
+    for (int i = 0; i < 8; ++i) c |= (v[i] > A.iso) ? (1 << i) : 0;
     return c;
 }
 
-// One block per brick of 8 × 8 × 4 cubes (x fastest inside the brick,
-// bricks in x, y, z order): a surface crosses few bricks, so the emit pass
-// returns at once from the many bricks without triangles.
-// a brick is 8 × 8 × 16 cubes: each of the 256 threads takes a column of
-// kZPer = 4 cubes along z, so a launch has a quarter of the blocks a cube
-// per thread would need (short blocks are bound by their dispatch), and
-// consecutive cubes of a column share 4 corner loads
+// A block takes a brick of 8 × 8 × 16 cubes, each thread a column of 4
+// cubes along z whose shared faces it loads once.  Bricks are numbered x,
+// y, z fastest first and dealt to the XCDs in contiguous runs
+// (xcd_block), so the rows two neighbouring bricks share come from one L2.
+// (Bricks of whole 64-cube rows read the grid in longer runs but leave a
+// wave few lanes with triangles to write: the emit pass doubles.)
 constexpr int kBx = 8, kBy = 8, kZPer = 4, kBz = 4 * kZPer;
 static_assert(kBx * kBy * (kBz / kZPer) == kMcThreads, "one column of kZPer cubes per thread");
 
@@ -61,16 +71,30 @@ struct McBricks {
     int nbx, nby, nbz;
 };
 
-// the first cube (x, y, z0) of this thread's column
-__device__ __forceinline__ void column_of(const McBricks& B, int& x, int& y, int& z0) {
-    const int64_t b = blockIdx.x;
+__device__ __forceinline__ int64_t brick_of_block() { return xcd_block(blockIdx.x, gridDim.x); }
+
+__device__ __forceinline__ void column_of(const McBricks& B, int64_t b, int& x, int& y, int& z0) {
     x = int(b % B.nbx) * kBx + (threadIdx.x & (kBx - 1));
     y = int((b / B.nbx) % B.nby) * kBy + ((threadIdx.x / kBx) & (kBy - 1));
     z0 = int(b / (int64_t(B.nbx) * B.nby)) * kBz + (threadIdx.x / (kBx * kBy)) * kZPer;
 }
 
-__device__ __forceinline__ bool cube_valid(const McArgs& A, int x, int y, int z) {
-    return x < A.rx - 1 && y < A.ry - 1 && z < A.rz - 1;
+// the cases of a thread's column; 0 past the grid
+__device__ __forceinline__ void column_cases(const McArgs& A, int x, int y, int z0, int (&cases)[kZPer]) {
+    float lo[4] = {0.f, 0.f, 0.f, 0.f};
+    const bool col = x < A.rx - 1 && y < A.ry - 1;
+    if (col && z0 < A.rz - 1) load_face(A, x, y, z0, lo);
+#pragma unroll
+    for (int j = 0; j < kZPer; ++j) {
+        cases[j] = 0;
+        if (col && z0 + j < A.rz - 1) {
+            float hi[4], v[8];
+            load_face(A, x, y, z0 + j + 1, hi);
+            cases[j] = cube_case(A, lo, hi, v);
+#pragma unroll
+            for (int k = 0; k < 4; ++k) lo[k] = hi[k];
+        }
+    }
 }
 
 __device__ __forceinline__ int wave_inclusive_sum(int v) {
@@ -145,47 +169,43 @@ __device__ __forceinline__ void emit_cube(const McArgs& A, int x, int y, int z, 
 }
 
 __global__ __launch_bounds__(kMcThreads) void mc_count_kernel(McArgs A, McBricks B, int64_t* __restrict__ tile_counts) {
-    int x, y, z0, cnt = 0;
-    column_of(B, x, y, z0);
+    const int64_t b = brick_of_block();
+    int x, y, z0;
+    column_of(B, b, x, y, z0);
+    int cases[kZPer];
+    column_cases(A, x, y, z0, cases);
+    int cnt = 0;
 #pragma unroll
-    for (int j = 0; j < kZPer; ++j) {
-        if (cube_valid(A, x, y, z0 + j)) {
-            float v[8];
-            cnt += int(c_mc_rows[cube_case(A, x, y, z0 + j, v)] >> 60);
-        }
-    }
+    for (int j = 0; j < kZPer; ++j) cnt += int(c_mc_rows[cases[j]] >> 60);
     int total;
     (void)block_exclusive_sum(cnt, total);
-    if (threadIdx.x == 0) tile_counts[blockIdx.x] = total;
+    if (threadIdx.x == 0) tile_counts[b] = total;
 }
 
 __global__ __launch_bounds__(kMcThreads) void mc_emit_kernel(McArgs A, McBricks B, const int64_t* __restrict__ tile_offsets,
                                                              float* __restrict__ tris, float* __restrict__ cent,
                                                              float* __restrict__ nrm) {
-    if (tile_offsets[blockIdx.x + 1] == tile_offsets[blockIdx.x]) return;  // no triangle in this brick
-    int x, y, z0, cnt = 0;
-    column_of(B, x, y, z0);
+    const int64_t b = brick_of_block();
+    if (tile_offsets[b + 1] == tile_offsets[b]) return;  // no triangle in this brick
+    int x, y, z0;
+    column_of(B, b, x, y, z0);
     int cases[kZPer];
+    column_cases(A, x, y, z0, cases);
+    int cnt = 0;
 #pragma unroll
-    for (int j = 0; j < kZPer; ++j) {
-        cases[j] = 0;
-        if (cube_valid(A, x, y, z0 + j)) {
-            float v[8];
-            cases[j] = cube_case(A, x, y, z0 + j, v);
-            cnt += int(c_mc_rows[cases[j]] >> 60);
-        }
-    }
+    for (int j = 0; j < kZPer; ++j) cnt += int(c_mc_rows[cases[j]] >> 60);
     int total;
-    int64_t o = tile_offsets[blockIdx.x] + block_exclusive_sum(cnt, total);
+    int64_t o = tile_offsets[b] + block_exclusive_sum(cnt, total);
     if (cnt == 0) return;
     for (int j = 0; j < kZPer; ++j) {  // the column's cubes in z order, each in table order
-        const int z = z0 + j;
         const uint64_t row = c_mc_rows[cases[j]];
         const int nt = int(row >> 60);
         if (nt == 0) continue;
-        float v[8];
-        (void)cube_case(A, x, y, z, v);
-        emit_cube(A, x, y, z, v, row, nt, o, tris, cent, nrm);
+        float lo[4], hi[4], v[8];  // its corners again: few cubes have triangles
+        load_face(A, x, y, z0 + j, lo);
+        load_face(A, x, y, z0 + j + 1, hi);
+        (void)cube_case(A, lo, hi, v);
+        emit_cube(A, x, y, z0 + j, v, row, nt, o, tris, cent, nrm);
         o += nt;
     }
 }

@@ -2,8 +2,9 @@
 
 The GPU emits one brick of 8 × 8 × 16 cubes per block: bricks in x, y, z
 order; inside a brick, threads in x, y, z order, each thread a column of 4
-cubes along z in z order; each cube's triangles in table order.  The oracle emits in plain cube order (x fastest).  `to_brick_order`
-returns the permutation that takes the oracle's triangles to the GPU's.
+cubes along z in z order; each cube's triangles in table order.  The
+oracle emits in plain cube order (x fastest).  `to_brick_order` returns
+the permutation that takes the oracle's triangles to the GPU's.
 """
 import numpy as np
 
