@@ -310,6 +310,29 @@ __global__ void box_up_kernel(const float4* __restrict__ lower, int dx, int dy, 
     out[2 * b + 1] = hi;
 }
 
+// the points' bounding boxes of every cell and pyramid block, built the
+// first time a walk needs them (only the wave-shared walk does)
+static void ensure_boxes(dwg_ctx* ctx, dwg_knn_index::Level& L) {
+    if (L.pbox) return;
+    const int levels = L.pyr_levels;
+    const int4* dims = L.pyr_dim_h;
+    const int64_t* off = L.pyr_off_h;
+    int64_t total = 0;
+    for (int l = 1; l < levels; ++l) total += int64_t(dims[l].x) * dims[l].y * dims[l].z;
+    DWG_HIP_CHECK(hipMalloc(&L.pbox, sizeof(float4) * 2 * size_t(L.ncells + total)));
+    hipLaunchKernelGGL(cell_box_kernel, dim3(grid_for(L.ncells, 256)), dim3(256), 0, ctx->stream, L.cell_start, L.pts,
+                       L.ncells, L.pbox);
+    check_launch();
+    for (int l = 1; l < levels; ++l) {
+        const int4 lo = dims[l - 1], up = dims[l];
+        const int64_t nb = int64_t(up.x) * up.y * up.z;
+        const float4* lower = L.pbox + 2 * (l == 1 ? 0 : L.ncells + off[l - 1]);
+        hipLaunchKernelGGL(box_up_kernel, dim3(grid_for(nb, 256)), dim3(256), 0, ctx->stream, lower, lo.x, lo.y, lo.z,
+                           up.x, up.y, up.z, L.pbox + 2 * (L.ncells + off[l]));
+        check_launch();
+    }
+}
+
 // cell boundaries are computed as lo + b·w in float and points are
 // assigned by floor((v − lo)/cell): both are within a few ulp of the exact
 // boundary, so a box of cells is shrunk by 16 ulp of the largest coordinate
@@ -352,18 +375,6 @@ static void pyramid(dwg_ctx* ctx, dwg_knn_index::Level& L) {
         else
             hipLaunchKernelGGL(pyr_up_kernel, dim3(grid_for(nb, 256)), dim3(256), 0, ctx->stream, L.pyr + off[l - 1],
                                lo.x, lo.y, lo.z, up.x, up.y, up.z, L.pyr + off[l]);
-        check_launch();
-    }
-    DWG_HIP_CHECK(hipMalloc(&L.pbox, sizeof(float4) * 2 * size_t(L.ncells + total)));
-    hipLaunchKernelGGL(cell_box_kernel, dim3(grid_for(L.ncells, 256)), dim3(256), 0, ctx->stream, L.cell_start, L.pts,
-                       L.ncells, L.pbox);
-    check_launch();
-    for (int l = 1; l < levels; ++l) {
-        const int4 lo = dims[l - 1], up = dims[l];
-        const int64_t nb = int64_t(up.x) * up.y * up.z;
-        const float4* lower = L.pbox + 2 * (l == 1 ? 0 : L.ncells + off[l - 1]);
-        hipLaunchKernelGGL(box_up_kernel, dim3(grid_for(nb, 256)), dim3(256), 0, ctx->stream, lower, lo.x, lo.y, lo.z,
-                           up.x, up.y, up.z, L.pbox + 2 * (L.ncells + off[l]));
         check_launch();
     }
     L.pyr_margin = cell_margin(L.G);
@@ -1405,6 +1416,9 @@ static void launch_far(dwg_ctx* ctx, const dwg_knn_index* I, const float* q, int
         launch_wave(ctx, I, q, nq, qlist, k, skip, out_idx, out_d2);
         return;
     }
+    // the boxes are a cache of the index's own points: building them
+    // leaves the index unchanged
+    if (shared) ensure_boxes(ctx, const_cast<dwg_knn_index::Level&>(L));
     const PyrArgs Y{L.pyr, L.pyr_dim, L.pyr_off, L.pyr_levels, L.pyr_margin, L.pbox, L.ncells};
     if (shared) {
         DWG_REQUIRE(L.pyr_levels <= kMaxPyr && 7 * L.pyr_levels + 1 <= kPwStack, "pyramid deeper than the wave stack");
