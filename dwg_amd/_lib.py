@@ -30,7 +30,8 @@ EXPORTS = [
     "dwg_ctx_profile_get", "dwg_ctx_profile_reset", "dwg_ctx_counter_get", "dwg_gwn_direct",
     "dwg_gwn_direct_noalloc", "dwg_octree_build", "dwg_octree_build_device",
     "dwg_octree_leaves", "dwg_octree_leaf_normals",
-    "dwg_octree_destroy", "dwg_octree_info", "dwg_octree_update", "dwg_octree_set_point_areas", "dwg_gwn_tree", "dwg_gwn_tree_grad", "dwg_gwn_tree_self", "dwg_gwn_tree_grid",
+    "dwg_octree_destroy", "dwg_octree_info", "dwg_octree_point_order", "dwg_octree_update",
+    "dwg_octree_set_point_areas", "dwg_gwn_tree", "dwg_gwn_tree_grad", "dwg_gwn_tree_self", "dwg_gwn_tree_grid",
     "dwg_gwn_tree_terms", "dwg_gwn_tree_grid_terms", "dwg_knn_build", "dwg_knn_destroy", "dwg_knn_query",
     "dwg_marching_cubes", "dwg_diffuse_normals", "dwg_radii", "dwg_point_areas", "dwg_pca_normals", "dwg_gauss_map", "dwg_transfer_normals",
     "dwg_outlier_filter", "dwg_normal_change", "dwg_reduce_sum", "dwg_bounds", "dwg_options_default", "dwg_launch",
@@ -108,6 +109,7 @@ def load():
     L.dwg_octree_leaf_normals.argtypes = [_vp, _vp, _int, _vp, _vp, _vp, P(_i64)]
     L.dwg_octree_destroy.restype = None
     L.dwg_octree_info.argtypes = [_vp, P(_i64), P(_i64), P(_int)]
+    L.dwg_octree_point_order.argtypes = [_vp, _vp, _vp]
     L.dwg_octree_update.argtypes = [_vp, _vp, _vp, _vp, _vp]
     L.dwg_octree_set_point_areas.argtypes = [_vp, _vp, _vp, _vp]
     L.dwg_gwn_tree.argtypes = [_vp, _vp, _vp, _i64, _f, _f, _u32, _vp]

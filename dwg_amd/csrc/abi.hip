@@ -221,6 +221,16 @@ int dwg_octree_info(const dwg_octree* tree, int64_t* n_nodes, int64_t* n_leaves,
     return DWG_OK;
 }
 
+int dwg_octree_point_order(dwg_ctx* ctx, const dwg_octree* tree, int* d_order) {
+    if (!ctx || !tree) return DWG_ERR_ARG;
+    return guarded(ctx, [&] {
+        DWG_REQUIRE(d_order || tree->n_points == 0, "null output");
+        set_device(ctx);
+        DWG_HIP_CHECK(hipMemcpyAsync(d_order, tree->perm, sizeof(int) * size_t(tree->n_points),
+                                     hipMemcpyDeviceToDevice, ctx->stream));
+    });
+}
+
 int dwg_octree_update(dwg_ctx* ctx, dwg_octree* tree, const float* d_nrm, const float* d_area, const float* d_radii) {
     if (!ctx || !tree) return DWG_ERR_ARG;
     return guarded(ctx, [&] {

@@ -21,7 +21,6 @@
 //    the node (order-free), taken by a walk from the root per point with
 //    wave-aggregated 64-bit atomic max on the IEEE bits (monotone for
 //    non-negative doubles).
-#include <rocprim/device/device_radix_sort.hpp>
 
 #include <algorithm>
 #include <cmath>
@@ -412,13 +411,7 @@ dwg_octree* octree_build(dwg_ctx* ctx, const double* P, const double* A, int64_t
         hipLaunchKernelGGL(octree_codes_kernel, dim3(grid_for(n, 256)), dim3(256), 0, ctx->stream, P, n, c0[0], c0[1],
                            c0[2], w0, max_depth, code_in, idx_in);
         check_launch();
-        const int bits = std::max(1, 3 * max_depth);
-        size_t temp_bytes = 0;
-        DWG_HIP_CHECK(rocprim::radix_sort_pairs(nullptr, temp_bytes, code_in, code, idx_in, T->perm, size_t(n), 0, bits,
-                                                ctx->stream));
-        void* temp = ctx->buf("oct_sort_temp", temp_bytes);
-        DWG_HIP_CHECK(rocprim::radix_sort_pairs(temp, temp_bytes, code_in, code, idx_in, T->perm, size_t(n), 0, bits,
-                                                ctx->stream));
+        radix_sort_pairs(ctx, code_in, idx_in, code, T->perm, n, std::max(1, 3 * max_depth));
 
         // 2. nodes, one level at a time
         int64_t cap = std::max<int64_t>(1024, 2 * n + 9);

@@ -16,6 +16,12 @@ struct Double3 {
 template <class T>
 void exclusive_scan(dwg_ctx* ctx, T* d, int64_t n, int level = 0);
 
+// Stable sort of n (key, value) pairs by the low `bits` bits of the key
+// (higher bits must be 0): LSD radix, 8 bits a pass.  kout/vout must not
+// alias kin/vin.
+void radix_sort_pairs(dwg_ctx* ctx, const uint64_t* kin, const int* vin, uint64_t* kout, int* vout, int64_t n,
+                      int bits);
+
 // FP64 sum of a float array (synchronises, returns the host value).
 double reduce_sum_f32(dwg_ctx* ctx, const float* d, int64_t n);
 

@@ -115,7 +115,109 @@ __global__ void reduce_final_f64(const double* __restrict__ partial, int n, doub
     if (threadIdx.x == 0) *out = lds[0];
 }
 
+// Stable LSD radix sort of (uint64 key, int value) pairs, 8 bits a pass.
+// A block owns a tile of kSortTile consecutive items, taken in rounds of
+// 256 (item r·256 + t of the tile in round r, thread t).  Pass: every block
+// counts its tile's digits; one exclusive scan over the counts laid out
+// digit-major ([digit][block]) gives each (digit, block) its first output
+// slot; every block then writes its items to slot + rank, the rank of an
+// item among its tile's items with the same digit that precede it (earlier
+// rounds, earlier waves of its round, lower lanes of its wave: the wave's
+// lanes with the same digit come from 8 ballots), so equal digits keep their
+// input order and the passes compose into a stable sort.
+constexpr int kSortThreads = 256, kSortRounds = 16, kSortTile = kSortThreads * kSortRounds;
+static_assert(kSortThreads == 256, "one thread per digit value");
+
+__global__ __launch_bounds__(kSortThreads) void radix_count_kernel(const uint64_t* __restrict__ keys, int64_t n,
+                                                                   int shift, int64_t nblocks,
+                                                                   int64_t* __restrict__ counts) {
+    __shared__ unsigned s_h[256];
+    s_h[threadIdx.x] = 0;
+    __syncthreads();
+    const int64_t base = int64_t(blockIdx.x) * kSortTile;
+    for (int r = 0; r < kSortRounds; ++r) {
+        const int64_t i = base + r * kSortThreads + threadIdx.x;
+        if (i < n) atomicAdd(&s_h[(keys[i] >> shift) & 255u], 1u);
+    }
+    __syncthreads();
+    counts[int64_t(threadIdx.x) * nblocks + blockIdx.x] = s_h[threadIdx.x];
+}
+
+__global__ __launch_bounds__(kSortThreads) void radix_scatter_kernel(const uint64_t* __restrict__ kin,
+                                                                     const int* __restrict__ vin, int64_t n, int shift,
+                                                                     int64_t nblocks, const int64_t* __restrict__ first,
+                                                                     uint64_t* __restrict__ kout,
+                                                                     int* __restrict__ vout) {
+    constexpr int kWaves = kSortThreads / 64;
+    __shared__ int s_run[256];          // items of each digit in the tile's earlier rounds
+    __shared__ int s_wave[kWaves][256];  // items of each digit in each wave of this round
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    s_run[threadIdx.x] = 0;
+    for (int w = 0; w < kWaves; ++w) s_wave[w][threadIdx.x] = 0;
+    __syncthreads();
+    const int64_t base = int64_t(blockIdx.x) * kSortTile;
+    for (int r = 0; r < kSortRounds; ++r) {
+        const int64_t i = base + r * kSortThreads + threadIdx.x;
+        const bool ok = i < n;
+        const uint64_t k = ok ? kin[i] : 0;
+        const int v = ok ? vin[i] : 0;
+        const int dg = int((k >> shift) & 255u);
+        uint64_t same = __builtin_amdgcn_ballot_w64(ok);
+#pragma unroll
+        for (int b = 0; b < 8; ++b) {
+            const uint64_t m = __builtin_amdgcn_ballot_w64((dg >> b) & 1);
+            same &= ((dg >> b) & 1) ? m : ~m;
+        }
+        const int below = __builtin_popcountll(same & ((1ull << lane) - 1ull));
+        if (ok && below == 0) s_wave[wave][dg] = __builtin_popcountll(same);
+        __syncthreads();
+        if (ok) {
+            int rank = s_run[dg] + below;
+            for (int w = 0; w < wave; ++w) rank += s_wave[w][dg];
+            const int64_t dst = first[int64_t(dg) * nblocks + blockIdx.x] + rank;
+            kout[dst] = k;
+            vout[dst] = v;
+        }
+        __syncthreads();
+        int add = 0;
+        for (int w = 0; w < kWaves; ++w) {
+            add += s_wave[w][threadIdx.x];
+            s_wave[w][threadIdx.x] = 0;
+        }
+        s_run[threadIdx.x] += add;
+        __syncthreads();
+    }
+}
+
 }  // namespace
+
+void radix_sort_pairs(dwg_ctx* ctx, const uint64_t* kin, const int* vin, uint64_t* kout, int* vout, int64_t n,
+                      int bits) {
+    DWG_REQUIRE(n >= 0 && bits >= 1 && bits <= 64, "bad radix sort request");
+    if (n == 0) return;
+    const int64_t nblocks = (n + kSortTile - 1) / kSortTile;
+    int64_t* first = static_cast<int64_t*>(ctx->buf("sort_first", sizeof(int64_t) * (256 * nblocks + 1)));
+    uint64_t* kt[2] = {static_cast<uint64_t*>(ctx->buf("sort_k0", sizeof(uint64_t) * n)),
+                       static_cast<uint64_t*>(ctx->buf("sort_k1", sizeof(uint64_t) * n))};
+    int* vt[2] = {static_cast<int*>(ctx->buf("sort_v0", sizeof(int) * n)),
+                  static_cast<int*>(ctx->buf("sort_v1", sizeof(int) * n))};
+    const int passes = (bits + 7) / 8;
+    const uint64_t* ks = kin;
+    const int* vs = vin;
+    for (int p = 0; p < passes; ++p) {
+        uint64_t* kd = p == passes - 1 ? kout : kt[p & 1];
+        int* vd = p == passes - 1 ? vout : vt[p & 1];
+        hipLaunchKernelGGL(radix_count_kernel, dim3(unsigned(nblocks)), dim3(kSortThreads), 0, ctx->stream, ks, n, 8 * p,
+                           nblocks, first);
+        check_launch();
+        exclusive_scan<int64_t>(ctx, first, 256 * nblocks);
+        hipLaunchKernelGGL(radix_scatter_kernel, dim3(unsigned(nblocks)), dim3(kSortThreads), 0, ctx->stream, ks, vs, n,
+                           8 * p, nblocks, first, kd, vd);
+        check_launch();
+        ks = kd;
+        vs = vd;
+    }
+}
 
 template <class T>
 void exclusive_scan(dwg_ctx* ctx, T* d, int64_t n, int level) {

@@ -146,6 +146,13 @@ class Octree:
         self.ctx.L.dwg_octree_info(self.h, ctypes.byref(m), ctypes.byref(l), ctypes.byref(d))
         return {"nodes": m.value, "leaves": l.value, "depth": d.value}
 
+    def point_order(self):
+        """Caller index of each point in tree order (dwg_octree_point_order)."""
+        c = self.ctx
+        out = torch.empty(max(self.n, 1), device=c.device, dtype=torch.int32)
+        c.check(c.L.dwg_octree_point_order(c.h, self.h, _ptr(out)))
+        return out[:self.n].cpu().numpy()
+
     def leaves(self, min_num=0):
         """Centres and widths of the leaf cells holding more than min_num
         points, breadth-first (GetOctreeLeaves)."""

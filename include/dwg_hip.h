@@ -121,6 +121,12 @@ int dwg_octree_leaves(dwg_ctx* ctx, const dwg_octree* tree, int min_num, double*
 int dwg_octree_leaf_normals(dwg_ctx* ctx, const dwg_octree* tree, int min_num, const double* d_pts,
                             const double* d_nrm, float* d_out, int64_t* out_count);
 int dwg_octree_info(const dwg_octree* tree, int64_t* n_nodes, int64_t* n_leaves, int* max_level);
+/* The tree's points in tree order (libigl octree's point_indices,
+ * octree.cpp:171-335, concatenated over the leaves): d_order (device,
+ * room for n) receives the caller index of each point, sorted by octant
+ * path with equal paths in caller order; every node's points are one
+ * contiguous run of it. */
+int dwg_octree_point_order(dwg_ctx* ctx, const dwg_octree* tree, int* d_order);
 /* Per-iteration refresh of the far-field moment EC = Σ A·n per node and of
  * the packed sources (TurboLibiGWN::UpdateMassNormals, TurboLibiGWN.cu:1051-1059).
  * d_radii may be NULL. */

@@ -68,3 +68,35 @@ def test_normalisation(ctx):
     assert np.allclose(r.center, c, rtol=0, atol=1e-12 * np.abs(c).max())
     s = (2 * np.abs(P.astype(np.float64) - np.asarray(r.center))).max()
     assert r.scale == s
+
+
+def _octant_paths(P, max_depth):
+    # each point's octant path from the root cell (centre of the bounding
+    # box, width its largest extent), 3 bits a level, child bit x | y << 1 |
+    # z << 2 set where the coordinate is ≥ the cell's centre: the split rule
+    # of octree.cpp:171-335 in FP64, the arithmetic the build's
+    # octree_codes_kernel uses
+    lo, hi = P.min(0), P.max(0)
+    c = np.tile((lo + hi) / 2, (len(P), 1))
+    w = np.full(len(P), (hi - lo).max())
+    k = np.zeros(len(P), np.uint64)
+    for _ in range(max_depth):
+        o = (P >= c).astype(np.uint64) << np.arange(3, dtype=np.uint64)
+        o = o.sum(1, dtype=np.uint64)
+        k = (k << np.uint64(3)) | o
+        q = w / 4
+        for d in range(3):
+            c[:, d] += np.where((o >> np.uint64(d)) & np.uint64(1), q, -q)
+        w = w / 2
+    return k
+
+
+@pytest.mark.parametrize("n,max_depth,dup", [(50_000, 10, 5000), (20_000, 21, 300), (300_000, 10, 0), (1, 10, 0)])
+def test_octree_point_order_is_stable_sort_by_path(ctx, n, max_depth, dup):
+    # the build's radix sort: points ordered by octant path, equal paths
+    # (duplicates, or points sharing a cell at max_depth) in caller order —
+    # numpy's stable argsort of the same paths
+    P, _ = _cloud(n, 40 + n % 11, dup)
+    t = gpu.Octree(ctx, P, max_depth=max_depth)
+    ref = np.argsort(_octant_paths(P, max_depth), kind="stable")
+    assert np.array_equal(t.point_order(), ref)
