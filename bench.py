@@ -50,6 +50,9 @@ FLOPS_PER_TERM = {"screened": 27, "plain": 20}
 # HBM bytes per launch of the iteration grid kernel, from the rocprofv3
 # FETCH_SIZE / WRITE_SIZE runs recorded in this summary (copy-calibrated)
 TRAFFIC_SUMMARY = os.path.join(ROOT, "profiles", "gwn_tree_grid_screened_bf16_1M_512_traffic.txt")
+# and of the marching-cubes passes on the same workload
+MC_TRAFFIC_SUMMARY = os.path.join(ROOT, "profiles", "mc_1M_512_traffic.txt")
+PEAK_HBM = 8000.0  # GB/s, MI355X HBM3E
 
 
 def log(*a):
@@ -167,16 +170,34 @@ def direct_sums(ctx=None, reps=5):
     return out
 
 
-def measured_traffic():
-    """`traffic_bytes_per_launch` from the committed counter summary, or None."""
+def measured_traffic(path=TRAFFIC_SUMMARY, key="traffic_bytes_per_launch"):
+    """`<key>: bytes` from a committed counter summary, or None."""
     try:
-        with open(TRAFFIC_SUMMARY) as f:
+        with open(path) as f:
             for line in f:
-                if line.startswith("traffic_bytes_per_launch:"):
+                if line.startswith(key + ":"):
                     return int(float(line.split(":", 1)[1]))
     except OSError:
         pass
     return None
+
+
+def hbm_rates(ktimes):
+    """HBM GB/s of the grid walk and the marching-cubes passes: the bytes per
+    launch that the FETCH_SIZE/WRITE_SIZE counters gave on this workload
+    (committed summaries) over this run's mean launch times (HIP events)."""
+    out = {}
+    for name, path, key in [("gwn_tree_grid_screened", TRAFFIC_SUMMARY, "traffic_bytes_per_launch"),
+                            ("mc_count", MC_TRAFFIC_SUMMARY, "mc_count_bytes_per_launch"),
+                            ("mc_emit", MC_TRAFFIC_SUMMARY, "mc_emit_bytes_per_launch")]:
+        b = measured_traffic(path, key)
+        if b is None or name not in ktimes:
+            continue
+        ms = ktimes[name][0] / ktimes[name][1]
+        gbs = b / (ms * 1e-3) / 1e9
+        out[name] = {"bytes_per_launch": b, "avg_launch_ms": round(ms, 3), "gb_per_s": round(gbs, 1),
+                     "frac_of_8000": round(gbs / PEAK_HBM, 4)}
+    return out
 
 
 def cpu_baseline(P, grid_reso, iters, normals=None, budget_s=20.0):
@@ -386,7 +407,8 @@ def main():
                 "achieved": round(achieved, 2), "peak": PEAK_FP32, "unit": "TFLOP/s",
                 "frac": round(achieved / PEAK_FP32, 4), "traffic": measured_traffic(),
                 "terms_per_launch": int(prof.terms_grid), "flops_per_term": FLOPS_PER_TERM["screened"],
-                "avg_launch_ms": round(avg_ms, 3), "launches": int(g_cnt), "dominant_by_time": dom}
+                "avg_launch_ms": round(avg_ms, 3), "launches": int(g_cnt), "dominant_by_time": dom,
+                "hbm": hbm_rates(ktimes)}
     q_plain = q_learned = passed = None
     at_gate = {}
     if not args.no_quality:
