@@ -58,15 +58,6 @@ __global__ void nonzero_flags_kernel(const uint8_t* __restrict__ zero, int64_t n
     slot[i] = (i < n && !zero[i]) ? 1 : 0;
 }
 
-__global__ void nonzero_gather_kernel(const int64_t* __restrict__ slot, const float* __restrict__ p, int64_t n,
-                                      float* __restrict__ out, int32_t* __restrict__ index) {
-    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (i >= n || slot[i + 1] == slot[i]) return;
-    const int64_t j = slot[i];
-    for (int d = 0; d < 3; ++d) out[3 * j + d] = p[3 * i + d];
-    index[j] = int32_t(i);
-}
-
 // the zero sources, compacted: their indices and positions (slot[i] counts
 // the non-zero sources before i, so i − slot[i] counts the zero ones)
 __global__ void zero_gather_kernel(const int64_t* __restrict__ slot, const float* __restrict__ p, int64_t n,
@@ -80,10 +71,9 @@ __global__ void zero_gather_kernel(const int64_t* __restrict__ slot, const float
 
 // SimpleUpdateNonzeroNormals (TurboDWG.cu:322-359): a zero source takes the
 // normalised sum of its nearest non-zero sources' normals.
-// knn row t belongs to zero source zlist[t] and holds positions in the list
-// of non-zero sources; nz_index maps them back to source indices.
+// knn row t belongs to zero source zlist[t] and holds source indices.
 __global__ void zero_fill_kernel(const int32_t* __restrict__ knn, int k, const int32_t* __restrict__ zlist,
-                                 int64_t nzero, const int32_t* __restrict__ nz_index, float* __restrict__ nrm) {
+                                 int64_t nzero, float* __restrict__ nrm) {
     const int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (t >= nzero) return;
     const int64_t i = zlist[t];
@@ -91,7 +81,7 @@ __global__ void zero_fill_kernel(const int32_t* __restrict__ knn, int k, const i
     for (int j = 0; j < k; ++j) {
         const int32_t c = knn[t * k + j];
         if (c < 0) continue;
-        const int64_t s = nz_index[c];
+        const int64_t s = c;
         sx += nrm[3 * s];
         sy += nrm[3 * s + 1];
         sz += nrm[3 * s + 2];
@@ -125,13 +115,11 @@ void diffuse_normals(dwg_ctx* ctx, const dwg_knn_index* I, const float* src, int
     const int64_t nzero = int64_t(read_scalar<unsigned long long>(ctx, nz));
     if (h_nzero) *h_nzero = nzero;
     if (nzero > zero_threshold && nzero < n) {
-        // an index over the non-zero sources only (same cell grids, kept in
-        // source order so (d², index) ties break as over the full set): the
-        // search never wades through zero sources, which cluster
-        const int64_t m = n - nzero;
+        // the zero sources query the sources' own index with the zero ones
+        // skipped (knn_query then walks a pyramid of the non-zero sources
+        // only, as the zero sources come in clusters); results are source
+        // indices, ties broken by source index as over the non-zero set
         int64_t* slot = static_cast<int64_t*>(ctx->buf("diff_nz_slot", sizeof(int64_t) * (n + 1)));
-        float* nzp = static_cast<float*>(ctx->buf("diff_nz_pts", sizeof(float) * 3 * m));
-        int32_t* nzi = static_cast<int32_t*>(ctx->buf("diff_nz_index", sizeof(int32_t) * m));
         float* zq = static_cast<float*>(ctx->buf("diff_z_pts", sizeof(float) * 3 * nzero));
         int32_t* zlist = static_cast<int32_t*>(ctx->buf("diff_z_list", sizeof(int32_t) * nzero));
         {
@@ -140,20 +128,15 @@ void diffuse_normals(dwg_ctx* ctx, const dwg_knn_index* I, const float* src, int
                                slot);
             check_launch();
             exclusive_scan<int64_t>(ctx, slot, n);
-            hipLaunchKernelGGL(nonzero_gather_kernel, dim3(grid_for(n, 256)), dim3(256), 0, ctx->stream, slot, src, n,
-                               nzp, nzi);
-            check_launch();
             hipLaunchKernelGGL(zero_gather_kernel, dim3(grid_for(n, 256)), dim3(256), 0, ctx->stream, slot, src, n, zq,
                                zlist);
             check_launch();
         }
-        std::unique_ptr<dwg_knn_index> J(knn_build_like(ctx, I, nzp, m));
-        // only the zero sources are queried
         int32_t* zknn = static_cast<int32_t*>(ctx->buf("diff_zknn", sizeof(int32_t) * nzero * zero_k));
-        knn_query(ctx, J.get(), zq, nzero, zero_k, nullptr, nullptr, zknn, nullptr, "knn_query_zero_fill");
+        knn_query(ctx, I, zq, nzero, zero_k, zero, nullptr, zknn, nullptr, "knn_query_zero_fill");
         ProfileScope ps(ctx, "diffuse_zero_fill");
         hipLaunchKernelGGL(zero_fill_kernel, dim3(grid_for(nzero, 256)), dim3(256), 0, ctx->stream, zknn, zero_k, zlist,
-                           nzero, nzi, nrm);
+                           nzero, nrm);
         check_launch();
     }
 }

@@ -51,8 +51,6 @@ struct dwg_knn_index {
 namespace dwg {
 void bounds(dwg_ctx* ctx, const float* p, int64_t n, float* lo, float* hi);
 dwg_knn_index* knn_build(dwg_ctx* ctx, const float* p, int64_t n, int k_hint);
-// an index over other points (inside the same bounds) on the cell grids of `like`
-dwg_knn_index* knn_build_like(dwg_ctx* ctx, const dwg_knn_index* like, const float* p, int64_t n);
 void knn_query(dwg_ctx* ctx, const dwg_knn_index* I, const float* q, int64_t nq, int k, const uint8_t* skip,
                const uint8_t* qmask, int32_t* out_idx, float* out_d2, const char* tag = nullptr,
                const KnnScatter* scatter = nullptr);

@@ -344,6 +344,52 @@ static float cell_margin(const KnnGridArgs& G) {
     return 16.f * 1.1920929e-7f * ext + 1e-5f * G.cell;
 }
 
+// the first pyramid level counting only points that `skip` keeps
+__global__ void pyr_first_masked_kernel(const int64_t* __restrict__ cell_start, const float4* __restrict__ pts,
+                                        const uint8_t* __restrict__ skip, int dx, int dy, int dz, int ux, int uy,
+                                        int uz, uint8_t* __restrict__ out) {
+    const int64_t b = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (b >= int64_t(ux) * uy * uz) return;
+    const int x = int(b % ux), y = int((b / ux) % uy), z = int(b / (int64_t(ux) * uy));
+    uint32_t m = 0;
+    for (int c = 0; c < 8; ++c) {
+        const int X = 2 * x + (c & 1), Y = 2 * y + ((c >> 1) & 1), Z = 2 * z + (c >> 2);
+        if (X < dx && Y < dy && Z < dz) {
+            const int64_t cell = X + int64_t(dx) * (Y + int64_t(dy) * Z);
+            for (int64_t s = cell_start[cell], e = cell_start[cell + 1]; s < e; ++s)
+                if (!skip[__float_as_int(pts[s].w)]) {
+                    m |= 1u << c;
+                    break;
+                }
+        }
+    }
+    out[b] = uint8_t(m);
+}
+
+// An occupancy pyramid of the fine level that ignores the points `skip`
+// drops (same layout as Level::pyr, in a context buffer): a walk with that
+// skip mask then never opens a block of skipped points only
+static const uint8_t* masked_pyramid(dwg_ctx* ctx, const dwg_knn_index::Level& L, const uint8_t* skip) {
+    const int levels = L.pyr_levels;
+    const int4* dims = L.pyr_dim_h;
+    const int64_t* off = L.pyr_off_h;
+    int64_t total = 0;
+    for (int l = 1; l < levels; ++l) total += int64_t(dims[l].x) * dims[l].y * dims[l].z;
+    uint8_t* pyr = static_cast<uint8_t*>(ctx->buf("knn_pyr_masked", size_t(total) + 64));
+    for (int l = 1; l < levels; ++l) {
+        const int4 lo = dims[l - 1], up = dims[l];
+        const int64_t nb = int64_t(up.x) * up.y * up.z;
+        if (l == 1)
+            hipLaunchKernelGGL(pyr_first_masked_kernel, dim3(grid_for(nb, 256)), dim3(256), 0, ctx->stream,
+                               L.cell_start, L.pts, skip, lo.x, lo.y, lo.z, up.x, up.y, up.z, pyr + off[1]);
+        else
+            hipLaunchKernelGGL(pyr_up_kernel, dim3(grid_for(nb, 256)), dim3(256), 0, ctx->stream, pyr + off[l - 1],
+                               lo.x, lo.y, lo.z, up.x, up.y, up.z, pyr + off[l]);
+        check_launch();
+    }
+    return pyr;
+}
+
 static void pyramid(dwg_ctx* ctx, dwg_knn_index::Level& L) {
     int4* dims = L.pyr_dim_h;
     int64_t* off = L.pyr_off_h;
@@ -418,25 +464,6 @@ dwg_knn_index* knn_build(dwg_ctx* ctx, const float* p, int64_t n, int k_hint) {
         bucket(ctx, p, n, I->lvl[0], true);
         I->lvl[1].G = grid_args(lo, hi, cell * 8.0);
         bucket(ctx, p, n, I->lvl[1], true);
-        empty_distance(ctx, I->lvl[1]);
-        pyramid(ctx, I->lvl[0]);
-    } catch (...) {
-        delete I;
-        throw;
-    }
-    return I;
-}
-
-dwg_knn_index* knn_build_like(dwg_ctx* ctx, const dwg_knn_index* like, const float* p, int64_t n) {
-    DWG_REQUIRE(like && n > 0 && n < (int64_t(1) << 31) - 1, "bad kNN index request");
-    auto* I = new dwg_knn_index();
-    try {
-        I->n = n;
-        ProfileScope ps(ctx, "knn_build");
-        for (int l = 0; l < 2; ++l) {
-            I->lvl[l].G = like->lvl[l].G;
-            bucket(ctx, p, n, I->lvl[l], true);
-        }
         empty_distance(ctx, I->lvl[1]);
         pyramid(ctx, I->lvl[0]);
     } catch (...) {
@@ -1409,7 +1436,7 @@ constexpr int64_t kSparseMaxQueries = 65536;
 
 static void launch_far(dwg_ctx* ctx, const dwg_knn_index* I, const float* q, int64_t nq, const int* qlist, int k,
                        const uint8_t* skip, int32_t* out_idx, float* out_d2, const float* init_kth = nullptr,
-                       bool shared = false) {
+                       bool shared = false, const uint8_t* pyr_mask = nullptr) {
     if (nq == 0) return;
     const auto& L = I->lvl[0];
     if (L.pyr_levels < 2 || nq < kPyrMinQueries) {
@@ -1419,7 +1446,7 @@ static void launch_far(dwg_ctx* ctx, const dwg_knn_index* I, const float* q, int
     // the boxes are a cache of the index's own points: building them
     // leaves the index unchanged
     if (shared) ensure_boxes(ctx, const_cast<dwg_knn_index::Level&>(L));
-    const PyrArgs Y{L.pyr, L.pyr_dim, L.pyr_off, L.pyr_levels, L.pyr_margin, L.pbox, L.ncells};
+    const PyrArgs Y{pyr_mask ? pyr_mask : L.pyr, L.pyr_dim, L.pyr_off, L.pyr_levels, L.pyr_margin, L.pbox, L.ncells};
     if (shared) {
         DWG_REQUIRE(L.pyr_levels <= kMaxPyr && 7 * L.pyr_levels + 1 <= kPwStack, "pyramid deeper than the wave stack");
         dim3 grid(grid_for(nq, kPwWaves * 64)), block(kPwWaves * 64);
@@ -1619,7 +1646,12 @@ void knn_query(dwg_ctx* ctx, const dwg_knn_index* I, const float* q, int64_t nq,
         // points serves one or two queries: walking the occupancy pyramid per
         // lane costs less (1.2 vs 2.1 ms for 35k queries over 950k points)
         ctx->count(base + ":sparse", nq);
-        launch_far(ctx, I, q, nq, nullptr, k, skip, out_idx, out_d2);
+        // with a skip mask, a pyramid of the kept points only: the walk then
+        // never wades through blocks of skipped points (the zero fill's
+        // zero sources come in clusters)
+        const uint8_t* pm = (skip && nq >= kPyrMinQueries) ? masked_pyramid(ctx, I->lvl[0], skip) : nullptr;
+        // (the sparse path needs a pyramid, checked above)
+        launch_far(ctx, I, q, nq, nullptr, k, skip, out_idx, out_d2, nullptr, false, pm);
         if (sc) {
             hipLaunchKernelGGL(scatter_pairs_kernel, dim3(grid_for(nq * k, 256)), dim3(256), 0, ctx->stream, out_idx, k,
                                nullptr, nq, *sc);
@@ -1651,7 +1683,10 @@ void knn_query(dwg_ctx* ctx, const dwg_knn_index* I, const float* q, int64_t nq,
         // points); sparser far queries are spread apart and walk per lane
         const bool shared = nq >= I->n;
         if (shared) ctx->count(base + ":far_shared", n3);
-        launch_far(ctx, I, q, n3, redo3, k, skip, out_idx, out_d2, kth3, shared);
+        const uint8_t* pm = (skip && n3 >= kPyrMinQueries && I->lvl[0].pyr_levels >= 2)
+                                ? masked_pyramid(ctx, I->lvl[0], skip)
+                                : nullptr;
+        launch_far(ctx, I, q, n3, redo3, k, skip, out_idx, out_d2, kth3, shared, pm);
     }
     if (sc && n3 > 0) {
         const std::string name = base + ":scatter";
