@@ -543,10 +543,10 @@ __device__ bool search_level(const KnnGridArgs& G, const int64_t* __restrict__ c
                     for (int64_t s = sb; s < se; ++s) {
                         const float4 p = pts[s];
                         const int idx = __float_as_int(p.w);
-                        if (skip && skip[idx]) continue;
                         const float d2 = dist2(qx - p.x, qy - p.y, qz - p.z);
                         const uint64_t key = (uint64_t(__float_as_uint(d2)) << 32) | uint32_t(idx);
-                        if (key < best[0]) knn_insert<K>(best, key);
+                        // the skip byte only for a point that would enter the list
+                        if (key < best[0] && !(skip && skip[idx])) knn_insert<K>(best, key);
                     }
                 }
             }
@@ -1239,10 +1239,10 @@ __global__ __launch_bounds__(kPyrThreads) void knn_pyramid_kernel(KnnGridArgs G,
             for (int64_t s = cell_start[c]; s < se; ++s) {
                 const float4 p = pts[s];
                 const int idx = __float_as_int(p.w);
-                if (skip && skip[idx]) continue;
                 const float d2 = dist2(qx - p.x, qy - p.y, qz - p.z);
                 const uint64_t key = (uint64_t(__float_as_uint(d2)) << 32) | uint32_t(idx);
-                if (key < best[0]) knn_insert<K>(best, key);
+                // the skip byte only for a point that would enter the list
+                if (key < best[0] && !(skip && skip[idx])) knn_insert<K>(best, key);
             }
         } else {
             const int4 cd = s_dim[l - 1];
