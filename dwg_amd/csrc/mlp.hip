@@ -12,7 +12,7 @@
 //
 // Every product is one launch of gemm_bf16: C = A·B with arbitrary element
 // strides for A (M×K), B (K×N) and C (M×N), BF16 operands, FP32
-// accumulation, one 32×32 output tile per wave on v_mfma_f32_32x32x8_bf16.
+// accumulation, one 32×32 output tile per wave on v_mfma_f32_32x32x16_bf16.
 // GELU is the tanh form of Hendrycks & Gimpel (2016).  The loss is the mean
 // over the batch of 1 − ŷ·t (oriented) or 1 − (ŷ·t)² (normal line, sign
 // free), ŷ = y/|y|, t the unit target normal.  The initialisation orients
@@ -35,8 +35,16 @@ dwg_mlp::~dwg_mlp() {
 
 namespace dwg {
 
-typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef short bf16x8 __attribute__((ext_vector_type(8)));    // 8 BF16 bit patterns
+typedef __bf16 bf16x8v __attribute__((ext_vector_type(8)));  // the MFMA operand type
+
 typedef float f32x16 __attribute__((ext_vector_type(16)));
+
+// v_mfma_f32_32x32x16_bf16 (gfx950): 32×32 += 32×16 · 16×32 in one instruction
+__device__ __forceinline__ f32x16 mfma_32x32x16(const bf16x8 a, const bf16x8 b, const f32x16 c) {
+    return __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8v, a), __builtin_bit_cast(bf16x8v, b), c,
+                                                   0, 0, 0);
+}
 
 __device__ __forceinline__ uint16_t to_bf16(float f) {
     // round to nearest even; NaN stays NaN
@@ -130,9 +138,9 @@ struct GemmArgs {
     int accumulate;
 };
 
-// Lane layout of v_mfma_f32_32x32x8_bf16 (one wave, 32×32 += 32×8 · 8×32):
-//   A: lane l holds A[l%32][4(l/32) .. 4(l/32)+3]
-//   B: lane l holds B[4(l/32) .. 4(l/32)+3][l%32]
+// Lane layout of v_mfma_f32_32x32x16_bf16 (one wave, 32×32 += 32×16 · 16×32):
+//   A: lane l holds A[l%32][8(l/32) .. 8(l/32)+7]
+//   B: lane l holds B[8(l/32) .. 8(l/32)+7][l%32]
 //   D: lane l, register r holds D[8(r/4) + 4(l/32) + r%4][l%32]
 __global__ __launch_bounds__(256) void gemm_bf16_kernel(GemmArgs g, int64_t tiles_n) {
     const int lane = threadIdx.x & 63;
@@ -142,19 +150,19 @@ __global__ __launch_bounds__(256) void gemm_bf16_kernel(GemmArgs g, int64_t tile
     const int64_t ti = tile / tiles_n, tj = tile % tiles_n;
     const int64_t row = ti * 32 + (lane & 31);
     const int64_t col = tj * 32 + (lane & 31);
-    const int kq = 4 * (lane >> 5);
+    const int kq = 8 * (lane >> 5);
     f32x16 acc;
 #pragma unroll
     for (int r = 0; r < 16; ++r) acc[r] = 0.f;
-    for (int64_t k0 = 0; k0 < g.K; k0 += 8) {
-        bf16x4 a, b;
+    for (int64_t k0 = 0; k0 < g.K; k0 += 16) {
+        bf16x8 a, b;
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
+        for (int j = 0; j < 8; ++j) {
             const int64_t k = k0 + kq + j;
             a[j] = (row < g.M && k < g.K) ? short(g.A[row * g.sAi + k * g.sAk]) : short(0);
             b[j] = (col < g.N && k < g.K) ? short(g.B[k * g.sBk + col * g.sBj]) : short(0);
         }
-        acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, b, acc, 0, 0, 0);
+        acc = mfma_32x32x16(a, b, acc);
     }
 #pragma unroll
     for (int r = 0; r < 16; ++r) {
@@ -452,7 +460,7 @@ static void forward(dwg_ctx* ctx, const dwg_mlp* m, const uint16_t* x, int64_t r
 }
 
 // Inference of the whole network, 32 patches per wave: the three layers as
-// the same sequence of v_mfma_f32_32x32x8_bf16 steps as the layer-by-layer
+// the same sequence of v_mfma_f32_32x32x16_bf16 steps as the layer-by-layer
 // GEMMs (so the same sums, bit for bit), bias and GELU applied to the
 // accumulators, the BF16 activations staged in LDS between layers, and the
 // output bias and normalisation at the end: one launch, no activations in
@@ -479,7 +487,7 @@ __global__ __launch_bounds__(64 * kInferWaves) void mlp_infer_kernel(const uint1
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     uint16_t(*h)[kHPad] = hs[wave];
     const int64_t row0 = (int64_t(blockIdx.x) * kInferWaves + wave) * 32;
-    const int r = lane & 31, kq = 4 * (lane >> 5);
+    const int r = lane & 31, kq = 8 * (lane >> 5);
     const int64_t row = row0 + r;
     const bool ok = row < n;
     f32x16 acc[4];
@@ -488,16 +496,16 @@ __global__ __launch_bounds__(64 * kInferWaves) void mlp_infer_kernel(const uint1
     for (int t = 0; t < 4; ++t)
 #pragma unroll
         for (int q = 0; q < 16; ++q) acc[t][q] = 0.f;
-    for (int k0 = 0; k0 < kIn; k0 += 8) {
-        bf16x4 a;
+    for (int k0 = 0; k0 < kIn; k0 += 16) {
+        bf16x8 a;
 #pragma unroll
-        for (int j = 0; j < 4; ++j) a[j] = ok ? short(x[row * kIn + k0 + kq + j]) : short(0);
+        for (int j = 0; j < 8; ++j) a[j] = ok ? short(x[row * kIn + k0 + kq + j]) : short(0);
 #pragma unroll
         for (int t = 0; t < 4; ++t) {
-            bf16x4 b;
+            bf16x8 b;
 #pragma unroll
-            for (int j = 0; j < 4; ++j) b[j] = short(W[kOffW1 + int64_t(t * 32 + r) * kIn + k0 + kq + j]);
-            acc[t] = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, b, acc[t], 0, 0, 0);
+            for (int j = 0; j < 8; ++j) b[j] = short(W[kOffW1 + int64_t(t * 32 + r) * kIn + k0 + kq + j]);
+            acc[t] = mfma_32x32x16(a, b, acc[t]);
         }
     }
     infer_layer(acc, P + kOffB1, lane, h);
@@ -507,16 +515,16 @@ __global__ __launch_bounds__(64 * kInferWaves) void mlp_infer_kernel(const uint1
     for (int t = 0; t < 4; ++t)
 #pragma unroll
         for (int q = 0; q < 16; ++q) acc[t][q] = 0.f;
-    for (int k0 = 0; k0 < kHid; k0 += 8) {
-        bf16x4 a;
+    for (int k0 = 0; k0 < kHid; k0 += 16) {
+        bf16x8 a;
 #pragma unroll
-        for (int j = 0; j < 4; ++j) a[j] = short(h[r][k0 + kq + j]);
+        for (int j = 0; j < 8; ++j) a[j] = short(h[r][k0 + kq + j]);
 #pragma unroll
         for (int t = 0; t < 4; ++t) {
-            bf16x4 b;
+            bf16x8 b;
 #pragma unroll
-            for (int j = 0; j < 4; ++j) b[j] = short(W[kOffW2 + int64_t(t * 32 + r) * kHid + k0 + kq + j]);
-            acc[t] = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, b, acc[t], 0, 0, 0);
+            for (int j = 0; j < 8; ++j) b[j] = short(W[kOffW2 + int64_t(t * 32 + r) * kHid + k0 + kq + j]);
+            acc[t] = mfma_32x32x16(a, b, acc[t]);
         }
     }
     __syncthreads();
@@ -526,14 +534,14 @@ __global__ __launch_bounds__(64 * kInferWaves) void mlp_infer_kernel(const uint1
     f32x16 y;
 #pragma unroll
     for (int q = 0; q < 16; ++q) y[q] = 0.f;
-    for (int k0 = 0; k0 < kHid; k0 += 8) {
-        bf16x4 a, b;
+    for (int k0 = 0; k0 < kHid; k0 += 16) {
+        bf16x8 a, b;
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
+        for (int j = 0; j < 8; ++j) {
             a[j] = short(h[r][k0 + kq + j]);
             b[j] = r < kOut ? short(W[kOffW3 + int64_t(r) * kHid + k0 + kq + j]) : short(0);
         }
-        y = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, b, y, 0, 0, 0);
+        y = mfma_32x32x16(a, b, y);
     }
     __syncthreads();
     // column c of row i sits in lane c (+32), register 8(i/8)... stage through LDS as floats
