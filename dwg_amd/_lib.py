@@ -26,7 +26,7 @@ DWG_NEAR_FP64 = 16
 DWG_NOALLOC_MAX_SOURCES = 262144
 
 EXPORTS = [
-    "dwg_ctx_create", "dwg_ctx_destroy", "dwg_ctx_error", "dwg_ctx_sync", "dwg_ctx_profile",
+    "dwg_ctx_create", "dwg_ctx_destroy", "dwg_ctx_error", "dwg_ctx_sync", "dwg_cache_release", "dwg_ctx_profile",
     "dwg_ctx_profile_get", "dwg_ctx_profile_reset", "dwg_ctx_counter_get", "dwg_gwn_direct",
     "dwg_gwn_direct_noalloc", "dwg_octree_build", "dwg_octree_build_device",
     "dwg_octree_leaves", "dwg_octree_leaf_normals",
@@ -96,6 +96,7 @@ def load():
     L.dwg_ctx_error.argtypes = [_vp]
     L.dwg_ctx_error.restype = ctypes.c_char_p
     L.dwg_ctx_sync.argtypes = [_vp]
+    L.dwg_cache_release.argtypes = [P(_i64)]
     L.dwg_ctx_profile.argtypes = [_vp, _int]
     L.dwg_ctx_profile_get.argtypes = [_vp, ctypes.c_char_p, P(ctypes.c_double), P(_i64)]
     L.dwg_ctx_profile_reset.argtypes = [_vp]

@@ -58,6 +58,16 @@ int dwg_ctx_create(int device, void* stream, dwg_ctx** out) {
     return DWG_OK;
 }
 
+int dwg_cache_release(int64_t* out_bytes) {
+    try {
+        const size_t freed = dwg::dev_cache_release();
+        if (out_bytes) *out_bytes = int64_t(freed);
+        return DWG_OK;
+    } catch (...) {
+        return DWG_ERR_HIP;
+    }
+}
+
 void dwg_ctx_destroy(dwg_ctx* ctx) {
     if (!ctx) return;
     (void)hipSetDevice(ctx->device);

@@ -2,6 +2,7 @@
 #pragma once
 
 #include <hip/hip_runtime.h>
+#include "pool.h"
 
 #include <cstdint>
 #include <cstdio>
@@ -40,7 +41,7 @@ struct DevBuf {
     size_t bytes = 0;
     ~DevBuf() { release(); }
     void release() {
-        if (ptr) (void)hipFree(ptr);
+        dev_free(ptr);
         ptr = nullptr;
         bytes = 0;
     }
@@ -48,7 +49,7 @@ struct DevBuf {
         if (need <= bytes && ptr) return ptr;
         release();
         size_t b = need < 256 ? 256 : need;
-        DWG_HIP_CHECK(hipMalloc(&ptr, b));
+        ptr = dev_malloc(b);
         bytes = b;
         return ptr;
     }

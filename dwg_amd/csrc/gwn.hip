@@ -346,7 +346,7 @@ void gwn_direct_noalloc(dwg_ctx* ctx, const float* pts, const float* nrm, const 
 dwg_octree::~dwg_octree() {
     for (void* p : {(void*)info, (void*)kid, (void*)cm_r, (void*)cm_lo, (void*)ec, (void*)sa, (void*)sb, (void*)perm,
                     (void*)an64, (void*)cell, (void*)cm64, (void*)payload, (void*)radius})
-        if (p) (void)hipFree(p);
+        dwg::dev_free(p);
 }
 
 namespace dwg {

@@ -17,13 +17,13 @@
 
 dwg_knn_index::~dwg_knn_index() {
     for (auto& L : lvl) {
-        if (L.cell_start) (void)hipFree(L.cell_start);
-        if (L.pts) (void)hipFree(L.pts);
-        if (L.empty) (void)hipFree(L.empty);
-        if (L.pyr) (void)hipFree(L.pyr);
-        if (L.pyr_dim) (void)hipFree(L.pyr_dim);
-        if (L.pyr_off) (void)hipFree(L.pyr_off);
-        if (L.pbox) (void)hipFree(L.pbox);
+        dwg::dev_free(L.cell_start);
+        dwg::dev_free(L.pts);
+        dwg::dev_free(L.empty);
+        dwg::dev_free(L.pyr);
+        dwg::dev_free(L.pyr_dim);
+        dwg::dev_free(L.pyr_off);
+        dwg::dev_free(L.pbox);
     }
 }
 
@@ -171,9 +171,9 @@ static double cells_of(const float* lo, const float* hi, double cell) {
 // histogram + scan (+ scatter when out != nullptr) of the points into level L
 static void bucket(dwg_ctx* ctx, const float* p, int64_t n, dwg_knn_index::Level& L, bool scatter) {
     L.ncells = int64_t(L.G.dim[0]) * L.G.dim[1] * L.G.dim[2];
-    if (L.cell_start) (void)hipFree(L.cell_start);
+    dwg::dev_free(L.cell_start);
     L.cell_start = nullptr;
-    DWG_HIP_CHECK(hipMalloc(&L.cell_start, sizeof(int64_t) * (L.ncells + 1)));
+    L.cell_start = static_cast<decltype(L.cell_start)>(dev_malloc(sizeof(int64_t) * (L.ncells + 1)));
     int* cell_of = static_cast<int*>(ctx->buf("knn_cell_of", sizeof(int) * n));
     fill_u32(ctx, L.cell_start, 0, 2 * (L.ncells + 1));
     hipLaunchKernelGGL(knn_count_kernel, dim3(grid_for(n, 256)), dim3(256), 0, ctx->stream, p, n, L.G,
@@ -185,7 +185,7 @@ static void bucket(dwg_ctx* ctx, const float* p, int64_t n, dwg_knn_index::Level
     hipLaunchKernelGGL(copy_i64_kernel, dim3(grid_for(L.ncells, 256)), dim3(256), 0, ctx->stream, L.cell_start,
                        cursor, L.ncells);
     check_launch();
-    DWG_HIP_CHECK(hipMalloc(&L.pts, sizeof(float4) * n));
+    L.pts = static_cast<decltype(L.pts)>(dev_malloc(sizeof(float4) * n));
     hipLaunchKernelGGL(knn_scatter_kernel, dim3(grid_for(n, 256)), dim3(256), 0, ctx->stream, p, n, cell_of, cursor,
                        L.pts);
     check_launch();
@@ -222,7 +222,7 @@ __global__ void empty_dist_pass_kernel(const int* __restrict__ in, int dx, int d
 
 static void empty_distance(dwg_ctx* ctx, dwg_knn_index::Level& L) {
     const int dx = L.G.dim[0], dy = L.G.dim[1], dz = L.G.dim[2];
-    DWG_HIP_CHECK(hipMalloc(&L.empty, sizeof(int) * L.ncells));
+    L.empty = static_cast<decltype(L.empty)>(dev_malloc(sizeof(int) * L.ncells));
     int* tmp = static_cast<int*>(ctx->buf("knn_empty_tmp", sizeof(int) * L.ncells));
     const unsigned g = grid_for(L.ncells, 256);
     hipLaunchKernelGGL(empty_dist_x_kernel, dim3(g), dim3(256), 0, ctx->stream, L.cell_start, dx, dy, dz, L.empty);
@@ -319,7 +319,7 @@ static void ensure_boxes(dwg_ctx* ctx, dwg_knn_index::Level& L) {
     const int64_t* off = L.pyr_off_h;
     int64_t total = 0;
     for (int l = 1; l < levels; ++l) total += int64_t(dims[l].x) * dims[l].y * dims[l].z;
-    DWG_HIP_CHECK(hipMalloc(&L.pbox, sizeof(float4) * 2 * size_t(L.ncells + total)));
+    L.pbox = static_cast<decltype(L.pbox)>(dev_malloc(sizeof(float4) * 2 * size_t(L.ncells + total)));
     hipLaunchKernelGGL(cell_box_kernel, dim3(grid_for(L.ncells, 256)), dim3(256), 0, ctx->stream, L.cell_start, L.pts,
                        L.ncells, L.pbox);
     check_launch();
@@ -407,9 +407,9 @@ static void pyramid(dwg_ctx* ctx, dwg_knn_index::Level& L) {
         ++levels;
     }
     if (levels == 1) return;  // one cell: the shell search is already optimal
-    DWG_HIP_CHECK(hipMalloc(&L.pyr, size_t(total)));
-    DWG_HIP_CHECK(hipMalloc(&L.pyr_dim, sizeof(int4) * kMaxPyr));
-    DWG_HIP_CHECK(hipMalloc(&L.pyr_off, sizeof(int64_t) * kMaxPyr));
+    L.pyr = static_cast<decltype(L.pyr)>(dev_malloc(size_t(total)));
+    L.pyr_dim = static_cast<decltype(L.pyr_dim)>(dev_malloc(sizeof(int4) * kMaxPyr));
+    L.pyr_off = static_cast<decltype(L.pyr_off)>(dev_malloc(sizeof(int64_t) * kMaxPyr));
     DWG_HIP_CHECK(hipMemcpyAsync(L.pyr_dim, dims, sizeof(int4) * levels, hipMemcpyHostToDevice, ctx->stream));
     DWG_HIP_CHECK(hipMemcpyAsync(L.pyr_off, off, sizeof(int64_t) * levels, hipMemcpyHostToDevice, ctx->stream));
     for (int l = 1; l < levels; ++l) {
@@ -453,7 +453,7 @@ dwg_knn_index* knn_build(dwg_ctx* ctx, const float* p, int64_t n, int k_hint) {
                            occ);
         check_launch();
         const double occupied = double(std::max<unsigned long long>(1, read_scalar<unsigned long long>(ctx, occ)));
-        (void)hipFree(probe.cell_start);
+        dev_free(probe.cell_start);
         probe.cell_start = nullptr;
         const double o = double(n) / occupied;
         double scale = std::sqrt(target / o);

@@ -30,7 +30,7 @@
 
 dwg_mlp::~dwg_mlp() {
     for (void* p : {(void*)params, (void*)bf16, (void*)grad, (void*)adam_m, (void*)adam_v})
-        if (p) (void)hipFree(p);
+        dwg::dev_free(p);
 }
 
 namespace dwg {
@@ -327,11 +327,11 @@ __global__ void augment_kernel(const uint16_t* __restrict__ pin, const float* __
 static dwg_mlp* mlp_alloc() {
     auto* m = new dwg_mlp();
     try {
-        DWG_HIP_CHECK(hipMalloc(&m->params, sizeof(float) * kParams));
-        DWG_HIP_CHECK(hipMalloc(&m->bf16, sizeof(uint16_t) * kParams));
-        DWG_HIP_CHECK(hipMalloc(&m->grad, sizeof(float) * kParams));
-        DWG_HIP_CHECK(hipMalloc(&m->adam_m, sizeof(float) * kParams));
-        DWG_HIP_CHECK(hipMalloc(&m->adam_v, sizeof(float) * kParams));
+        m->params = static_cast<decltype(m->params)>(dev_malloc(sizeof(float) * kParams));
+        m->bf16 = static_cast<decltype(m->bf16)>(dev_malloc(sizeof(uint16_t) * kParams));
+        m->grad = static_cast<decltype(m->grad)>(dev_malloc(sizeof(float) * kParams));
+        m->adam_m = static_cast<decltype(m->adam_m)>(dev_malloc(sizeof(float) * kParams));
+        m->adam_v = static_cast<decltype(m->adam_v)>(dev_malloc(sizeof(float) * kParams));
     } catch (...) {
         delete m;
         throw;

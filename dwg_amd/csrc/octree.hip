@@ -360,9 +360,7 @@ constexpr int kReduceBlocks = 256;
 
 template <class T>
 T* dev_alloc(int64_t count) {
-    T* p = nullptr;
-    DWG_HIP_CHECK(hipMalloc(&p, sizeof(T) * size_t(std::max<int64_t>(count, 1))));
-    return p;
+    return static_cast<T*>(dev_malloc(sizeof(T) * size_t(std::max<int64_t>(count, 1))));
 }
 
 }  // namespace
@@ -441,8 +439,8 @@ dwg_octree* octree_build(dwg_ctx* ctx, const double* P, const double* A, int64_t
                 DWG_HIP_CHECK(hipMemcpyAsync(ni, info, sizeof(int4) * le, hipMemcpyDeviceToDevice, ctx->stream));
                 DWG_HIP_CHECK(hipMemcpyAsync(nc, cell, sizeof(double4) * le, hipMemcpyDeviceToDevice, ctx->stream));
                 DWG_HIP_CHECK(hipStreamSynchronize(ctx->stream));
-                (void)hipFree(info);
-                (void)hipFree(cell);
+                dev_free(info);
+                dev_free(cell);
                 info = ni;
                 cell = nc;
                 cap = ncap;
@@ -502,8 +500,8 @@ dwg_octree* octree_build(dwg_ctx* ctx, const double* P, const double* A, int64_t
         exclusive_scan<int64_t>(ctx, lslot, m);
         T->n_leaves = read_scalar<int64_t>(ctx, lslot + m);
     } catch (...) {
-        if (info) (void)hipFree(info);
-        if (cell) (void)hipFree(cell);
+        if (info) dev_free(info);
+        if (cell) dev_free(cell);
         delete T;
         throw;
     }

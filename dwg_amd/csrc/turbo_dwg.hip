@@ -87,17 +87,16 @@ void free_host(void* p) {
 
 template <class T>
 DevArray<T>::~DevArray() {
-    if (ptr) (void)hipFree(ptr);
+    dwg::dev_free(ptr);
 }
 
 template <class T>
 void DevArray<T>::resize(int64_t n) {
     if (n <= count && ptr) return;
-    if (ptr) (void)hipFree(ptr);
+    dwg::dev_free(ptr);
     ptr = nullptr;
     count = 0;
-    if (hipMalloc(&ptr, sizeof(T) * std::max<int64_t>(n, 1)) != hipSuccess)
-        throw dwg::Error(DWG_ERR_HIP, "device allocation failed");
+    ptr = static_cast<T*>(dwg::dev_malloc(sizeof(T) * size_t(std::max<int64_t>(n, 1))));
     count = n;
 }
 

@@ -33,6 +33,15 @@ def _f32(t, device):
     return t.to(device=device, dtype=torch.float32).contiguous()
 
 
+def cache_release() -> int:
+    """Hand the library's cached device blocks back to HIP; returns the bytes freed."""
+    L = _lib.load()
+    b = ctypes.c_int64()
+    if L.dwg_cache_release(ctypes.byref(b)) != 0:
+        raise DwgError(2, "dwg_cache_release failed")
+    return b.value
+
+
 class Context:
     """dwg_ctx on one device, issuing work on torch's current stream."""
 

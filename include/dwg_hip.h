@@ -57,6 +57,12 @@ int dwg_ctx_create(int device, void* stream, dwg_ctx** out);
 void dwg_ctx_destroy(dwg_ctx* ctx);
 const char* dwg_ctx_error(const dwg_ctx* ctx);
 int dwg_ctx_sync(dwg_ctx* ctx);
+/* The library keeps the device blocks it releases (trees, indices, scratch)
+ * for reuse by later calls; this hands every cached block back to HIP
+ * (after synchronising each device that has some) and returns the bytes
+ * freed in *out_bytes (may be NULL).  No reference counterpart: for
+ * long-lived processes that want the memory back between workloads. */
+int dwg_cache_release(int64_t* out_bytes);
 /* Per-kernel timing with HIP events on the context stream (off by default). */
 int dwg_ctx_profile(dwg_ctx* ctx, int enable);
 /* Sum of recorded durations (ms) and launch count of kernel `name`; syncs. */

@@ -240,3 +240,18 @@ def test_small_sphere_direct_config(ctx):
     assert err < 1e-5
     assert abs(win.mean() - 1) < 0.01 and np.all(np.abs(win - 1) < 0.05) and np.all(np.abs(wout) < 0.05)
     assert np.median(ang) < 6.0 and np.mean(ang < 90) > 0.99
+
+
+def test_cached_blocks_are_reused_and_released(ctx):
+    # the library keeps the device blocks it releases for later calls: runs
+    # that reuse them (and a run after the cache is handed back) give the
+    # same result bit for bit
+    P, _ = shapes.torus(20_000, 5)
+    kw = dict(grid_reso=96, final_reso=96, total_iter=4)
+    a = pipeline.reconstruct(ctx, P, **kw)
+    b = pipeline.reconstruct(ctx, P, **kw)
+    freed = gpu.cache_release()
+    c = pipeline.reconstruct(ctx, P, **kw)
+    assert freed > 0
+    for x in (b, c):
+        assert np.array_equal(a.normals, x.normals) and np.array_equal(a.tris, x.tris)
