@@ -1650,8 +1650,11 @@ void knn_query(dwg_ctx* ctx, const dwg_knn_index* I, const float* q, int64_t nq,
         // never wades through blocks of skipped points (the zero fill's
         // zero sources come in clusters)
         const uint8_t* pm = (skip && nq >= kPyrMinQueries) ? masked_pyramid(ctx, I->lvl[0], skip) : nullptr;
-        // (the sparse path needs a pyramid, checked above)
-        launch_far(ctx, I, q, nq, nullptr, k, skip, out_idx, out_d2, nullptr, false, pm);
+        // the queries in tile order: a wave's 64 walks then start near each
+        // other and take similar paths (0.84 instead of 1.19 ms for the
+        // zero fill's ~30k queries)
+        const TileBuckets B = tile_buckets(ctx, I, q, nq, nullptr, nullptr);
+        launch_far(ctx, I, q, nq, B.order, k, skip, out_idx, out_d2, nullptr, false, pm);
         if (sc) {
             hipLaunchKernelGGL(scatter_pairs_kernel, dim3(grid_for(nq * k, 256)), dim3(256), 0, ctx->stream, out_idx, k,
                                nullptr, nq, *sc);
