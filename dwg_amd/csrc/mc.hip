@@ -168,18 +168,36 @@ __device__ __forceinline__ void emit_cube(const McArgs& A, int x, int y, int z, 
     }
 }
 
-__global__ __launch_bounds__(kMcThreads) void mc_count_kernel(McArgs A, McBricks B, int64_t* __restrict__ tile_counts) {
-    const int64_t b = brick_of_block();
-    int x, y, z0;
-    column_of(B, b, x, y, z0);
-    int cases[kZPer];
-    column_cases(A, x, y, z0, cases);
-    int cnt = 0;
+// The count pass: one wave per brick, each lane a whole column of kBz cubes
+// (its kBz + 1 z faces loaded once), the brick's total a wave sum
+constexpr int kCountThreads = kBx * kBy;
+static_assert(kCountThreads == 64, "one wave per brick");
+
+__global__ __launch_bounds__(kCountThreads) void mc_count_kernel(McArgs A, McBricks B, int64_t* __restrict__ tile_counts) {
+    __shared__ uint8_t s_ntri[256];  // triangles of each case, read per lane from LDS
 #pragma unroll
-    for (int j = 0; j < kZPer; ++j) cnt += int(c_mc_rows[cases[j]] >> 60);
-    int total;
-    (void)block_exclusive_sum(cnt, total);
-    if (threadIdx.x == 0) tile_counts[b] = total;
+    for (int c = threadIdx.x; c < 256; c += kCountThreads) s_ntri[c] = uint8_t(c_mc_rows[c] >> 60);
+    __syncthreads();
+    const int64_t b = brick_of_block();
+    const int x = int(b % B.nbx) * kBx + (threadIdx.x & (kBx - 1));
+    const int y = int((b / B.nbx) % B.nby) * kBy + int(threadIdx.x) / kBx;
+    const int z0 = int(b / (int64_t(B.nbx) * B.nby)) * kBz;
+    int cnt = 0;
+    if (x < A.rx - 1 && y < A.ry - 1 && z0 < A.rz - 1) {
+        float lo[4], hi[4], v[8];
+        load_face(A, x, y, z0, lo);
+#pragma unroll
+        for (int j = 0; j < kBz; ++j) {
+            if (z0 + j < A.rz - 1) {
+                load_face(A, x, y, z0 + j + 1, hi);
+                cnt += s_ntri[cube_case(A, lo, hi, v)];
+#pragma unroll
+                for (int k = 0; k < 4; ++k) lo[k] = hi[k];
+            }
+        }
+    }
+    const int incl = wave_inclusive_sum(cnt);
+    if (threadIdx.x == 63) tile_counts[b] = incl;
 }
 
 __global__ __launch_bounds__(kMcThreads) void mc_emit_kernel(McArgs A, McBricks B, const int64_t* __restrict__ tile_offsets,
@@ -223,7 +241,7 @@ int64_t marching_cubes(dwg_ctx* ctx, const float* field, const dwg_grid* g, floa
     int64_t* offs = static_cast<int64_t*>(ctx->buf("mc_tiles", sizeof(int64_t) * (tiles + 1)));
     {
         ProfileScope ps(ctx, "mc_count");
-        hipLaunchKernelGGL(mc_count_kernel, dim3(unsigned(tiles)), dim3(kMcThreads), 0, ctx->stream, A, B, offs);
+        hipLaunchKernelGGL(mc_count_kernel, dim3(unsigned(tiles)), dim3(kCountThreads), 0, ctx->stream, A, B, offs);
         check_launch();
         exclusive_scan<int64_t>(ctx, offs, tiles);
     }
