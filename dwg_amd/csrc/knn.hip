@@ -691,7 +691,11 @@ __global__ __launch_bounds__(kTileThreads) void knn_tile_kernel(
     float* __restrict__ out_d2, int* __restrict__ redo, float* __restrict__ redo_kth, unsigned int* __restrict__ n_redo,
     int64_t* __restrict__ redo_flag, KnnScatter sc) {
     constexpr int kRegion = kTile + 2 * HALO;
-    constexpr int kRows = kRegion * kRegion;
+    // the first pass splits each row into the tile's own cells and the halo
+    // cells at x distance 1 and 2 on either side, so that the staging order
+    // (by Chebyshev ring from the tile) starts with the tile's own cells
+    constexpr int kParts = HALO == kHalo1 ? 5 : 1;
+    constexpr int kRows = kRegion * kRegion * kParts;
     constexpr int kHash = SCATTER ? 512 : 1;
     __shared__ int s_hkey[kHash];
     __shared__ unsigned long long s_hacc[kHash][3];
@@ -712,11 +716,12 @@ __global__ __launch_bounds__(kTileThreads) void knn_tile_kernel(
     const int x0 = max(tx * kTile - HALO, 0), x1 = min(tx * kTile + kTile - 1 + HALO, G.dim[0] - 1);
     const int y0 = max(ty * kTile - HALO, 0), y1 = min(ty * kTile + kTile - 1 + HALO, G.dim[1] - 1);
     const int z0 = max(tz * kTile - HALO, 0), z1 = min(tz * kTile + kTile - 1 + HALO, G.dim[2] - 1);
-    const int ny = y1 - y0 + 1, nrows = ny * (z1 - z0 + 1);
+    const int ny = y1 - y0 + 1, nrows = ny * (z1 - z0 + 1) * kParts;
     if (threadIdx.x < HALO + 2) s_ring[threadIdx.x] = 0;
     __syncthreads();
-    // each (y, z) row of cells x0..x1 is one contiguous range of points;
-    // a thread takes rows r0 = threadIdx.x and r0 + 256 (kRows ≤ 512)
+    // each (y, z) row of cells x0..x1 (or each of its parts) is one
+    // contiguous range of points; a thread takes rows r0 = threadIdx.x and
+    // r0 + 256 (kRows ≤ 512)
     static_assert(kRows <= 2 * kTileThreads, "at most two rows per thread");
     int ring[2] = {0, 0}, rank[2] = {0, 0}, rc[2] = {0, 0};
     int64_t rb[2] = {0, 0};
@@ -724,13 +729,24 @@ __global__ __launch_bounds__(kTileThreads) void knn_tile_kernel(
     for (int h = 0; h < 2; ++h) {
         const int r = threadIdx.x + h * kTileThreads;
         if (r < nrows) {
-            const int y = y0 + r % ny, z = z0 + r / ny;
-            const int64_t c = x0 + int64_t(G.dim[0]) * (y + int64_t(G.dim[1]) * z);
+            const int row = r / kParts, part = r % kParts;
+            const int y = y0 + row % ny, z = z0 + row / ny;
+            // the part's cells xa..xb and their x distance from the tile
+            int xa = x0, xb = x1, dx = 0;
+            if (kParts > 1) {
+                const int t0 = tx * kTile, t1 = tx * kTile + kTile - 1;
+                if (part == 0) { xa = t0; xb = t1; }
+                else if (part <= 2) { xa = xb = t0 - part; dx = part; }
+                else { xa = xb = t1 + (part - 2); dx = part - 2; }
+                xa = max(xa, x0);
+                xb = min(xb, x1);
+            }
+            const int64_t c = xa + int64_t(G.dim[0]) * (y + int64_t(G.dim[1]) * z);
             rb[h] = cell_start[c];
-            rc[h] = int(cell_start[c + (x1 - x0) + 1] - rb[h]);
+            rc[h] = xb >= xa ? int(cell_start[c + (xb - xa) + 1] - rb[h]) : 0;
             const int dy = max(max(ty * kTile - y, y - (ty * kTile + kTile - 1)), 0);
             const int dz = max(max(tz * kTile - z, z - (tz * kTile + kTile - 1)), 0);
-            ring[h] = max(dy, dz);
+            ring[h] = max(max(dy, dz), dx);
             rank[h] = atomicAdd(&s_ring[ring[h] + 1], 1);
         }
     }
@@ -742,7 +758,7 @@ __global__ __launch_bounds__(kTileThreads) void knn_tile_kernel(
     for (int h = 0; h < 2; ++h) {
         if (threadIdx.x + h * kTileThreads < nrows) {
             const int pos = s_ring[ring[h]] + rank[h];
-            const int r = threadIdx.x + h * kTileThreads;
+            const int r = (threadIdx.x + h * kTileThreads) / kParts;
             s_rb[pos] = rb[h];
             s_cnt[pos] = rc[h];
             s_ryz[pos] = (y0 + r % ny) | ((z0 + r / ny) << 16);
