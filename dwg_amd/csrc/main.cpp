@@ -97,6 +97,10 @@ int main(int argc, char* argv[]) {
             return 1;
         }
     }
+    if (opt.total_iter < 1) {
+        std::cerr << "--total_iter must be at least 1\n";
+        return 1;
+    }
     // output naming (src/main.cpp:151-156)
     const std::string full_in = in_path + "/" + in_name;
     const std::string pure = in_name.size() > 4 ? in_name.substr(0, in_name.size() - 4) : in_name;

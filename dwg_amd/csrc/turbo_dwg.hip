@@ -582,6 +582,9 @@ void GetRandomNs(int64_t n, uint64_t seed, std::vector<double>& N) {
 void DWGLaunch(dwg_ctx* ctx, const float* pts, const float* nrm, int64_t n, const dwg_options& opt,
                const char* out_base, dwg_result* res) {
     if (n <= 0 || !pts) throw dwg::Error(DWG_ERR_ARG, "no input points");
+    // a run without iterations once ended in a host crash whose cause is not
+    // known: refused instead of run
+    if (opt.total_iter < 1) throw dwg::Error(DWG_ERR_ARG, "total_iter must be at least 1");
     if (opt.input_normal == 1 && !nrm) throw dwg::Error(DWG_ERR_ARG, "--in_normal 1 needs input normals");
     if (opt.input_normal == 4 && !opt.mlp) throw dwg::Error(DWG_ERR_ARG, "--in_normal 4 needs MLP weights");
     const auto t0 = Clock::now();

@@ -35,6 +35,13 @@ def test_missing_input(tmp_path):
     assert r.returncode == 2 and "cannot read point cloud" in r.stderr
 
 
+def test_zero_iterations_refused(tmp_path):
+    """--total_iter 0 is refused before any input is read (the library
+    refuses it as well, with DWG_ERR_ARG)."""
+    r = _run(["--in_path", str(tmp_path), "--in_name", "none.xyz", "--out_path", str(tmp_path), "--total_iter", "0"])
+    assert r.returncode == 1 and "--total_iter must be at least 1" in r.stderr
+
+
 def _write_xyz(path, P, N):
     with open(path, "w") as f:
         for p, n in zip(P, N):
