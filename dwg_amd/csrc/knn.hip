@@ -1678,29 +1678,40 @@ void knn_query(dwg_ctx* ctx, const dwg_knn_index* I, const float* q, int64_t nq,
         }
         return;
     }
-    int* redo1 = static_cast<int*>(ctx->buf("knnt_redo1", sizeof(int) * nq + 64));
-    int* redo2 = static_cast<int*>(ctx->buf("knnt_redo2", sizeof(int) * nq + 64));
     int* redo3 = static_cast<int*>(ctx->buf("knnt_redo3", sizeof(int) * nq + 64));
-    const int64_t n1 = tiled_pass<kHalo1>(ctx, I, q, nq, nullptr, qmask, k, skip, out_idx, out_d2, redo1,
-                                          base + ":tiles", sc);
-    // every tiled pass scatters the queries it closes (summed per tile in
-    // LDS first); the far queries scatter after the last search
-    const int64_t n2 = n1 > 0 ? tiled_pass<kHalo2>(ctx, I, q, n1, redo1, nullptr, k, skip, out_idx, out_d2, redo2,
-                                                   base + ":tiles2", sc)
-                              : 0;
     float* kth3 = static_cast<float*>(ctx->buf("knnt_kth3", sizeof(float) * nq + 64));
-    const int64_t n3 = n2 > 0 ? tiled_pass<kHalo3>(ctx, I, q, n2, redo2, nullptr, k, skip, out_idx, out_d2, redo3,
-                                                   base + ":tiles3", sc, kth3, true)
-                              : 0;
+    // at least as many queries as indexed points (triangle centroids): the
+    // far ones come in dense patches (spurious surface pieces) and in tile
+    // order, so a wave's 64 share most of their search and take the
+    // wave-shared walk (6.4 vs 7.2 ms per centroid query at 1M points);
+    // sparser far queries are spread apart and walk per lane
+    const bool shared = nq >= I->n;
+    int64_t n3 = 0;
+    if (shared) {
+        // the queries the first pass leaves open go straight to the shared
+        // walk: its time is set by its longest walks (far queries deep
+        // inside large closed pieces), not by how many queries it gets, so
+        // wider tiled passes before it cost more than they save (4.41
+        // instead of 5.20 ms per centroid query at 1M points)
+        n3 = tiled_pass<kHalo1>(ctx, I, q, nq, nullptr, qmask, k, skip, out_idx, out_d2, redo3, base + ":tiles", sc,
+                                kth3, true);
+    } else {
+        int* redo1 = static_cast<int*>(ctx->buf("knnt_redo1", sizeof(int) * nq + 64));
+        int* redo2 = static_cast<int*>(ctx->buf("knnt_redo2", sizeof(int) * nq + 64));
+        const int64_t n1 = tiled_pass<kHalo1>(ctx, I, q, nq, nullptr, qmask, k, skip, out_idx, out_d2, redo1,
+                                              base + ":tiles", sc);
+        // every tiled pass scatters the queries it closes (summed per tile
+        // in LDS first); the far queries scatter after the last search
+        const int64_t n2 = n1 > 0 ? tiled_pass<kHalo2>(ctx, I, q, n1, redo1, nullptr, k, skip, out_idx, out_d2,
+                                                       redo2, base + ":tiles2", sc)
+                                  : 0;
+        n3 = n2 > 0 ? tiled_pass<kHalo3>(ctx, I, q, n2, redo2, nullptr, k, skip, out_idx, out_d2, redo3,
+                                         base + ":tiles3", sc, kth3, true)
+                    : 0;
+    }
     {
         const std::string name = base + ":redo";
         ProfileScope pr(ctx, name.c_str());
-        // at least as many queries as indexed points (triangle centroids):
-        // the far ones come in dense patches (spurious surface pieces) and
-        // in tile order, so a wave's 64 share most of their search and take
-        // the wave-shared walk (6.4 vs 7.2 ms per centroid query at 1M
-        // points); sparser far queries are spread apart and walk per lane
-        const bool shared = nq >= I->n;
         if (shared) ctx->count(base + ":far_shared", n3);
         const uint8_t* pm = (skip && n3 >= kPyrMinQueries && I->lvl[0].pyr_levels >= 2)
                                 ? masked_pyramid(ctx, I->lvl[0], skip)
