@@ -619,17 +619,29 @@ constexpr int kTileThreads = 256;
 constexpr int kHalo1 = 2;
 constexpr int kHalo2 = 5;
 constexpr int kHalo3 = 9;
+// the tiled passes bucket their queries by 2×2×2-cell eighth of the tile
+// (8 buckets per tile), so that the 64 queries of a wave sit close together
+// (first pass 2.07 → 1.97 ms, and the far walk that takes its open queries
+// in this order 1.89 → 1.81 ms); the sparse walk's order keeps one bucket
+// per tile (0.86 instead of 0.93 ms)
+constexpr int kSubTiled = 8;
 
 struct TileArgs {
     int tdim[3];   // tiles per axis
     float margin;  // rounding allowance of cell boundaries (cell_margin)
+    int sub;       // query buckets per tile: 1 or 8 (by eighth of the tile)
 };
 
-__device__ __forceinline__ int tile_of_point(const KnnGridArgs& G, const TileArgs& T, float x, float y, float z) {
-    const int cx = cell_coord(x, G.lo[0], G.inv_cell, G.dim[0]) / kTile;
-    const int cy = cell_coord(y, G.lo[1], G.inv_cell, G.dim[1]) / kTile;
-    const int cz = cell_coord(z, G.lo[2], G.inv_cell, G.dim[2]) / kTile;
-    return cx + T.tdim[0] * (cy + T.tdim[1] * cz);
+// bucket of a point: its tile, times 8 plus the eighth of the tile it lies
+// in when T.sub is 8
+__device__ __forceinline__ int bucket_of_point(const KnnGridArgs& G, const TileArgs& T, float x, float y, float z) {
+    const int cx = cell_coord(x, G.lo[0], G.inv_cell, G.dim[0]);
+    const int cy = cell_coord(y, G.lo[1], G.inv_cell, G.dim[1]);
+    const int cz = cell_coord(z, G.lo[2], G.inv_cell, G.dim[2]);
+    const int tile = cx / kTile + T.tdim[0] * (cy / kTile + T.tdim[1] * (cz / kTile));
+    if (T.sub == 1) return tile;
+    constexpr int h = kTile / 2;
+    return tile * 8 + int(cx % kTile >= h) + 2 * int(cy % kTile >= h) + 4 * int(cz % kTile >= h);
 }
 
 // query t of the pass is qlist[t] (or t); masked queries are left out
@@ -640,7 +652,7 @@ __global__ void tile_count_kernel(const float* __restrict__ q, int64_t nq, const
     int tl = -1;  // every lane takes part in the run detection
     if (t < nq) {
         const int64_t i = qlist ? int64_t(qlist[t]) : t;
-        if (!qmask || qmask[i]) tl = tile_of_point(G, T, q[3 * i], q[3 * i + 1], q[3 * i + 2]);
+        if (!qmask || qmask[i]) tl = bucket_of_point(G, T, q[3 * i], q[3 * i + 1], q[3 * i + 2]);
         tile_of[t] = tl;
     }
     const int lane = threadIdx.x & 63;
@@ -662,10 +674,11 @@ __global__ void tile_scatter_kernel(const int* __restrict__ tile_of, int64_t nq,
     if (tl >= 0) order[hb + (lane - head)] = qlist ? qlist[t] : int(t);
 }
 
-__global__ void tile_flags_kernel(const int64_t* __restrict__ start, int64_t ntiles, int64_t* __restrict__ flags) {
+__global__ void tile_flags_kernel(const int64_t* __restrict__ start, int64_t ntiles, int sub,
+                                  int64_t* __restrict__ flags) {
     const int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (t > ntiles) return;
-    flags[t] = (t < ntiles && start[t + 1] > start[t]) ? 1 : 0;
+    flags[t] = (t < ntiles && start[sub * (t + 1)] > start[sub * t]) ? 1 : 0;
 }
 
 __global__ void tile_list_kernel(const int64_t* __restrict__ flags, int64_t ntiles, int* __restrict__ list) {
@@ -710,6 +723,7 @@ __global__ __launch_bounds__(kTileThreads) void knn_tile_kernel(
     __shared__ int s_cnt[kRows];
     __shared__ int s_pref[kRows + 1];
     __shared__ int s_ryz[kRows];  // y | z << 16 of each row
+    __shared__ int s_rx[kRows];   // its cells xa | xb << 16
     __shared__ int s_ring[HALO + 2];
     const int tile = tiles[blockIdx.x];
     const int tx = tile % T.tdim[0], ty = (tile / T.tdim[0]) % T.tdim[1], tz = tile / (T.tdim[0] * T.tdim[1]);
@@ -723,7 +737,7 @@ __global__ __launch_bounds__(kTileThreads) void knn_tile_kernel(
     // contiguous range of points; a thread takes rows r0 = threadIdx.x and
     // r0 + 256 (kRows ≤ 512)
     static_assert(kRows <= 2 * kTileThreads, "at most two rows per thread");
-    int ring[2] = {0, 0}, rank[2] = {0, 0}, rc[2] = {0, 0};
+    int ring[2] = {0, 0}, rank[2] = {0, 0}, rc[2] = {0, 0}, rx[2] = {0, 0};
     int64_t rb[2] = {0, 0};
 #pragma unroll
     for (int h = 0; h < 2; ++h) {
@@ -744,6 +758,7 @@ __global__ __launch_bounds__(kTileThreads) void knn_tile_kernel(
             const int64_t c = xa + int64_t(G.dim[0]) * (y + int64_t(G.dim[1]) * z);
             rb[h] = cell_start[c];
             rc[h] = xb >= xa ? int(cell_start[c + (xb - xa) + 1] - rb[h]) : 0;
+            rx[h] = xa | (xb << 16);
             const int dy = max(max(ty * kTile - y, y - (ty * kTile + kTile - 1)), 0);
             const int dz = max(max(tz * kTile - z, z - (tz * kTile + kTile - 1)), 0);
             ring[h] = max(max(dy, dz), dx);
@@ -762,6 +777,7 @@ __global__ __launch_bounds__(kTileThreads) void knn_tile_kernel(
             s_rb[pos] = rb[h];
             s_cnt[pos] = rc[h];
             s_ryz[pos] = (y0 + r % ny) | ((z0 + r / ny) << 16);
+            s_rx[pos] = rx[h];
         }
     }
     __syncthreads();
@@ -771,7 +787,6 @@ __global__ __launch_bounds__(kTileThreads) void knn_tile_kernel(
     }
     __syncthreads();
     const int total = s_pref[nrows];
-    const float rxlo = __builtin_fmaf(float(x0), G.cell, G.lo[0]), rxhi = __builtin_fmaf(float(x1 + 1), G.cell, G.lo[0]);
     // faces of the staged box that are not on the grid boundary
     const float flx = x0 > 0 ? G.lo[0] + float(x0) * G.cell : -INFINITY;
     const float fhx = x1 + 1 < G.dim[0] ? G.lo[0] + float(x1 + 1) * G.cell : INFINITY;
@@ -779,7 +794,7 @@ __global__ __launch_bounds__(kTileThreads) void knn_tile_kernel(
     const float fhy = y1 + 1 < G.dim[1] ? G.lo[1] + float(y1 + 1) * G.cell : INFINITY;
     const float flz = z0 > 0 ? G.lo[2] + float(z0) * G.cell : -INFINITY;
     const float fhz = z1 + 1 < G.dim[2] ? G.lo[2] + float(z1 + 1) * G.cell : INFINITY;
-    const int64_t qs = tile_start[tile], qe = tile_start[tile + 1];
+    const int64_t qs = tile_start[int64_t(tile) * T.sub], qe = tile_start[int64_t(tile + 1) * T.sub];
     for (int64_t qb = qs; qb < qe; qb += kTileThreads) {
         const int64_t t = qb + threadIdx.x;
         const bool active = t < qe;
@@ -833,7 +848,9 @@ __global__ __launch_bounds__(kTileThreads) void knn_tile_kernel(
                     const int a = max(s_pref[r], cb), b = min(s_pref[r + 1], cb + cnt);
                     if (a >= cb + cnt) break;
                     if (a >= b) continue;
-                    const int yz = s_ryz[r];
+                    const int yz = s_ryz[r], xr = s_rx[r];
+                    const float rxlo = __builtin_fmaf(float(xr & 0xFFFF), G.cell, G.lo[0]);
+                    const float rxhi = __builtin_fmaf(float((xr >> 16) + 1), G.cell, G.lo[0]);
                     const float ylo = __builtin_fmaf(float(yz & 0xFFFF), G.cell, G.lo[1]);
                     const float zlo = __builtin_fmaf(float(yz >> 16), G.cell, G.lo[2]);
                     const float gy = fmaxf(fmaxf(ylo - qy, qy - (ylo + G.cell)) - T.margin, 0.f);
@@ -1511,8 +1528,9 @@ static void launch_per_lane(dwg_ctx* ctx, const dwg_knn_index* I, const float* q
 }
 
 // The queries qlist[0..nq) (or 0..nq−1, masked by qmask) bucketed by tile
-// of kTile³ fine cells: order[start[t]..start[t+1]) are tile t's queries,
-// list[0..nlist) the tiles that have any
+// of kTile³ fine cells (and by eighth of it when sub is 8): order[start[sub·t]
+// ..start[sub·(t+1))) are tile t's queries, list[0..nlist) the tiles that
+// have any
 struct TileBuckets {
     TileArgs T;
     int64_t ntiles = 0, nlist = 0;
@@ -1522,32 +1540,34 @@ struct TileBuckets {
 };
 
 static TileBuckets tile_buckets(dwg_ctx* ctx, const dwg_knn_index* I, const float* q, int64_t nq, const int* qlist,
-                                const uint8_t* qmask) {
+                                const uint8_t* qmask, int sub) {
     const KnnGridArgs& G = I->lvl[0].G;
     TileBuckets B;
     TileArgs& T = B.T;
     for (int d = 0; d < 3; ++d) T.tdim[d] = (G.dim[d] + kTile - 1) / kTile;
     T.margin = cell_margin(G);
-    const int64_t ntiles = int64_t(T.tdim[0]) * T.tdim[1] * T.tdim[2];
+    T.sub = sub;
+    const int64_t ntiles = int64_t(T.tdim[0]) * T.tdim[1] * T.tdim[2], nb = ntiles * sub;
+    DWG_REQUIRE(nb < (int64_t(1) << 31), "too many query buckets");
     B.ntiles = ntiles;
     int* tile_of = static_cast<int*>(ctx->buf("knnt_tile_of", sizeof(int) * nq));
-    int64_t* start = static_cast<int64_t*>(ctx->buf("knnt_start", sizeof(int64_t) * (ntiles + 1)));
-    unsigned long long* cursor = static_cast<unsigned long long*>(ctx->buf("knnt_cursor", sizeof(uint64_t) * ntiles));
+    int64_t* start = static_cast<int64_t*>(ctx->buf("knnt_start", sizeof(int64_t) * (nb + 1)));
+    unsigned long long* cursor = static_cast<unsigned long long*>(ctx->buf("knnt_cursor", sizeof(uint64_t) * nb));
     int* order = static_cast<int*>(ctx->buf("knnt_order", sizeof(int) * nq));
     int64_t* flags = static_cast<int64_t*>(ctx->buf("knnt_flags", sizeof(int64_t) * (ntiles + 1)));
     int* list = static_cast<int*>(ctx->buf("knnt_list", sizeof(int) * ntiles));
-    fill_u32(ctx, start, 0, 2 * (ntiles + 1));
+    fill_u32(ctx, start, 0, 2 * (nb + 1));
     hipLaunchKernelGGL(tile_count_kernel, dim3(grid_for(nq, 256)), dim3(256), 0, ctx->stream, q, nq, qlist, qmask, G, T,
                        tile_of, reinterpret_cast<unsigned long long*>(start));
     check_launch();
-    exclusive_scan<int64_t>(ctx, start, ntiles);
-    hipLaunchKernelGGL(copy_i64_kernel, dim3(grid_for(ntiles, 256)), dim3(256), 0, ctx->stream, start, cursor, ntiles);
+    exclusive_scan<int64_t>(ctx, start, nb);
+    hipLaunchKernelGGL(copy_i64_kernel, dim3(grid_for(nb, 256)), dim3(256), 0, ctx->stream, start, cursor, nb);
     check_launch();
     hipLaunchKernelGGL(tile_scatter_kernel, dim3(grid_for(nq, 256)), dim3(256), 0, ctx->stream, tile_of, nq, qlist,
                        cursor, order);
     check_launch();
     hipLaunchKernelGGL(tile_flags_kernel, dim3(grid_for(ntiles + 1, 256)), dim3(256), 0, ctx->stream, start, ntiles,
-                       flags);
+                       sub, flags);
     check_launch();
     exclusive_scan<int64_t>(ctx, flags, ntiles);
     hipLaunchKernelGGL(tile_list_kernel, dim3(grid_for(ntiles, 256)), dim3(256), 0, ctx->stream, flags, ntiles, list);
@@ -1579,7 +1599,7 @@ static int64_t tiled_pass(dwg_ctx* ctx, const dwg_knn_index* I, const float* q, 
                           int* redo, const std::string& name, const KnnScatter* sc = nullptr,
                           float* redo_kth = nullptr, bool ordered = false) {
     const KnnGridArgs& G = I->lvl[0].G;
-    const TileBuckets B = tile_buckets(ctx, I, q, nq, qlist, qmask);
+    const TileBuckets B = tile_buckets(ctx, I, q, nq, qlist, qmask, kSubTiled);
     const TileArgs& T = B.T;
     const int64_t ntiles = B.ntiles, nlist = B.nlist;
     const int64_t* start = B.start;
@@ -1587,7 +1607,7 @@ static int64_t tiled_pass(dwg_ctx* ctx, const dwg_knn_index* I, const float* q, 
     const int* list = B.list;
     unsigned int* n_redo = static_cast<unsigned int*>(ctx->buf("knnt_nredo", 64));
     fill_u32(ctx, n_redo, 0, 1);
-    if (ctx->profile) ctx->count(name + ":queries", read_scalar<int64_t>(ctx, start + ntiles));
+    if (ctx->profile) ctx->count(name + ":queries", read_scalar<int64_t>(ctx, start + ntiles * T.sub));
     // ordered: each query's place t in tile order gets its flag (and, when
     // open, its index and bound) in scratch arrays, compacted below
     int64_t* rflag = nullptr;
@@ -1669,7 +1689,7 @@ void knn_query(dwg_ctx* ctx, const dwg_knn_index* I, const float* q, int64_t nq,
         // the queries in tile order: a wave's 64 walks then start near each
         // other and take similar paths (0.84 instead of 1.19 ms for the
         // zero fill's ~30k queries)
-        const TileBuckets B = tile_buckets(ctx, I, q, nq, nullptr, nullptr);
+        const TileBuckets B = tile_buckets(ctx, I, q, nq, nullptr, nullptr, 1);
         launch_far(ctx, I, q, nq, B.order, k, skip, out_idx, out_d2, nullptr, false, pm);
         if (sc) {
             hipLaunchKernelGGL(scatter_pairs_kernel, dim3(grid_for(nq * k, 256)), dim3(256), 0, ctx->stream, out_idx, k,
