@@ -443,6 +443,8 @@ void octree_set_point_areas(dwg_ctx* ctx, dwg_octree* T, const float* nrm, const
 constexpr int kTreeThreads = 64;
 constexpr int kTreeWaves = kTreeThreads / 64;
 constexpr int kStack = 7 * 21 + 8;
+constexpr int kGridPts = 2;  // grid points per lane (bricks of 4 × 4 × 4·kGridPts)
+constexpr int kPointPts = 1;  // explicit queries per lane
 
 struct TreeArgs {
     const int4* __restrict__ info;
@@ -490,7 +492,7 @@ struct QueryArgs {
     int64_t nq;
     float ox, oy, oz, h;
     int rx, ry, rz;
-    int bx, by, bz;  // bricks per axis (4×4×4 points per brick)
+    int bx, by, bz;  // bricks per axis (4 × 4 × 4·kGridPts points per brick)
     int64_t n_bricks;
 };
 
@@ -499,55 +501,79 @@ template <bool SCREENED, bool COUNT, bool BF16, bool GRID, bool NEAR64 = false>
 __global__ __launch_bounds__(kTreeThreads) void gwn_tree_kernel(TreeArgs T, QueryArgs Q, float beta, float sigma,
                                                                  uint32_t flags, float* __restrict__ out,
                                                                  int64_t* __restrict__ terms_out) {
+    // grid walks: each lane evaluates kGridPts points (one per 4×4×4 part of
+    // the brick), so that a pop, its loads and its pushes serve them all
+    constexpr int H = GRID ? kGridPts : kPointPts;
     __shared__ int2 s_node[kTreeWaves][kStack];
-    __shared__ uint64_t s_mask[kTreeWaves][kStack];
+    __shared__ uint64_t s_mask[kTreeWaves][kStack][H];
     __shared__ float4 s_pay[kTreeWaves][16];  // the 8 children's payloads of the node being opened
     __shared__ float4 s_cmr[kTreeWaves][16];  // and their (CM, w) records (twice)
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int64_t blk = xcd_block(blockIdx.x, gridDim.x);
     const int64_t wid = blk * kTreeWaves + wave;
-    int64_t qi;
-    bool valid;
-    float qx, qy, qz;
+    int64_t qi[H];
+    bool valid[H];
+    float qx[H], qy[H], qz[H];
     if (!GRID) {
-        qi = wid * 64 + lane;
-        valid = qi < Q.nq;
-        if (Q.self_perm) {  // consecutive tree slots: spatially coherent waves
-            const float4 a = valid ? T.sa[qi] : make_float4(0.f, 0.f, 0.f, 0.f);
-            qx = a.x;
-            qy = a.y;
-            qz = a.z;
-            qi = valid ? int64_t(Q.self_perm[qi]) : 0;
-        } else {
-            qx = valid ? Q.q[3 * qi] : 0.f;
-            qy = valid ? Q.q[3 * qi + 1] : 0.f;
-            qz = valid ? Q.q[3 * qi + 2] : 0.f;
+        // H consecutive runs of 64 queries
+#pragma unroll
+        for (int h = 0; h < H; ++h) {
+            qi[h] = (wid * H + h) * 64 + lane;
+            valid[h] = qi[h] < Q.nq;
+            if (Q.self_perm) {  // consecutive tree slots: spatially coherent waves
+                const float4 a = valid[h] ? T.sa[qi[h]] : make_float4(0.f, 0.f, 0.f, 0.f);
+                qx[h] = a.x;
+                qy[h] = a.y;
+                qz[h] = a.z;
+                qi[h] = valid[h] ? int64_t(Q.self_perm[qi[h]]) : 0;
+            } else {
+                qx[h] = valid[h] ? Q.q[3 * qi[h]] : 0.f;
+                qy[h] = valid[h] ? Q.q[3 * qi[h] + 1] : 0.f;
+                qz[h] = valid[h] ? Q.q[3 * qi[h] + 2] : 0.f;
+            }
         }
     } else {
+        // brick of 4 × 4 × 4H points; part h holds z = 4h .. 4h + 3
         const int64_t br = wid;
         const int bxi = int(br % Q.bx), byi = int((br / Q.bx) % Q.by), bzi = int(br / (int64_t(Q.bx) * Q.by));
-        const int x = bxi * 4 + (lane & 3), y = byi * 4 + ((lane >> 2) & 3), z = bzi * 4 + (lane >> 4);
-        valid = br < Q.n_bricks && x < Q.rx && y < Q.ry && z < Q.rz;
-        qi = int64_t(x) + int64_t(Q.rx) * (int64_t(y) + int64_t(Q.ry) * z);
-        qx = __builtin_fmaf(float(x), Q.h, Q.ox);
-        qy = __builtin_fmaf(float(y), Q.h, Q.oy);
-        qz = __builtin_fmaf(float(z), Q.h, Q.oz);
+        const int x = bxi * 4 + (lane & 3), y = byi * 4 + ((lane >> 2) & 3);
+#pragma unroll
+        for (int h = 0; h < H; ++h) {
+            const int z = (bzi * H + h) * 4 + (lane >> 4);
+            valid[h] = br < Q.n_bricks && x < Q.rx && y < Q.ry && z < Q.rz;
+            qi[h] = int64_t(x) + int64_t(Q.rx) * (int64_t(y) + int64_t(Q.ry) * z);
+            qx[h] = __builtin_fmaf(float(x), Q.h, Q.ox);
+            qy[h] = __builtin_fmaf(float(y), Q.h, Q.oy);
+            qz[h] = __builtin_fmaf(float(z), Q.h, Q.oz);
+        }
     }
-    const uint64_t all = __builtin_amdgcn_ballot_w64(valid);
-    if (all == 0) return;
+    uint64_t all[H];
+    bool any = false;
+#pragma unroll
+    for (int h = 0; h < H; ++h) {
+        all[h] = __builtin_amdgcn_ballot_w64(valid[h]);
+        any = any || all[h] != 0;
+    }
+    if (!any) return;
     const float sl2e = sigma * kLog2e;
-    float sum = 0.f, comp = 0.f;
-    int64_t nterms = 0;
+    float sum[H], comp[H];
+    int64_t nterms[H];
+    double near[H];  // DWG_NEAR_FP64: point terms inside the screening radius, in FP64
+#pragma unroll
+    for (int h = 0; h < H; ++h) {
+        sum[h] = comp[h] = 0.f;
+        nterms[h] = 0;
+        near[h] = 0.0;
+    }
     const bool kahan = flags & DWG_KAHAN;
 
-    double near = 0.0;  // DWG_NEAR_FP64: point terms inside the screening radius, in FP64
-    auto point_term = [&](const float4 a, const float4 b, bool on, float part) {
-        const float dx = a.x - qx, dy = a.y - qy, dz = a.z - qz;
+    auto point_term = [&](const float4 a, const float4 b, bool on, float part, int h) {
+        const float dx = a.x - qx[h], dy = a.y - qy[h], dz = a.z - qz[h];
         const float r2 = dist2(dx, dy, dz);
         if constexpr (NEAR64) {
             const bool nr = on && near_field(r2, a.w);
             if (__builtin_amdgcn_ballot_w64(nr)) {  // rare: lanes next to a source
-                if (nr) near += near_term64(a, b, qx, qy, qz, sigma);
+                if (nr) near[h] += near_term64(a, b, qx[h], qy[h], qz[h], sigma);
             }
             on = on && !nr;
         }
@@ -557,35 +583,55 @@ __global__ __launch_bounds__(kTreeThreads) void gwn_tree_kernel(TreeArgs T, Quer
         if (BF16) return __builtin_fmaf((on && ok) ? x : 0.f, inv, part);  // last product fused into the sum
         return part + ((on && ok) ? x * inv : 0.f);
     };
-    auto leaf_sum = [&](int b, int e, bool act) {
-        float part = 0.f;
-        for (int j = b; j < e; ++j) part = point_term(scalar_load(T.sa, j), scalar_load(T.sb, j), act, part);
-        if (COUNT && act) nterms += e - b;
-        if (kahan) kahan_add(sum, comp, part); else sum += part;
+    auto add = [&](float part, int h) {
+        if (kahan) kahan_add(sum[h], comp[h], part); else sum[h] += part;
+    };
+    // a leaf of several points, summed point by point for the parts whose
+    // lanes opened it
+    auto leaf_sum = [&](int b, int e, const uint64_t* m) {
+        float part[H];
+#pragma unroll
+        for (int h = 0; h < H; ++h) part[h] = 0.f;
+        for (int j = b; j < e; ++j) {
+            const float4 pa = scalar_load(T.sa, j), pb = scalar_load(T.sb, j);
+#pragma unroll
+            for (int h = 0; h < H; ++h)
+                if (H == 1 || m[h]) part[h] = point_term(pa, pb, __builtin_amdgcn_inverse_ballot_w64(m[h]), part[h], h);
+        }
+#pragma unroll
+        for (int h = 0; h < H; ++h) {
+            if (COUNT && __builtin_amdgcn_inverse_ballot_w64(m[h])) nterms[h] += e - b;
+            if (H == 1 || m[h]) add(part[h], h);
+        }
     };
 
     // the stack holds (traversal record of a node to open, lanes that open
-    // it): an internal node's first child, or a leaf's (−count, begin)
+    // it, per part): an internal node's first child, or a leaf's
+    // (−count, begin)
     int sp = 0;
     const int4 root = scalar_load(T.info, 0);
     if (root.x < 0) {
-        leaf_sum(root.y, root.z, valid);
+        leaf_sum(root.y, root.z, all);
     } else {
         s_node[wave][0] = make_int2(root.x, 0);
-        s_mask[wave][0] = all;
+#pragma unroll
+        for (int h = 0; h < H; ++h) s_mask[wave][0][h] = all[h];
         sp = 1;
     }
     while (sp > 0) {
         --sp;
         const int2 top = s_node[wave][sp];
         const int fc = uniform(top.x);
-        const uint64_t mask = uniform64(s_mask[wave][sp]);
-        const bool act = __builtin_amdgcn_inverse_ballot_w64(mask);
+        uint64_t mask[H];
+#pragma unroll
+        for (int h = 0; h < H; ++h) mask[h] = uniform64(s_mask[wave][sp][h]);
         if (fc < 0) {  // a leaf of several points: summed point by point
-            leaf_sum(uniform(top.y), uniform(top.y) - fc, act);
+            leaf_sum(uniform(top.y), uniform(top.y) - fc, mask);
             continue;
         }
-        float part = 0.f;
+        float part[H];
+#pragma unroll
+        for (int h = 0; h < H; ++h) part[h] = 0.f;
         // the 8 children are contiguous: their records come in one batch of
         // scalar loads, their payloads in one coalesced vector load (word
         // 8c + w of child c in lane 8c + w), so a node costs one memory
@@ -612,67 +658,85 @@ __global__ __launch_bounds__(kTreeThreads) void gwn_tree_kernel(TreeArgs T, Quer
             // record makes the test its term's guard, a leaf of several
             // points is never far (∞) and is opened like an internal node
             const float4 cr = s_cmr[wave][c];
-            const float dx = cr.x - qx, dy = cr.y - qy, dz = cr.z - qz;
-            const float r2 = dist2(dx, dy, dz);
-            bool far = r2 > cr.w;
-            if constexpr (NEAR64) {
-                // point terms inside the screening radius go to the FP64 sum
-                if (nk.x == -1) {
-                    const bool nr = act && far && r2 < s_pay[wave][2 * c].w;
-                    if (__builtin_amdgcn_ballot_w64(nr)) {  // rare: lanes next to a source
-                        if (nr) near += near_term64(scalar_load(T.sa, nk.y), scalar_load(T.sb, nk.y), qx, qy, qz, sigma);
+            uint64_t nm[H];
+            bool push = false;
+#pragma unroll
+            for (int h = 0; h < H; ++h) {
+                nm[h] = 0;
+                if (H > 1 && !mask[h]) continue;  // no lane of this part opened the node
+                const bool act = __builtin_amdgcn_inverse_ballot_w64(mask[h]);
+                const float dx = cr.x - qx[h], dy = cr.y - qy[h], dz = cr.z - qz[h];
+                const float r2 = dist2(dx, dy, dz);
+                bool far = r2 > cr.w;
+                if constexpr (NEAR64) {
+                    // point terms inside the screening radius go to the FP64 sum
+                    if (nk.x == -1) {
+                        const bool nr = act && far && r2 < s_pay[wave][2 * c].w;
+                        if (__builtin_amdgcn_ballot_w64(nr)) {  // rare: lanes next to a source
+                            if (nr)
+                                near[h] += near_term64(scalar_load(T.sa, nk.y), scalar_load(T.sb, nk.y), qx[h], qy[h],
+                                                       qz[h], sigma);
+                        }
+                        far = far && !nr;
                     }
-                    far = far && !nr;
                 }
-            }
-            // lane masks straight from the compare: far for the lanes that
-            // opened this node, near ones open the child
-            const uint64_t farm = __builtin_amdgcn_ballot_w64(far);
-            if (mask & farm) {
-                const float4 e4 = s_pay[wave][2 * c];
-                if (BF16) {
-                    // BF16 moments carry a relative error of 2^-9, far above
-                    // the rounding of the centre: d = CM_f − q and its |d|²
-                    // from the far test serve as they are, and the term's
-                    // last product is fused into the sum.  r > 1e-10 holds
-                    // for every far node here (far radius floor) and is the
-                    // far test of a point.  The lanes that take the term are
-                    // selected by the exec mask (mask & farm), not by a
-                    // select on the result: one VALU instruction less per term
-                    if (__builtin_amdgcn_inverse_ballot_w64(mask & farm)) {
+                // lane masks straight from the compare: far for the lanes that
+                // opened this node, near ones open the child
+                const uint64_t farm = __builtin_amdgcn_ballot_w64(far);
+                if (mask[h] & farm) {
+                    const float4 e4 = s_pay[wave][2 * c];
+                    if (BF16) {
+                        // BF16 moments carry a relative error of 2^-9, far above
+                        // the rounding of the centre: d = CM_f − q and its |d|²
+                        // from the far test serve as they are, and the term's
+                        // last product is fused into the sum.  r > 1e-10 holds
+                        // for every far node here (far radius floor) and is the
+                        // far test of a point.  The lanes that take the term are
+                        // selected by the exec mask (mask & farm), not by a
+                        // select on the result: one VALU instruction less per term
+                        if (__builtin_amdgcn_inverse_ballot_w64(mask[h] & farm)) {
+                            float inv;
+                            bool ok;
+                            const float x = tree_term_split<SCREENED>(dx, dy, dz, r2, e4.w, e4, sigma, sl2e, inv, ok);
+                            part[h] = __builtin_fmaf(x, inv, part[h]);
+                        }
+                    } else {
+                        // d = (CM_f − q) + δ: the first difference is exact for
+                        // nearby values, so d carries a single rounding (δ = 0
+                        // for a point)
+                        const float4 lo = s_pay[wave][2 * c + 1];
+                        const float ex = dx + lo.x, ey = dy + lo.y, ez = dz + lo.z;
                         float inv;
                         bool ok;
-                        const float x = tree_term_split<SCREENED>(dx, dy, dz, r2, e4.w, e4, sigma, sl2e, inv, ok);
-                        part = __builtin_fmaf(x, inv, part);
+                        const float x =
+                            tree_term_split<SCREENED>(ex, ey, ez, dist2(ex, ey, ez), e4.w, e4, sigma, sl2e, inv, ok);
+                        // the plain dipole has no r > 1e-10 guard in the reference either
+                        part[h] += ((act && far && ok) ? x * inv : 0.f);
                     }
-                } else {
-                    // d = (CM_f − q) + δ: the first difference is exact for
-                    // nearby values, so d carries a single rounding (δ = 0
-                    // for a point)
-                    const float4 lo = s_pay[wave][2 * c + 1];
-                    const float ex = dx + lo.x, ey = dy + lo.y, ez = dz + lo.z;
-                    float inv;
-                    bool ok;
-                    const float x = tree_term_split<SCREENED>(ex, ey, ez, dist2(ex, ey, ez), e4.w, e4, sigma, sl2e, inv, ok);
-                    // the plain dipole has no r > 1e-10 guard in the reference either
-                    part += ((act && far && ok) ? x * inv : 0.f);
                 }
+                if (COUNT && act && (far || nk.x == -1)) nterms[h] += 1;
+                nm[h] = mask[h] & ~farm;
+                push = push || nm[h] != 0;
             }
-            if (COUNT && act && (far || nk.x == -1)) nterms += 1;
-            const uint64_t nm = mask & ~farm;
-            if (nm && nk.x != -1) {
+            if (push && nk.x != -1) {
                 // every lane writes the same entry: no divergent branch, so
                 // the stack pointer stays in a scalar register
                 s_node[wave][sp] = nk;
-                s_mask[wave][sp] = nm;
+#pragma unroll
+                for (int h = 0; h < H; ++h) s_mask[wave][sp][h] = nm[h];
                 ++sp;
             }
         }
-        if (kahan) kahan_add(sum, comp, part); else sum += part;
+#pragma unroll
+        for (int h = 0; h < H; ++h)
+            if (H == 1 || mask[h]) add(part[h], h);
     }
-    if (valid) {
-        if (out) out[qi] = NEAR64 ? float(double(sum - comp) + near) : sum - comp;
-        if (COUNT) terms_out[qi] = nterms;
+#pragma unroll
+    for (int h = 0; h < H; ++h) {
+        if (valid[h]) {
+            if (out) out[qi[h]] = NEAR64 ? float(double(sum[h] - comp[h]) + near[h]) : sum[h] - comp[h];
+            if (COUNT) terms_out[qi[h]] = nterms[h];
+        }
     }
 }
 
@@ -802,7 +866,7 @@ static QueryArgs make_grid(const dwg_grid* g) {
     Q.rz = g->res[2];
     Q.bx = (Q.rx + 3) / 4;
     Q.by = (Q.ry + 3) / 4;
-    Q.bz = (Q.rz + 3) / 4;
+    Q.bz = (Q.rz + 4 * kGridPts - 1) / (4 * kGridPts);
     Q.n_bricks = int64_t(Q.bx) * Q.by * Q.bz;
     Q.nq = int64_t(Q.rx) * Q.ry * Q.rz;
     return Q;
@@ -864,7 +928,7 @@ static void launch_tree(dwg_ctx* ctx, const dwg_octree* Tc, const QueryArgs& Q, 
         check_launch();
         T->payload_bf16 = want_bf16;
     }
-    const int64_t waves = Q.grid ? Q.n_bricks : (Q.nq + 63) / 64;
+    const int64_t waves = Q.grid ? Q.n_bricks : (Q.nq + 64 * kPointPts - 1) / (64 * kPointPts);
     if (waves == 0) return;
     TreeArgs A{T->info, T->kid, T->cm_r, reinterpret_cast<const float*>(T->payload), T->sa, T->sb};
     dim3 grid(grid_for(waves, kTreeWaves)), block(kTreeThreads);
