@@ -442,8 +442,15 @@ void octree_set_point_areas(dwg_ctx* ctx, dwg_octree* T, const float* nrm, const
 // ---------------------------------------------------------------------------
 constexpr int kTreeThreads = 64;
 constexpr int kTreeWaves = kTreeThreads / 64;
-constexpr int kStack = 7 * 21 + 8;
-constexpr int kGridPts = 2;  // grid points per lane (bricks of 4 × 4 × 4·kGridPts)
+constexpr int kMaxTreeLevel = 21;
+// stack entries a walk of a tree of max_level levels needs at most: 7 left
+// over per level and the 8 children of the deepest node opened
+__host__ __device__ constexpr int tree_stack_entries(int max_level) { return 7 * max_level + 8; }
+constexpr int kStack = tree_stack_entries(kMaxTreeLevel);  // the gradient walk's fixed stack
+// grid walks: each lane evaluates kGridParts points, one in each 4×4×4 part
+// of a brick of kPartsX × kPartsY × kPartsZ parts
+constexpr int kPartsX = 2, kPartsY = 2, kPartsZ = 1;
+constexpr int kGridParts = kPartsX * kPartsY * kPartsZ;
 constexpr int kPointPts = 1;  // explicit queries per lane
 
 struct TreeArgs {
@@ -492,8 +499,9 @@ struct QueryArgs {
     int64_t nq;
     float ox, oy, oz, h;
     int rx, ry, rz;
-    int bx, by, bz;  // bricks per axis (4 × 4 × 4·kGridPts points per brick)
+    int bx, by, bz;  // bricks per axis (kPartsX·4 × kPartsY·4 × kPartsZ·4 points per brick)
     int64_t n_bricks;
+    int stack;  // walk stack entries (tree_stack_entries of the tree's depth)
 };
 
 
@@ -503,9 +511,13 @@ __global__ __launch_bounds__(kTreeThreads) void gwn_tree_kernel(TreeArgs T, Quer
                                                                  int64_t* __restrict__ terms_out) {
     // grid walks: each lane evaluates kGridPts points (one per 4×4×4 part of
     // the brick), so that a pop, its loads and its pushes serve them all
-    constexpr int H = GRID ? kGridPts : kPointPts;
-    __shared__ int2 s_node[kTreeWaves][kStack];
-    __shared__ uint64_t s_mask[kTreeWaves][kStack][H];
+    constexpr int H = GRID ? kGridParts : kPointPts;
+    static_assert(kTreeWaves == 1, "one wave per block: the stack lives in the block's dynamic LDS");
+    // the stack: Q.stack entries of (record, H lane masks), in dynamic LDS
+    // sized by the tree's depth (a shallow tree leaves room for more blocks)
+    extern __shared__ uint64_t s_dyn[];
+    int2* s_node_w = reinterpret_cast<int2*>(s_dyn);
+    uint64_t* s_mask_w = s_dyn + Q.stack;
     __shared__ float4 s_pay[kTreeWaves][16];  // the 8 children's payloads of the node being opened
     __shared__ float4 s_cmr[kTreeWaves][16];  // and their (CM, w) records (twice)
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -533,13 +545,14 @@ __global__ __launch_bounds__(kTreeThreads) void gwn_tree_kernel(TreeArgs T, Quer
             }
         }
     } else {
-        // brick of 4 × 4 × 4H points; part h holds z = 4h .. 4h + 3
+        // brick of kPartsX × kPartsY × kPartsZ parts of 4 × 4 × 4 points
         const int64_t br = wid;
         const int bxi = int(br % Q.bx), byi = int((br / Q.bx) % Q.by), bzi = int(br / (int64_t(Q.bx) * Q.by));
-        const int x = bxi * 4 + (lane & 3), y = byi * 4 + ((lane >> 2) & 3);
 #pragma unroll
         for (int h = 0; h < H; ++h) {
-            const int z = (bzi * H + h) * 4 + (lane >> 4);
+            const int x = (bxi * kPartsX + h % kPartsX) * 4 + (lane & 3);
+            const int y = (byi * kPartsY + (h / kPartsX) % kPartsY) * 4 + ((lane >> 2) & 3);
+            const int z = (bzi * kPartsZ + h / (kPartsX * kPartsY)) * 4 + (lane >> 4);
             valid[h] = br < Q.n_bricks && x < Q.rx && y < Q.ry && z < Q.rz;
             qi[h] = int64_t(x) + int64_t(Q.rx) * (int64_t(y) + int64_t(Q.ry) * z);
             qx[h] = __builtin_fmaf(float(x), Q.h, Q.ox);
@@ -613,18 +626,18 @@ __global__ __launch_bounds__(kTreeThreads) void gwn_tree_kernel(TreeArgs T, Quer
     if (root.x < 0) {
         leaf_sum(root.y, root.z, all);
     } else {
-        s_node[wave][0] = make_int2(root.x, 0);
+        s_node_w[0] = make_int2(root.x, 0);
 #pragma unroll
-        for (int h = 0; h < H; ++h) s_mask[wave][0][h] = all[h];
+        for (int h = 0; h < H; ++h) s_mask_w[h] = all[h];
         sp = 1;
     }
     while (sp > 0) {
         --sp;
-        const int2 top = s_node[wave][sp];
+        const int2 top = s_node_w[sp];
         const int fc = uniform(top.x);
         uint64_t mask[H];
 #pragma unroll
-        for (int h = 0; h < H; ++h) mask[h] = uniform64(s_mask[wave][sp][h]);
+        for (int h = 0; h < H; ++h) mask[h] = uniform64(s_mask_w[sp * H + h]);
         if (fc < 0) {  // a leaf of several points: summed point by point
             leaf_sum(uniform(top.y), uniform(top.y) - fc, mask);
             continue;
@@ -721,9 +734,9 @@ __global__ __launch_bounds__(kTreeThreads) void gwn_tree_kernel(TreeArgs T, Quer
             if (push && nk.x != -1) {
                 // every lane writes the same entry: no divergent branch, so
                 // the stack pointer stays in a scalar register
-                s_node[wave][sp] = nk;
+                s_node_w[sp] = nk;
 #pragma unroll
-                for (int h = 0; h < H; ++h) s_mask[wave][sp][h] = nm[h];
+                for (int h = 0; h < H; ++h) s_mask_w[sp * H + h] = nm[h];
                 ++sp;
             }
         }
@@ -864,9 +877,9 @@ static QueryArgs make_grid(const dwg_grid* g) {
     Q.rx = g->res[0];
     Q.ry = g->res[1];
     Q.rz = g->res[2];
-    Q.bx = (Q.rx + 3) / 4;
-    Q.by = (Q.ry + 3) / 4;
-    Q.bz = (Q.rz + 4 * kGridPts - 1) / (4 * kGridPts);
+    Q.bx = (Q.rx + 4 * kPartsX - 1) / (4 * kPartsX);
+    Q.by = (Q.ry + 4 * kPartsY - 1) / (4 * kPartsY);
+    Q.bz = (Q.rz + 4 * kPartsZ - 1) / (4 * kPartsZ);
     Q.n_bricks = int64_t(Q.bx) * Q.by * Q.bz;
     Q.nq = int64_t(Q.rx) * Q.ry * Q.rz;
     return Q;
@@ -909,7 +922,7 @@ __global__ void payload_precision_kernel(const int2* __restrict__ kid, const flo
 static void launch_tree(dwg_ctx* ctx, const dwg_octree* Tc, const QueryArgs& Q, float beta, float sigma,
                         uint32_t flags, float* out, int64_t* terms, bool count) {
     DWG_REQUIRE(Tc && Tc->updated, "octree has no moments yet: call dwg_octree_update first");
-    DWG_REQUIRE(Tc->max_level <= 21, "octree deeper than the traversal stack");
+    DWG_REQUIRE(Tc->max_level <= kMaxTreeLevel, "octree deeper than the traversal stack");
     auto* T = const_cast<dwg_octree*>(Tc);  // the far radii follow β; the tree itself is unchanged
     const bool want_bf16 = (flags & DWG_FAR_BF16) && !count;
     const float floor = want_bf16 ? kFarFloorBf16 : 0.f;
@@ -932,13 +945,17 @@ static void launch_tree(dwg_ctx* ctx, const dwg_octree* Tc, const QueryArgs& Q, 
     if (waves == 0) return;
     TreeArgs A{T->info, T->kid, T->cm_r, reinterpret_cast<const float*>(T->payload), T->sa, T->sb};
     dim3 grid(grid_for(waves, kTreeWaves)), block(kTreeThreads);
+    // the walk's stack: one record and one lane mask per part for each entry
+    QueryArgs Qs = Q;
+    Qs.stack = tree_stack_entries(std::max(T->max_level, 1));
+    const size_t lds = sizeof(uint64_t) * size_t(Qs.stack) * size_t(1 + (Q.grid ? kGridParts : kPointPts));
     const bool scr = flags & DWG_SCREENED, bf = flags & DWG_FAR_BF16;
     const char* name = count ? "gwn_tree_terms" : (!Q.grid ? (scr ? "gwn_tree_screened" : "gwn_tree_plain")
                                                            : (scr ? "gwn_tree_grid_screened" : "gwn_tree_grid_plain"));
     ProfileScope ps(ctx, name);
     const bool near64 = scr && (flags & DWG_NEAR_FP64);
 #define DWG_TREE_LAUNCH(S, C, B, G, N) \
-    hipLaunchKernelGGL((gwn_tree_kernel<S, C, B, G, N>), grid, block, 0, ctx->stream, A, Q, beta, sigma, flags, out, terms)
+    hipLaunchKernelGGL((gwn_tree_kernel<S, C, B, G, N>), grid, block, lds, ctx->stream, A, Qs, beta, sigma, flags, out, terms)
 #define DWG_TREE_LAUNCH_Q(S, C, B, N)       \
     do {                                    \
         if (!Q.grid)                        \
@@ -978,7 +995,7 @@ void gwn_tree_grid(dwg_ctx* ctx, const dwg_octree* T, const dwg_grid* g, float b
 void gwn_tree_grad(dwg_ctx* ctx, const dwg_octree* Tc, const float* q, int64_t nq, float beta, float sigma,
                    uint32_t flags, float* out_w, float* out_g) {
     DWG_REQUIRE(Tc && Tc->updated, "octree has no moments yet: call dwg_octree_update first");
-    DWG_REQUIRE(Tc->max_level <= 21, "octree deeper than the traversal stack");
+    DWG_REQUIRE(Tc->max_level <= kMaxTreeLevel, "octree deeper than the traversal stack");
     DWG_REQUIRE(nq >= 0 && (q || nq == 0) && (out_w || nq == 0), "bad query or output array");
     auto* T = const_cast<dwg_octree*>(Tc);  // far radii follow β and the payload precision; the tree is unchanged
     const float guard = (flags & DWG_SCREENED) ? 1e-20f : -1.f;
