@@ -72,15 +72,26 @@ __global__ __launch_bounds__(kScanThreads) void tile_scan(T* __restrict__ d, int
 }
 
 template <class T>
-__global__ void single_scan(T* __restrict__ d, int64_t n) {
-    // tiny inputs: one thread (n ≤ kScanTile)
-    T run = T();
-    for (int64_t i = 0; i < n; ++i) {
-        T v = d[i];
-        d[i] = run;
-        run = run + v;
+__global__ __launch_bounds__(kScanThreads) void single_scan(T* __restrict__ d, int64_t n) {
+    // inputs of at most kScanTile elements (the top level of a scan): one
+    // block, kScanItems consecutive elements a thread
+    __shared__ T lds[kScanThreads];
+    const int64_t base = int64_t(threadIdx.x) * kScanItems;
+    T items[kScanItems];
+    T s = T();
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k) {
+        items[k] = (base + k < n) ? d[base + k] : T();
+        s = s + items[k];
     }
-    d[n] = run;
+    T total;
+    T run = block_exclusive_scan(s, lds, total);
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k) {
+        if (base + k < n) d[base + k] = run;
+        run = run + items[k];
+    }
+    if (threadIdx.x == kScanThreads - 1) d[n] = total;
 }
 
 __global__ void fill_u32_kernel(uint32_t* d, uint32_t v, int64_t n) {
@@ -222,7 +233,7 @@ void radix_sort_pairs(dwg_ctx* ctx, const uint64_t* kin, const int* vin, uint64_
 template <class T>
 void exclusive_scan(dwg_ctx* ctx, T* d, int64_t n, int level) {
     if (n <= kScanTile) {
-        hipLaunchKernelGGL(single_scan<T>, dim3(1), dim3(1), 0, ctx->stream, d, n);
+        hipLaunchKernelGGL(single_scan<T>, dim3(1), dim3(kScanThreads), 0, ctx->stream, d, n);
         check_launch();
         return;
     }
