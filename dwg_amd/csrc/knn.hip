@@ -626,6 +626,25 @@ constexpr int kHalo3 = 9;
 // per tile (0.86 instead of 0.93 ms)
 constexpr int kSubTiled = 8;
 
+// exclusive sum over the block's threads (NT threads, a multiple of 64);
+// every thread calls it; wsum holds NT / 64 ints of LDS
+template <int NT>
+__device__ __forceinline__ int block_exclusive_sum(int v, int* wsum) {
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    int incl = v;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const int y = __shfl_up(incl, off, 64);
+        if (lane >= off) incl += y;
+    }
+    if (lane == 63) wsum[w] = incl;
+    __syncthreads();
+    int before = 0;
+#pragma unroll
+    for (int i = 0; i < NT / 64; ++i) before += i < w ? wsum[i] : 0;
+    return before + incl - v;
+}
+
 struct TileArgs {
     int tdim[3];   // tiles per axis
     float margin;  // rounding allowance of cell boundaries (cell_margin)
@@ -724,6 +743,7 @@ __global__ __launch_bounds__(kTileThreads) void knn_tile_kernel(
     __shared__ int s_pref[kRows + 1];
     __shared__ int s_ryz[kRows];  // y | z << 16 of each row
     __shared__ int s_rx[kRows];   // its cells xa | xb << 16
+    __shared__ int s_wsum[kTileThreads / 64];
     __shared__ int s_ring[HALO + 2];
     const int tile = tiles[blockIdx.x];
     const int tx = tile % T.tdim[0], ty = (tile / T.tdim[0]) % T.tdim[1], tz = tile / (T.tdim[0] * T.tdim[1]);
@@ -734,13 +754,15 @@ __global__ __launch_bounds__(kTileThreads) void knn_tile_kernel(
     if (threadIdx.x < HALO + 2) s_ring[threadIdx.x] = 0;
     __syncthreads();
     // each (y, z) row of cells x0..x1 (or each of its parts) is one
-    // contiguous range of points; a thread takes rows r0 = threadIdx.x and
-    // r0 + 256 (kRows ≤ 512)
-    static_assert(kRows <= 2 * kTileThreads, "at most two rows per thread");
-    int ring[2] = {0, 0}, rank[2] = {0, 0}, rc[2] = {0, 0}, rx[2] = {0, 0};
-    int64_t rb[2] = {0, 0};
+    // contiguous range of points; a thread takes rows threadIdx.x +
+    // h · kTileThreads, h < kRowIters
+    constexpr int kRowIters = (kRows + kTileThreads - 1) / kTileThreads;
+    int ring[kRowIters], rank[kRowIters], rc[kRowIters], rx[kRowIters];
+    int64_t rb[kRowIters];
 #pragma unroll
-    for (int h = 0; h < 2; ++h) {
+    for (int h = 0; h < kRowIters; ++h) {
+        ring[h] = rank[h] = rc[h] = rx[h] = 0;
+        rb[h] = 0;
         const int r = threadIdx.x + h * kTileThreads;
         if (r < nrows) {
             const int row = r / kParts, part = r % kParts;
@@ -770,7 +792,7 @@ __global__ __launch_bounds__(kTileThreads) void knn_tile_kernel(
         for (int r = 1; r <= HALO + 1; ++r) s_ring[r] += s_ring[r - 1];
     __syncthreads();
 #pragma unroll
-    for (int h = 0; h < 2; ++h) {
+    for (int h = 0; h < kRowIters; ++h) {
         if (threadIdx.x + h * kTileThreads < nrows) {
             const int pos = s_ring[ring[h]] + rank[h];
             const int r = (threadIdx.x + h * kTileThreads) / kParts;
@@ -781,9 +803,24 @@ __global__ __launch_bounds__(kTileThreads) void knn_tile_kernel(
         }
     }
     __syncthreads();
-    if (threadIdx.x == 0) {
-        s_pref[0] = 0;
-        for (int r = 0; r < nrows; ++r) s_pref[r + 1] = s_pref[r] + s_cnt[r];
+    {
+        // s_pref: exclusive prefix of the rows' point counts, thread t
+        // owning rows kRowIters·t .. kRowIters·t + kRowIters − 1
+        int loc[kRowIters], sum = 0;
+#pragma unroll
+        for (int i = 0; i < kRowIters; ++i) {
+            const int r = threadIdx.x * kRowIters + i;
+            loc[i] = r < nrows ? s_cnt[r] : 0;
+            sum += loc[i];
+        }
+        int run = block_exclusive_sum<kTileThreads>(sum, s_wsum);
+#pragma unroll
+        for (int i = 0; i < kRowIters; ++i) {
+            const int r = threadIdx.x * kRowIters + i;
+            if (r <= nrows) s_pref[r] = run;
+            run += loc[i];
+        }
+        if (threadIdx.x == kTileThreads - 1) s_pref[nrows] = run;  // the total, also when nrows fills every slot
     }
     __syncthreads();
     const int total = s_pref[nrows];
