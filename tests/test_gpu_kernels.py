@@ -126,6 +126,34 @@ def test_tree_grid_matches_points(ctx):
     assert t.grid_terms(grid, 2.3) == o.eval(G.astype(float), 2.3, kind="plain", count_terms=True)[1].sum()
 
 
+def test_tree_grid_deep_octree(ctx):
+    """The grid walk on an octree of the deepest depth the walk accepts (21
+    levels: a tight cluster of nearly coincident points splits to the
+    depth limit), where its stack is sized by the tree's depth: per-query
+    term counts equal the oracle's, values match the explicit-point walk
+    and the FP64 oracle, with the four points per lane of the grid bricks
+    and partial bricks at every grid edge."""
+    P, N, A = _sphere(2000, 40)
+    P = 0.5 * P
+    rng = np.random.default_rng(41)
+    C = (P[7] + rng.normal(0, 1e-6, (300, 3))).astype(np.float32)
+    P = np.concatenate([P, C]).astype(np.float32)
+    N = np.concatenate([N, np.repeat(N[7:8], len(C), 0)]).astype(np.float32)
+    A = np.full(len(P), 2e-6, np.float32)
+    t, o = _tree_pair(ctx, P, N, A, max_depth=21)
+    assert t.info()["depth"] >= 19
+    origin, h, res = O.voxel_grid(P.astype(float), 0.05, 21)
+    grid = gpu.make_grid(origin, h, res)
+    G = _grid_points_f32(grid.origin, grid.h, res)
+    for beta in (1.0, 2.3):
+        wg = t.eval_grid(grid, beta, sigma=20.0).cpu().numpy()
+        wp = t.eval(G, beta, sigma=20.0).cpu().numpy()
+        _close(wg, wp, atol=1e-6 * np.abs(wp).max(), rtol=1e-5)
+        ref = o.eval(G.astype(float), beta, kind="screened", sigma=20.0)
+        _close(wg, ref, atol=2e-5 * np.abs(ref).max(), rtol=2e-4)
+        assert t.grid_terms(grid, beta) == o.eval(G.astype(float), beta, kind="plain", count_terms=True)[1].sum()
+
+
 def test_tree_bf16_far_field_error(ctx):
     P, N, A = _sphere(20000, 11)
     t, o = _tree_pair(ctx, P, N, A)
