@@ -1525,6 +1525,7 @@ static void launch_far(dwg_ctx* ctx, const dwg_knn_index* I, const float* q, int
                        qlist, k, skip, init_kth, out_idx, out_d2)
         if (k <= 8) DWG_KNNPW(8);
         else if (k <= 10) DWG_KNNPW(10);
+        else if (k == 15) DWG_KNNPW(15);  // the diffusion's 15 neighbours: no spare slot to sink through
         else if (k <= 16) DWG_KNNPW(16);
         else if (k <= 17) DWG_KNNPW(17);  // the learned start's 16-point patches with the point itself
         else DWG_KNNPW(32);
@@ -1674,6 +1675,7 @@ static int64_t tiled_pass(dwg_ctx* ctx, const dwg_knn_index* I, const float* q, 
     } while (0)
         if (k <= 8) DWG_KNNT(8);
         else if (k <= 10) DWG_KNNT(10);
+        else if (k == 15) DWG_KNNT(15);  // the diffusion's 15 neighbours: no spare slot to sink through
         else if (k <= 16) DWG_KNNT(16);
         else if (k <= 17) DWG_KNNT(17);  // the learned start's 16-point patches with the point itself
         else DWG_KNNT(32);

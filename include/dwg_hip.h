@@ -305,7 +305,7 @@ typedef struct dwg_options {
     float offset_1;              /* --offset_1, 0.05 */
     float offset_2;              /* --offset_2, 0.01 */
     int32_t use_leaves_generate; /* --use_leaves_generate, 0 */
-    int32_t total_iter;          /* --total_iter, 40 */
+    int32_t total_iter;          /* --total_iter, 40 (at least 1) */
     int32_t test_iter;           /* --test_iter, 999999 */
     int32_t output_mode;         /* --output_mode, 1 */
     int32_t source_mode;         /* 0: octree-leaf centres as the reference, 1: the input points */

@@ -109,7 +109,7 @@ def test_knn_sparse_queries_pyramid(ctx, k):
     assert np.array_equal(d2.cpu().numpy(), rd2)
 
 
-@pytest.mark.parametrize("k", [10, 16])
+@pytest.mark.parametrize("k", [10, 15, 16])
 def test_knn_far_queries_wave_shared(ctx, k):
     # as many queries as points, most of them on a dense sheet far from every
     # point (a spurious surface piece's centroids): the far ones share one
