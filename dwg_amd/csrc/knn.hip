@@ -477,22 +477,43 @@ dwg_knn_index* knn_build(dwg_ctx* ctx, const float* p, int64_t n, int k_hint) {
 // best[0] is the k-th smallest key so far (the insertion threshold), the
 // slots k..K−1 hold 0 and never move (every key is ≥ 0).  Every access uses
 // a constant index, so the list is never demoted to scratch memory.
+// An empty slot: the bits of +∞ as a double, above every key (the high
+// word of a key is the float bits of a d² ≥ 0, at most 0x7F800000)
+constexpr uint64_t kNoKey = 0x7FF0000000000000ull;
+
 template <int K>
 __device__ __forceinline__ void knn_reset(uint64_t (&best)[K], int k) {
 #pragma unroll
-    for (int j = 0; j < K; ++j) best[j] = j < k ? ~0ull : 0ull;
+    for (int j = 0; j < K; ++j) best[j] = j < k ? kNoKey : 0ull;
 }
 
-// key < best[0]: it replaces the largest key and sinks to its place
+// key < best[0]: it replaces the largest key and sinks to its place.  Every
+// key (and kNoKey, and the 0 of an unused slot) read as a double is a
+// non-negative number, never NaN, and non-negative doubles order as their
+// bit patterns do (the kernels keep FP64 denormals): each compare-exchange
+// is one FP64 max and one min, which return one of their operands bit for
+// bit, instead of a 64-bit compare and four selects.  Written as the two
+// instructions themselves: the compiler would first quiet possible
+// signalling NaNs of both operands, which these keys never are
+__device__ __forceinline__ uint64_t key_max(uint64_t a, uint64_t b) {
+    uint64_t r;
+    asm("v_max_f64 %0, %1, %2" : "=v"(r) : "v"(a), "v"(b));
+    return r;
+}
+__device__ __forceinline__ uint64_t key_min(uint64_t a, uint64_t b) {
+    uint64_t r;
+    asm("v_min_f64 %0, %1, %2" : "=v"(r) : "v"(a), "v"(b));
+    return r;
+}
+
 template <int K>
 __device__ __forceinline__ void knn_insert(uint64_t (&best)[K], uint64_t key) {
     best[0] = key;
 #pragma unroll
     for (int i = 0; i < K - 1; ++i) {
         const uint64_t a = best[i], b = best[i + 1];
-        const bool sw = a < b;
-        best[i] = sw ? b : a;
-        best[i + 1] = sw ? a : b;
+        best[i] = key_max(a, b);
+        best[i + 1] = key_min(a, b);
     }
 }
 
@@ -505,8 +526,8 @@ __device__ __forceinline__ void knn_store(const uint64_t (&best)[K], int k, int6
         if (j < k) {
             const uint64_t b = best[j];
             const int64_t o = row * k + (k - 1 - j);
-            out_idx[o] = b == ~0ull ? -1 : int32_t(uint32_t(b));
-            if (out_d2) out_d2[o] = b == ~0ull ? INFINITY : __uint_as_float(uint32_t(b >> 32));
+            out_idx[o] = b == kNoKey ? -1 : int32_t(uint32_t(b));
+            if (out_d2) out_d2[o] = b == kNoKey ? INFINITY : __uint_as_float(uint32_t(b >> 32));
         }
     }
 }
@@ -562,7 +583,7 @@ __device__ bool search_level(const KnnGridArgs& G, const int64_t* __restrict__ c
         if (c0z - r > 0) dmin = fminf(dmin, qz - (G.lo[2] + float(c0z - r) * G.cell));
         if (c0z + r + 1 < G.dim[2]) dmin = fminf(dmin, (G.lo[2] + float(c0z + r + 1) * G.cell) - qz);
         if (dmin == INFINITY) return true;  // the block covers the whole grid
-        if (kth != ~0ull) {
+        if (kth != kNoKey) {
             const float kd2 = __uint_as_float(uint32_t(kth >> 32));
             const float safe = dmin * (1.0f - 1e-5f) - 1e-6f * G.cell;
             if (safe > 0.f && kd2 < safe * safe) return true;
@@ -865,7 +886,7 @@ __global__ __launch_bounds__(kTileThreads) void knn_tile_kernel(
                 const uint64_t key = (uint64_t(__float_as_uint(d2)) << 32) | uint32_t(idx);
                 if (key < best[0]) {
                     knn_insert<K>(best, key);
-                    kd2 = best[0] == ~0ull ? INFINITY : __uint_as_float(uint32_t(best[0] >> 32));
+                    kd2 = best[0] == kNoKey ? INFINITY : __uint_as_float(uint32_t(best[0] >> 32));
                 }
             };
             if (active && HALO == kHalo1) {
@@ -901,7 +922,7 @@ __global__ __launch_bounds__(kTileThreads) void knn_tile_kernel(
         if (!active) continue;
         const float dmin = fminf(fminf(fminf(qx - flx, fhx - qx), fminf(qy - fly, fhy - qy)), fminf(qz - flz, fhz - qz));
         bool exact = dmin == INFINITY;
-        if (!exact && best[0] != ~0ull) {
+        if (!exact && best[0] != kNoKey) {
             const float safe = dmin * (1.0f - 1e-5f) - 1e-6f * G.cell;
             exact = safe > 0.f && kd2 < safe * safe;
         }
@@ -933,7 +954,7 @@ __global__ __launch_bounds__(kTileThreads) void knn_tile_kernel(
             for (int d = 0; d < 3; ++d) v[d] = __double2ll_rn(double(sc.val[3 * int64_t(qi) + d]) * kScatterFix);
 #pragma unroll
             for (int j = 0; j < K; ++j) {
-                if (j >= k || best[j] == ~0ull) continue;
+                if (j >= k || best[j] == kNoKey) continue;
                 const int idx = int(uint32_t(best[j]));
                 int h = int((uint32_t(idx) * 0x9E3779B1u) >> 23);  // 9 bits
                 bool done = false;
@@ -1144,7 +1165,7 @@ __global__ __launch_bounds__(256) void knn_wave_kernel(KnnGridArgs G, const int6
         if (c0z + r + 1 < G.dim[2]) dmin = fminf(dmin, (G.lo[2] + float(c0z + r + 1) * G.cell) - qz);
         if (dmin == INFINITY) break;
         const uint64_t bound = wave_min_u64(best[0]);
-        if (bound != ~0ull) {
+        if (bound != kNoKey) {
             const float kd2 = __uint_as_float(uint32_t(bound >> 32));
             const float safe = dmin * (1.0f - 1e-5f) - 1e-6f * G.cell;
             if (safe > 0.f && kd2 < safe * safe) break;
@@ -1159,19 +1180,19 @@ __global__ __launch_bounds__(256) void knn_wave_kernel(KnnGridArgs G, const int6
     for (int s = 0; s < K - k; ++s) {
 #pragma unroll
         for (int c = 0; c < K - 1; ++c) asc[c] = asc[c + 1];
-        asc[K - 1] = ~0ull;
+        asc[K - 1] = kNoKey;
     }
     for (int j = 0; j < k; ++j) {
         const uint64_t m = wave_min_u64(asc[0]);
-        const uint64_t win = __ballot(asc[0] == m && m != ~0ull);
+        const uint64_t win = __ballot(asc[0] == m && m != kNoKey);
         if (win && lane == __ffsll((unsigned long long)win) - 1) {
 #pragma unroll
             for (int c = 0; c < K - 1; ++c) asc[c] = asc[c + 1];
-            asc[K - 1] = ~0ull;
+            asc[K - 1] = kNoKey;
         }
         if (lane == 0) {
-            out_idx[i * k + j] = m == ~0ull ? -1 : int32_t(uint32_t(m));
-            if (out_d2) out_d2[i * k + j] = m == ~0ull ? INFINITY : __uint_as_float(uint32_t(m >> 32));
+            out_idx[i * k + j] = m == kNoKey ? -1 : int32_t(uint32_t(m));
+            if (out_d2) out_d2[i * k + j] = m == kNoKey ? INFINITY : __uint_as_float(uint32_t(m >> 32));
         }
     }
 }
@@ -1295,7 +1316,7 @@ __global__ __launch_bounds__(kPyrThreads) void knn_pyramid_kernel(KnnGridArgs G,
                 bc = c;
             }
         }
-        const float kth = best[0] == ~0ull ? cap : fminf(cap, __uint_as_float(uint32_t(best[0] >> 32)));
+        const float kth = best[0] == kNoKey ? cap : fminf(cap, __uint_as_float(uint32_t(best[0] >> 32)));
         if (bc < 0 || !(bd <= kth)) {  // nothing left here can hold a closer point
             ++l;
             continue;
@@ -1376,7 +1397,7 @@ __global__ __launch_bounds__(kPwWaves * 64) void knn_pyramid_wave_kernel(KnnGrid
     const float cap = (valid && init_kth) ? init_kth[t] : INFINITY;
     const uint64_t all = __builtin_amdgcn_ballot_w64(valid);
     if (all == 0) return;
-    auto kth_of = [&]() { return best[0] == ~0ull ? cap : fminf(cap, __uint_as_float(uint32_t(best[0] >> 32))); };
+    auto kth_of = [&]() { return best[0] == kNoKey ? cap : fminf(cap, __uint_as_float(uint32_t(best[0] >> 32))); };
     int4* stk = s_stk[wave];
     float4* box = s_box[wave];
     const int top = Y.levels - 1;
