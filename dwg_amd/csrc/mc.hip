@@ -173,7 +173,9 @@ __device__ __forceinline__ void emit_cube(const McArgs& A, int x, int y, int z, 
 constexpr int kCountThreads = kBx * kBy;
 static_assert(kCountThreads == 64, "one wave per brick");
 
-__global__ __launch_bounds__(kCountThreads) void mc_count_kernel(McArgs A, McBricks B, int64_t* __restrict__ tile_counts) {
+__global__ __launch_bounds__(kCountThreads) void mc_count_kernel(McArgs A, McBricks B, int64_t* __restrict__ tile_counts,
+                                                                  int* __restrict__ list,
+                                                                  unsigned long long* __restrict__ nlist) {
     __shared__ uint8_t s_ntri[256];  // triangles of each case, read per lane from LDS
 #pragma unroll
     for (int c = threadIdx.x; c < 256; c += kCountThreads) s_ntri[c] = uint8_t(c_mc_rows[c] >> 60);
@@ -197,14 +199,18 @@ __global__ __launch_bounds__(kCountThreads) void mc_count_kernel(McArgs A, McBri
         }
     }
     const int incl = wave_inclusive_sum(cnt);
-    if (threadIdx.x == 63) tile_counts[b] = incl;
+    if (threadIdx.x == 63) {
+        tile_counts[b] = incl;
+        // the bricks with triangles, in any order: the emit pass takes one
+        // block each (its output place comes from the scan of the counts)
+        if (incl > 0) list[atomicAdd(nlist, 1ull)] = int(b);
+    }
 }
 
 __global__ __launch_bounds__(kMcThreads) void mc_emit_kernel(McArgs A, McBricks B, const int64_t* __restrict__ tile_offsets,
-                                                             float* __restrict__ tris, float* __restrict__ cent,
-                                                             float* __restrict__ nrm) {
-    const int64_t b = brick_of_block();
-    if (tile_offsets[b + 1] == tile_offsets[b]) return;  // no triangle in this brick
+                                                             const int* __restrict__ list, float* __restrict__ tris,
+                                                             float* __restrict__ cent, float* __restrict__ nrm) {
+    const int64_t b = list[blockIdx.x];  // a brick with triangles
     int x, y, z0;
     column_of(B, b, x, y, z0);
     int cases[kZPer];
@@ -238,19 +244,29 @@ int64_t marching_cubes(dwg_ctx* ctx, const float* field, const dwg_grid* g, floa
     const McBricks B{(A.rx - 1 + kBx - 1) / kBx, (A.ry - 1 + kBy - 1) / kBy, (A.rz - 1 + kBz - 1) / kBz};
     const int64_t tiles = int64_t(B.nbx) * B.nby * B.nbz;
     DWG_REQUIRE(tiles < (int64_t(1) << 31), "grid too large for the marching-cubes launch");
-    int64_t* offs = static_cast<int64_t*>(ctx->buf("mc_tiles", sizeof(int64_t) * (tiles + 1)));
+    // brick counts (scanned in place: offsets, the total at [tiles]) and, at
+    // [tiles + 1], the number of bricks with triangles
+    int64_t* offs = static_cast<int64_t*>(ctx->buf("mc_tiles", sizeof(int64_t) * (tiles + 2)));
+    int* list = static_cast<int*>(ctx->buf("mc_list", sizeof(int) * tiles));
+    auto* nlist = reinterpret_cast<unsigned long long*>(offs + tiles + 1);
+    fill_u32(ctx, nlist, 0u, 2);
     {
         ProfileScope ps(ctx, "mc_count");
-        hipLaunchKernelGGL(mc_count_kernel, dim3(unsigned(tiles)), dim3(kCountThreads), 0, ctx->stream, A, B, offs);
+        hipLaunchKernelGGL(mc_count_kernel, dim3(unsigned(tiles)), dim3(kCountThreads), 0, ctx->stream, A, B, offs,
+                           list, nlist);
         check_launch();
         exclusive_scan<int64_t>(ctx, offs, tiles);
     }
-    const int64_t total = read_scalar<int64_t>(ctx, offs + tiles);
+    struct Totals {
+        int64_t tris, bricks;
+    };
+    const Totals tot = read_scalar<Totals>(ctx, reinterpret_cast<const Totals*>(offs + tiles));
+    const int64_t total = tot.tris;
     if (total > capacity) throw Error(DWG_ERR_CAPACITY, "marching cubes output exceeds capacity", total);
     if (total > 0 && (tris || cent || nrm)) {
         ProfileScope ps(ctx, "mc_emit");
-        hipLaunchKernelGGL(mc_emit_kernel, dim3(unsigned(tiles)), dim3(kMcThreads), 0, ctx->stream, A, B, offs, tris,
-                           cent, nrm);
+        hipLaunchKernelGGL(mc_emit_kernel, dim3(unsigned(tot.bricks)), dim3(kMcThreads), 0, ctx->stream, A, B, offs,
+                           list, tris, cent, nrm);
         check_launch();
     }
     return total;
