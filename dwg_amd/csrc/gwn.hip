@@ -438,7 +438,8 @@ void octree_set_point_areas(dwg_ctx* ctx, dwg_octree* T, const float* nrm, const
 }
 
 // ---------------------------------------------------------------------------
-// Barnes-Hut traversal, one wave per 64 queries
+// Barnes-Hut traversal, one wave per brick of grid points (kGridParts per
+// lane) or per 64 explicit queries
 // ---------------------------------------------------------------------------
 constexpr int kTreeThreads = 64;
 constexpr int kTreeWaves = kTreeThreads / 64;
@@ -509,8 +510,8 @@ template <bool SCREENED, bool COUNT, bool BF16, bool GRID, bool NEAR64 = false>
 __global__ __launch_bounds__(kTreeThreads) void gwn_tree_kernel(TreeArgs T, QueryArgs Q, float beta, float sigma,
                                                                  uint32_t flags, float* __restrict__ out,
                                                                  int64_t* __restrict__ terms_out) {
-    // grid walks: each lane evaluates kGridPts points (one per 4×4×4 part of
-    // the brick), so that a pop, its loads and its pushes serve them all
+    // grid walks: each lane evaluates kGridParts points (one per 4×4×4 part
+    // of the brick), so that a pop, its loads and its pushes serve them all
     constexpr int H = GRID ? kGridParts : kPointPts;
     static_assert(kTreeWaves == 1, "one wave per block: the stack lives in the block's dynamic LDS");
     // the stack: Q.stack entries of (record, H lane masks), in dynamic LDS
