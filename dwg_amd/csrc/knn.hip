@@ -633,7 +633,7 @@ __global__ __launch_bounds__(256) void knn_query_kernel(KnnGridArgs G0, const in
 // ---------------------------------------------------------------------------
 constexpr double kScatterFix = 1099511627776.0;  // 2^40
 constexpr int kTile = 4;
-constexpr int kStage = 1024;
+constexpr int kStage = 512;
 constexpr int kTileThreads = 256;
 // halos of the tiled passes: most queries close within 2 cells; the ones
 // that do not (off-surface centroids) get passes with 5 and 9 cells
@@ -749,7 +749,7 @@ __global__ __launch_bounds__(kTileThreads) void knn_tile_kernel(
     // (by Chebyshev ring from the tile) starts with the tile's own cells
     constexpr int kParts = HALO == kHalo1 ? 5 : 1;
     constexpr int kRows = kRegion * kRegion * kParts;
-    constexpr int kHash = SCATTER ? 512 : 1;
+    constexpr int kHash = SCATTER ? 256 : 1;
     __shared__ int s_hkey[kHash];
     __shared__ unsigned long long s_hacc[kHash][3];
     if (SCATTER) {
@@ -956,7 +956,7 @@ __global__ __launch_bounds__(kTileThreads) void knn_tile_kernel(
             for (int j = 0; j < K; ++j) {
                 if (j >= k || best[j] == kNoKey) continue;
                 const int idx = int(uint32_t(best[j]));
-                int h = int((uint32_t(idx) * 0x9E3779B1u) >> 23);  // 9 bits
+                int h = int((uint32_t(idx) * 0x9E3779B1u) >> 24);  // 8 bits
                 bool done = false;
                 for (int probe = 0; probe < 8 && !done; ++probe) {
                     const int prev = atomicCAS(&s_hkey[h], -1, idx);
