@@ -130,3 +130,30 @@ def test_knn_far_queries_wave_shared(ctx, k):
     ref, rd2 = O.knn_brute(P, Q, k, return_d2=True)
     assert np.array_equal(got.cpu().numpy(), ref)
     assert np.array_equal(d2.cpu().numpy(), rd2)
+
+
+@pytest.mark.parametrize("k", [10, 15])
+def test_knn_denormal_and_zero_distances(ctx, k):
+    # a cluster near the origin with points 1e-21 apart (and duplicates):
+    # the squared distances are float denormals or 0, so the keys read as
+    # doubles are denormals, which the candidate lists order with FP64
+    # max/min; both the path of dense query sets (as many queries as points)
+    # and that of fewer queries
+    rng = np.random.default_rng(81 + k)
+    S = _noisy_sphere(20_000, 82)
+    C = np.zeros((300, 3), np.float32)
+    C[:, 0] = (np.arange(300) * 1e-21).astype(np.float32)
+    C = np.concatenate([C, C[:50]])
+    P = np.concatenate([S, C]).astype(np.float32)
+    Qc = C[rng.integers(0, len(C), 16_000)].copy()
+    Qc[:, 1] += (rng.uniform(-3e-22, 3e-22, len(Qc))).astype(np.float32)
+    Qs = S[rng.integers(0, len(S), 6000)] + rng.normal(0, 0.002, (6000, 3)).astype(np.float32)
+    idx = gpu.KnnIndex(ctx, P, k_hint=k)
+    for Q in (np.concatenate([Qc, Qs]).astype(np.float32), np.concatenate([Qc[:4000], Qs[:1000]]).astype(np.float32)):
+        got, d2 = idx.query(Q, k, d2=True)
+        ref, rd2 = O.knn_brute(P, Q, k, return_d2=True)
+        assert np.array_equal(got.cpu().numpy(), ref)
+        assert np.array_equal(d2.cpu().numpy(), rd2)
+        tiny = rd2[: len(Q) - len(Q) // 5]
+        assert np.mean((tiny > 0) & (tiny < np.finfo(np.float32).tiny)) > 0.5
+        assert np.any(tiny == 0)
