@@ -95,7 +95,8 @@ public:
     // octree was built with (SetSources with d_area builds it area-weighted)
     void SetAreas(const float* d_area);
     // Radii of the sources from their k nearest neighbours (GetRadiiandAs /
-    // TurboDWG::UpdateRadii).
+    // TurboDWG::UpdateRadii); only with --use_radii, the one option under
+    // which the sums read them.
     void UpdateRadii(int k);
     void GaussMapInit();
     // learned initialisation: MLP normals from the k-NN patches of the sources
@@ -141,8 +142,9 @@ private:
     DevArray<float> pts_, nrm_, area_, radii_, w_src_, w_grid_, cent_, tri_nrm_;
     Grid grid_{};
     dwg_octree* tree_ = nullptr;
-    dwg_knn_index* knn_ = nullptr;
+    dwg_knn_index* knn_ = nullptr;  // the sources' kNN index, built on first use (index())
     std::string dump_dir_;
+    dwg_knn_index* index();
 };
 
 // test3D entry points (DWGProcess.h) over in-memory points.

@@ -279,7 +279,11 @@ void TurboDWG::SetSources(const double* d_pts, int64_t n, const float* d_nrm, co
     } else {
         check(dwg_octree_build_device(ctx_, d_pts, nullptr, n_, opt_.max_depth, opt_.leaf_size, &tree_), "octree");
     }
-    check(dwg_knn_build(ctx_, pts_.ptr, n_, 15, &knn_), "knn index");
+}
+
+dwg_knn_index* TurboDWG::index() {
+    if (!knn_) check(dwg_knn_build(ctx_, pts_.ptr, n_, 15, &knn_), "knn index");
+    return knn_;
 }
 
 void TurboDWG::SetAreas(const float* d_area) {
@@ -288,7 +292,10 @@ void TurboDWG::SetAreas(const float* d_area) {
 }
 
 void TurboDWG::UpdateRadii(int k) {
-    check(dwg_radii(ctx_, knn_, pts_.ptr, n_, k, 0.0015f, 0.015f, radii_.ptr), "radii");
+    // without --use_radii no sum reads the radii (the reference computes them
+    // regardless, DWGProcess.h:283-343)
+    if (!opt_.use_radii) return;
+    check(dwg_radii(ctx_, index(), pts_.ptr, n_, k, 0.0015f, 0.015f, radii_.ptr), "radii");
 }
 
 void TurboDWG::GaussMapInit() {
@@ -384,7 +391,7 @@ double TurboDWG::TurboScreenedDWGLaunch(int total_iters, double* w_mean, int64_t
         if (want_residual)
             DWG_HIP_CHECK(hipMemcpyAsync(prev.ptr, nrm_.ptr, sizeof(float) * 3 * n_, hipMemcpyDeviceToDevice,
                                          ctx_->stream));
-        check(dwg_diffuse_normals(ctx_, knn_, pts_.ptr, n_, cent_.ptr, tri_nrm_.ptr, ntri, 15, 10, 10, nrm_.ptr, &nz),
+        check(dwg_diffuse_normals(ctx_, index(), pts_.ptr, n_, cent_.ptr, tri_nrm_.ptr, ntri, 15, 10, 10, nrm_.ptr, &nz),
               "diffuse");
         if (iter_zeros) iter_zeros[it] = nz;
         double res = -1.0;
@@ -506,7 +513,7 @@ void TurboDWG::GenerateMesh(const double* d_points, int64_t np, const Grid& fina
     // normals of the points from their normal_kw nearest sources (LoadNormals)
     DevArray<float> dp(3 * np), dn(3 * np);
     dwg::to_f32(ctx_, d_points, 3 * np, dp.ptr);
-    check(dwg_transfer_normals(ctx_, knn_, nrm_.ptr, dp.ptr, np, opt_.normal_kw, dn.ptr), "transfer normals");
+    check(dwg_transfer_normals(ctx_, index(), nrm_.ptr, dp.ptr, np, opt_.normal_kw, dn.ptr), "transfer normals");
     // --com_As 1: Voronoi areas of the points from their 15 nearest points and
     // the transferred normals (PreGeneration + point_areas + UpdateAs,
     // DWGProcess.h:771-792)
