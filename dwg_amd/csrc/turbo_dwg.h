@@ -109,10 +109,11 @@ public:
                                   double* residual = nullptr, int* iters_run = nullptr);
     // GenerateMesh (TurboDWG.cu:808-924): transfer the source normals to the
     // points, one screened evaluation at the points and on the final grid, MC.
-    // d_points: device FP64 (np×3); points_tree (nullable, ownership taken):
-    // their octree, reused as in TurboDWG.cu:869-870.
+    // d_points: device FP64 (np×3); point_normals (nullable): host buffer of
+    // np×3 floats for the points' normals; points_tree (nullable, ownership
+    // taken): their octree, reused as in TurboDWG.cu:869-870.
     void GenerateMesh(const double* d_points, int64_t np, const Grid& final_grid, int64_t* tri_count,
-                      HostMesh* tris, std::vector<float>* point_normals, dwg_octree* points_tree = nullptr);
+                      HostMesh* tris, float* point_normals, dwg_octree* points_tree = nullptr);
     // GenerateMesh(grid) variant of --use_leaves_generate 1 (TurboDWG.cu:768-806).
     void GenerateMeshFromSources(const Grid& final_grid, int64_t* tri_count, HostMesh* tris);
 

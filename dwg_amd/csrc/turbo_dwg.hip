@@ -508,7 +508,7 @@ void TurboDWG::mesh_pass(const Grid& grid, float iso_scale, int64_t* tri_count, 
 }
 
 void TurboDWG::GenerateMesh(const double* d_points, int64_t np, const Grid& final_grid, int64_t* tri_count,
-                            HostMesh* tris, std::vector<float>* point_normals, dwg_octree* points_tree) {
+                            HostMesh* tris, float* point_normals, dwg_octree* points_tree) {
     std::unique_ptr<dwg_octree, void (*)(dwg_octree*)> owned(points_tree, dwg_octree_destroy);
     // normals of the points from their normal_kw nearest sources (LoadNormals)
     DevArray<float> dp(3 * np), dn(3 * np);
@@ -531,10 +531,7 @@ void TurboDWG::GenerateMesh(const double* d_points, int64_t np, const Grid& fina
     SetSources(d_points, np, dn.ptr, grid_, owned.release(), opt_.com_As ? pa.ptr : nullptr);
     UpdateRadii(opt_.final_kw);
     mesh_pass(final_grid, 1.0f, tri_count, tris, true);
-    if (point_normals) {
-        point_normals->resize(3 * np);
-        dn.download(point_normals->data(), 3 * np, ctx_->stream);
-    }
+    if (point_normals) dn.download(point_normals, 3 * np, ctx_->stream);
 }
 
 void TurboDWG::GenerateMeshFromSources(const Grid& final_grid, int64_t* tri_count, HostMesh* tris) {
@@ -719,7 +716,12 @@ void DWGLaunch(dwg_ctx* ctx, const float* pts, const float* nrm, int64_t n, cons
     const auto t2 = Clock::now();
     int64_t ntri = 0;
     HostMesh tris;
+    // the points' final normals on the host: straight into the caller's buffer
+    // when it takes them (pn), else a local copy for the .xyz file
     std::vector<float> point_nrm;
+    float* pn = nullptr;
+    int64_t pn_count = 0;
+    const bool want_xyz = out_base && (opt.output_mode == 2 || opt.output_mode == 3);
     const bool want_tris = (res && res->want_mesh) || (out_base && (opt.output_mode == 1 || opt.output_mode == 3));
     if (opt.use_leaves_generate == 1) {
         double slo[3], shi[3];
@@ -730,29 +732,39 @@ void DWGLaunch(dwg_ctx* ctx, const float* pts, const float* nrm, int64_t n, cons
     } else {
         const Grid fg = voxel_grid_bounds(plo, phi, opt.offset_2, opt.final_reso);
         if (opt.source_mode == 0) {
-            dwg.GenerateMesh(P.ptr, n, fg, &ntri, want_tris ? &tris : nullptr, &point_nrm, points_tree.release());
+            if (res && res->normals) {
+                pn = res->normals;
+            } else if (want_xyz) {
+                point_nrm.resize(3 * n);
+                pn = point_nrm.data();
+            }
+            pn_count = pn ? n : 0;
+            dwg.GenerateMesh(P.ptr, n, fg, &ntri, want_tris ? &tris : nullptr, pn, points_tree.release());
         } else {
             dwg.GenerateMeshFromSources(fg, &ntri, want_tris ? &tris : nullptr);
             dwg.download_normals(point_nrm);
         }
     }
+    if (!pn) {
+        pn = point_nrm.data();
+        pn_count = int64_t(point_nrm.size() / 3);
+    }
     const double t_final = seconds_since(t2);
     if (out_base) {
         if (opt.output_mode == 1 || opt.output_mode == 3) save_binary_stl(std::string(out_base) + ".stl", tris.data, ntri);
-        if (opt.output_mode == 2 || opt.output_mode == 3) {
-            const int64_t np = int64_t(point_nrm.size() / 3);
+        if (want_xyz) {
+            const int64_t np = pn_count;
             DevArray<float> f(3 * np);
             dwg::to_f32(ctx, (opt.use_leaves_generate == 1) ? S : P.ptr, 3 * np, f.ptr);
             std::vector<float> spf(3 * np);
             f.download(spf.data(), 3 * np, stream);
-            save_xyz(std::string(out_base) + ".xyz", spf.data(), point_nrm.data(), np);
+            save_xyz(std::string(out_base) + ".xyz", spf.data(), pn, np);
         }
     }
     if (res) {
         // the points' normals (with --use_leaves_generate 1 the final pass
         // keeps the sources' normals: returned when the sources are the points)
-        if (res->normals && point_nrm.size() == size_t(3 * n))
-            std::copy(point_nrm.begin(), point_nrm.end(), res->normals);
+        if (res->normals && pn != res->normals && pn_count == n) std::copy(pn, pn + 3 * n, res->normals);
         res->tri_count = ntri;
         res->mesh = (res->want_mesh && ntri > 0) ? tris.release() : nullptr;
         res->n_sources = ns;
