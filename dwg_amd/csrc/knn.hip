@@ -1739,18 +1739,17 @@ void knn_query(dwg_ctx* ctx, const dwg_knn_index* I, const float* q, int64_t nq,
     if (!qmask && nq < kSparseMaxQueries && nq * 8 < I->n && I->lvl[0].pyr_levels >= 2) {
         // a few tens of thousands of queries spread thin over a large index
         // (the zero fill's) leave the tiles nearly empty, so staging a tile's
-        // points serves one or two queries: walking the occupancy pyramid per
-        // lane costs less (1.2 vs 2.1 ms for 35k queries over 950k points)
+        // points serves one or two queries: a search per query costs less
+        // (1.2 vs 2.1 ms for 35k queries over 950k points).  One wave per
+        // query, its 64 lanes scanning the points of the coarse grid's
+        // shells, beats one query per lane walking the occupancy pyramid
+        // (0.63 vs 0.79 ms for the zero fill's ~30k queries, skipped points
+        // included): the per-lane walks are few and long, the wave searches
+        // many and short
         ctx->count(base + ":sparse", nq);
-        // with a skip mask, a pyramid of the kept points only: the walk then
-        // never wades through blocks of skipped points (the zero fill's
-        // zero sources come in clusters)
-        const uint8_t* pm = (skip && nq >= kPyrMinQueries) ? masked_pyramid(ctx, I->lvl[0], skip) : nullptr;
-        // the queries in tile order: a wave's 64 walks then start near each
-        // other and take similar paths (0.84 instead of 1.19 ms for the
-        // zero fill's ~30k queries)
+        // the queries in tile order: neighbouring waves read the same cells
         const TileBuckets B = tile_buckets(ctx, I, q, nq, nullptr, nullptr, 1);
-        launch_far(ctx, I, q, nq, B.order, k, skip, out_idx, out_d2, nullptr, false, pm);
+        launch_wave(ctx, I, q, nq, B.order, k, skip, out_idx, out_d2);
         if (sc) {
             hipLaunchKernelGGL(scatter_pairs_kernel, dim3(grid_for(nq * k, 256)), dim3(256), 0, ctx->stream, out_idx, k,
                                nullptr, nq, *sc);

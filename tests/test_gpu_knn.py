@@ -89,9 +89,9 @@ def test_knn_far_queries(ctx):
 
 
 @pytest.mark.parametrize("k", [10, 16])
-def test_knn_sparse_queries_pyramid(ctx, k):
+def test_knn_sparse_queries(ctx, k):
     # 6,000 queries over 60,000 points, as the zero fill queries the non-zero
-    # sources: near and far queries and a skipped cap, all walked per lane
+    # sources: near and far queries and a skipped cap, one wave per query
     P = _noisy_sphere(60_000, 50 + k)
     rng = np.random.default_rng(60 + k)
     Q = np.concatenate([P[rng.integers(0, len(P), 3000)] + rng.normal(0, 0.003, (3000, 3)),
