@@ -95,7 +95,7 @@ def run_once(ctx, P, opts, profile=False, count_terms=False, mlp=None):
 
 
 KERNELS = ["gwn_tree_grid_screened", "gwn_tree_screened", "octree_pack", "octree_ec_scan", "octree_ec", "mc_count",
-           "mc_emit", "knn_query_centroids", "knn_query_zero_fill", "knn_query_radii", "knn_query_transfer",
+           "mc_emit", "mc_fused", "knn_query_centroids", "knn_query_zero_fill", "knn_query_radii", "knn_query_transfer",
            "knn_query_patches", "knn_build", "diffuse_nonzero_index", "diffuse_normalize", "diffuse_zero_fill", "radii",
            "gauss_map", "mlp_init", "mlp_forward"]
 

@@ -33,7 +33,7 @@ EXPORTS = [
     "dwg_octree_destroy", "dwg_octree_info", "dwg_octree_point_order", "dwg_octree_update",
     "dwg_octree_set_point_areas", "dwg_gwn_tree", "dwg_gwn_tree_grad", "dwg_gwn_tree_self", "dwg_gwn_tree_grid",
     "dwg_gwn_tree_terms", "dwg_gwn_tree_grid_terms", "dwg_knn_build", "dwg_knn_destroy", "dwg_knn_query",
-    "dwg_marching_cubes", "dwg_diffuse_normals", "dwg_radii", "dwg_point_areas", "dwg_pca_normals", "dwg_gauss_map", "dwg_transfer_normals",
+    "dwg_marching_cubes", "dwg_marching_cubes_any_order", "dwg_diffuse_normals", "dwg_radii", "dwg_point_areas", "dwg_pca_normals", "dwg_gauss_map", "dwg_transfer_normals",
     "dwg_outlier_filter", "dwg_normal_change", "dwg_reduce_sum", "dwg_bounds", "dwg_options_default", "dwg_launch",
     "dwg_free", "dwg_mlp_num_params", "dwg_mlp_create", "dwg_mlp_load", "dwg_mlp_save", "dwg_mlp_destroy",
     "dwg_mlp_get", "dwg_mlp_set", "dwg_mlp_patches", "dwg_mlp_patches_from", "dwg_mlp_init", "dwg_mlp_forward", "dwg_mlp_loss_grad", "dwg_mlp_adam",
@@ -124,6 +124,7 @@ def load():
     L.dwg_knn_destroy.restype = None
     L.dwg_knn_query.argtypes = [_vp, _vp, _vp, _i64, _int, _vp, _vp, _vp, _vp]
     L.dwg_marching_cubes.argtypes = [_vp, _vp, P(DwgGrid), _f, _i64, _vp, _vp, _vp, P(_i64)]
+    L.dwg_marching_cubes_any_order.argtypes = [_vp, _vp, P(DwgGrid), _f, _i64, _vp, _vp, P(_i64)]
     L.dwg_diffuse_normals.argtypes = [_vp, _vp, _vp, _i64, _vp, _vp, _i64, _int, _int, _int, _vp, P(_i64)]
     L.dwg_radii.argtypes = [_vp, _vp, _vp, _i64, _int, _f, _f, _vp]
     L.dwg_point_areas.argtypes = [_vp, _vp, _vp, _i64, _int, _vp, _vp]

@@ -359,6 +359,26 @@ int dwg_marching_cubes(dwg_ctx* ctx, const float* d_field, const dwg_grid* grid,
     }
 }
 
+int dwg_marching_cubes_any_order(dwg_ctx* ctx, const float* d_field, const dwg_grid* grid, float iso,
+                                 int64_t capacity, float* d_cent, float* d_nrm, int64_t* h_count) {
+    if (!ctx) return DWG_ERR_ARG;
+    try {
+        DWG_REQUIRE(d_field && grid, "null field/grid");
+        set_device(ctx);
+        int64_t t = dwg::marching_cubes_any_order(ctx, d_field, grid, iso, capacity, d_cent, d_nrm);
+        if (h_count) *h_count = t;
+        ctx->err.clear();
+        return DWG_OK;
+    } catch (const dwg::Error& e) {
+        ctx->err = e.what();
+        if (e.code == DWG_ERR_CAPACITY && h_count) *h_count = e.value;
+        return e.code;
+    } catch (const std::exception& e) {
+        ctx->err = e.what();
+        return DWG_ERR_HIP;
+    }
+}
+
 int dwg_diffuse_normals(dwg_ctx* ctx, const dwg_knn_index* src_index, const float* d_src, int64_t n_src,
                         const float* d_cent, const float* d_tri_nrm, int64_t ntri, int k, int zero_k,
                         int zero_threshold, float* d_nrm, int64_t* h_nzero) {

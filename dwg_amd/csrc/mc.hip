@@ -234,16 +234,106 @@ __global__ __launch_bounds__(kMcThreads) void mc_emit_kernel(McArgs A, McBricks 
     }
 }
 
+// One pass for callers that take the triangles in any order (the
+// diffusion's centroids and face normals, whose sums do not depend on it):
+// the count pass's wave per brick also writes the brick's triangles, at a
+// range taken with one atomic add once the brick's count is known, each
+// lane reloading the corners of the cubes of its column that have
+// triangles.  Triangles come brick by brick in the order the waves take
+// their ranges, each brick's by column and then in z and table order; their
+// values are those of the ordered passes.  A brick writes only when its
+// whole range fits in capacity; the count is complete either way.
+__global__ __launch_bounds__(kCountThreads) void mc_fused_kernel(McArgs A, McBricks B,
+                                                                  unsigned long long* __restrict__ ntri,
+                                                                  int64_t capacity, float* __restrict__ cent,
+                                                                  float* __restrict__ nrm) {
+    __shared__ uint8_t s_ntri[256];
+#pragma unroll
+    for (int c = threadIdx.x; c < 256; c += kCountThreads) s_ntri[c] = uint8_t(c_mc_rows[c] >> 60);
+    __syncthreads();
+    const int64_t b = brick_of_block();
+    const int x = int(b % B.nbx) * kBx + (threadIdx.x & (kBx - 1));
+    const int y = int((b / B.nbx) % B.nby) * kBy + int(threadIdx.x) / kBx;
+    const int z0 = int(b / (int64_t(B.nbx) * B.nby)) * kBz;
+    int cnt = 0;
+    uint32_t has = 0;  // bit j: cube z0 + j of the column has triangles
+    if (x < A.rx - 1 && y < A.ry - 1 && z0 < A.rz - 1) {
+        float lo[4], hi[4], v[8];
+        load_face(A, x, y, z0, lo);
+#pragma unroll
+        for (int j = 0; j < kBz; ++j) {
+            if (z0 + j < A.rz - 1) {
+                load_face(A, x, y, z0 + j + 1, hi);
+                const int n = s_ntri[cube_case(A, lo, hi, v)];
+                cnt += n;
+                has |= (n > 0 ? 1u : 0u) << j;
+#pragma unroll
+                for (int k = 0; k < 4; ++k) lo[k] = hi[k];
+            }
+        }
+    }
+    const int incl = wave_inclusive_sum(cnt);
+    const int total = __shfl(incl, 63, 64);
+    if (total == 0) return;
+    uint32_t base_lo = 0, base_hi = 0;
+    if (threadIdx.x == 63) {
+        const unsigned long long at = atomicAdd(ntri, (unsigned long long)total);
+        base_lo = uint32_t(at);
+        base_hi = uint32_t(at >> 32);
+    }
+    const int64_t base = int64_t((uint64_t(uint32_t(__shfl(int(base_hi), 63, 64))) << 32) |
+                                 uint32_t(__shfl(int(base_lo), 63, 64)));
+    if ((!cent && !nrm) || base + total > capacity) return;
+    int64_t o = base + (incl - cnt);
+    while (has) {
+        const int j = __builtin_ctz(has);
+        has &= has - 1u;
+        float lo[4], hi[4], v[8];
+        load_face(A, x, y, z0 + j, lo);
+        load_face(A, x, y, z0 + j + 1, hi);
+        const uint64_t row = c_mc_rows[cube_case(A, lo, hi, v)];
+        const int nt = int(row >> 60);
+        emit_cube(A, x, y, z0 + j, v, row, nt, o, nullptr, cent, nrm);
+        o += nt;
+    }
+}
+
+static McArgs mc_args(const float* field, const dwg_grid* g, float iso, McBricks& B, int64_t& tiles) {
+    DWG_REQUIRE(g && g->res[0] >= 2 && g->res[1] >= 2 && g->res[2] >= 2, "grid needs at least 2 samples per axis");
+    McArgs A{field, g->res[0], g->res[1], g->res[2], 0, g->origin[0], g->origin[1], g->origin[2], g->h, iso};
+    A.ncubes = int64_t(A.rx - 1) * (A.ry - 1) * (A.rz - 1);
+    B = McBricks{(A.rx - 1 + kBx - 1) / kBx, (A.ry - 1 + kBy - 1) / kBy, (A.rz - 1 + kBz - 1) / kBz};
+    tiles = int64_t(B.nbx) * B.nby * B.nbz;
+    DWG_REQUIRE(tiles < (int64_t(1) << 31), "grid too large for the marching-cubes launch");
+    return A;
+}
+
+int64_t marching_cubes_any_order(dwg_ctx* ctx, const float* field, const dwg_grid* g, float iso, int64_t capacity,
+                                 float* cent, float* nrm) {
+    DWG_REQUIRE(capacity >= 0, "negative capacity");
+    McBricks B;
+    int64_t tiles;
+    const McArgs A = mc_args(field, g, iso, B, tiles);
+    ensure_table(ctx);
+    auto* n = static_cast<unsigned long long*>(ctx->buf("mc_fused_count", 64));
+    fill_u32(ctx, n, 0u, 2);
+    {
+        ProfileScope ps(ctx, "mc_fused");
+        hipLaunchKernelGGL(mc_fused_kernel, dim3(unsigned(tiles)), dim3(kCountThreads), 0, ctx->stream, A, B, n,
+                           capacity, cent, nrm);
+        check_launch();
+    }
+    const int64_t total = int64_t(read_scalar<unsigned long long>(ctx, n));
+    if (total > capacity) throw Error(DWG_ERR_CAPACITY, "marching cubes output exceeds capacity", total);
+    return total;
+}
 
 int64_t marching_cubes(dwg_ctx* ctx, const float* field, const dwg_grid* g, float iso, int64_t capacity, float* tris,
                        float* cent, float* nrm) {
-    DWG_REQUIRE(g && g->res[0] >= 2 && g->res[1] >= 2 && g->res[2] >= 2, "grid needs at least 2 samples per axis");
+    McBricks B;
+    int64_t tiles;
+    const McArgs A = mc_args(field, g, iso, B, tiles);
     ensure_table(ctx);
-    McArgs A{field, g->res[0], g->res[1], g->res[2], 0, g->origin[0], g->origin[1], g->origin[2], g->h, iso};
-    A.ncubes = int64_t(A.rx - 1) * (A.ry - 1) * (A.rz - 1);
-    const McBricks B{(A.rx - 1 + kBx - 1) / kBx, (A.ry - 1 + kBy - 1) / kBy, (A.rz - 1 + kBz - 1) / kBz};
-    const int64_t tiles = int64_t(B.nbx) * B.nby * B.nbz;
-    DWG_REQUIRE(tiles < (int64_t(1) << 31), "grid too large for the marching-cubes launch");
     // brick counts (scanned in place: offsets, the total at [tiles]) and, at
     // [tiles + 1], the number of bricks with triangles
     int64_t* offs = static_cast<int64_t*>(ctx->buf("mc_tiles", sizeof(int64_t) * (tiles + 2)));

@@ -376,12 +376,13 @@ double TurboDWG::TurboScreenedDWGLaunch(int total_iters, double* w_mean, int64_t
         int64_t ntri = 0;
         cent_.resize(3 * capacity);
         tri_nrm_.resize(3 * capacity);
-        int rc = dwg_marching_cubes(ctx_, w_grid_.ptr, &g, iso, capacity, nullptr, cent_.ptr, tri_nrm_.ptr, &ntri);
+        // the diffusion's sums do not depend on the triangles' order: one pass
+        int rc = dwg_marching_cubes_any_order(ctx_, w_grid_.ptr, &g, iso, capacity, cent_.ptr, tri_nrm_.ptr, &ntri);
         if (rc == DWG_ERR_CAPACITY) {
             capacity = ntri + ntri / 4 + 1024;
             cent_.resize(3 * capacity);
             tri_nrm_.resize(3 * capacity);
-            rc = dwg_marching_cubes(ctx_, w_grid_.ptr, &g, iso, capacity, nullptr, cent_.ptr, tri_nrm_.ptr, &ntri);
+            rc = dwg_marching_cubes_any_order(ctx_, w_grid_.ptr, &g, iso, capacity, cent_.ptr, tri_nrm_.ptr, &ntri);
         }
         check(rc, "marching cubes");
         if (iter_tris) iter_tris[it] = ntri;

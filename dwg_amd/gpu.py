@@ -294,6 +294,24 @@ def marching_cubes(ctx: Context, field, grid: DwgGrid, iso, tris=False):
     return t, c, n
 
 
+def marching_cubes_any_order(ctx: Context, field, grid: DwgGrid, iso, capacity=1 << 16):
+    """→ (centroids, normals) as device tensors, one pass, in no fixed order.
+
+    Starts with `capacity` triangles and retries once at the reported count."""
+    field = ctx.f32(field)
+    cnt = ctypes.c_int64()
+    for cap in (int(capacity), None):
+        cap = cnt.value if cap is None else cap
+        c = torch.empty((max(cap, 1), 3), device=ctx.device, dtype=torch.float32)
+        n = torch.empty((max(cap, 1), 3), device=ctx.device, dtype=torch.float32)
+        rc = ctx.L.dwg_marching_cubes_any_order(ctx.h, _ptr(field), ctypes.byref(grid), float(iso), cap, _ptr(c),
+                                                _ptr(n), ctypes.byref(cnt))
+        if rc != _lib.DWG_ERR_CAPACITY:
+            ctx.check(rc)
+            return c[:cnt.value], n[:cnt.value]
+    ctx.check(rc)
+
+
 def diffuse_normals(ctx: Context, index: KnnIndex, cent, tri_nrm, nrm, k=15, zero_k=10, zero_threshold=10):
     """In-place update of nrm (n×3 float32 device tensor); returns the zero count."""
     cent, tri_nrm = ctx.f32(cent), ctx.f32(tri_nrm)

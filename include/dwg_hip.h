@@ -192,6 +192,14 @@ int dwg_knn_query(dwg_ctx* ctx, const dwg_knn_index* index, const float* d_q, in
 int dwg_marching_cubes(dwg_ctx* ctx, const float* d_field, const dwg_grid* grid, float iso,
                        int64_t capacity, float* d_tris, float* d_cent, float* d_nrm,
                        int64_t* h_count);
+/* No counterpart in the reference: the same triangles' centroids and
+ * normals (values as dwg_marching_cubes gives them) in one pass and in no
+ * fixed order, for callers whose use does not depend on the order (the
+ * normal diffusion's fixed-point sums).  *h_count receives T; if
+ * T > capacity, DWG_ERR_CAPACITY is returned and the buffers hold an
+ * unspecified part of the triangles. */
+int dwg_marching_cubes_any_order(dwg_ctx* ctx, const float* d_field, const dwg_grid* grid, float iso,
+                                 int64_t capacity, float* d_cent, float* d_nrm, int64_t* h_count);
 
 /* ---- normal diffusion ---------------------------------------------------- */
 /* One diffusion update of source normals from triangle face normals

@@ -215,6 +215,39 @@ def test_marching_cubes_matches_oracle(ctx):
         assert G32.shape[0] == f.shape[0]
 
 
+def _rows_sorted(c, n):
+    """(centroid, normal) rows as one array, sorted row by row on their bit patterns."""
+    rows = np.concatenate([c, n], axis=1).astype(np.float32).view(np.uint32)
+    return rows[np.lexsort(rows.T[::-1])]
+
+
+@pytest.mark.parametrize("res,iso", [((30, 27, 25), 0.5), ((70, 41, 37), 0.75), ((9, 9, 40), 0.5)])
+def test_marching_cubes_any_order_same_triangles(ctx, res, iso):
+    # the one-pass variant writes the ordered passes' triangles bit for bit,
+    # in another order; the grids cross brick edges in x, y and z (bricks of
+    # 8 x 8 x 16 cubes), a first call with a short capacity reports the count
+    res = np.array(res, np.int32)
+    h = 2.2 / float(res.max())
+    origin = np.array([0.1, 0.0, -0.05]) - 0.5 * h * (res - 1)  # centred on the surface below
+    G = O.grid_points(origin, h, res)
+    f = (np.linalg.norm(G - [0.1, 0.0, -0.05], axis=1) + 0.2 * np.sin(3 * G[:, 0]) * np.cos(2 * G[:, 1])).astype(np.float32)
+    grid = gpu.make_grid(origin, h, res)
+    ft = torch.from_numpy(f).cuda()
+    _, c0, n0 = gpu.marching_cubes(ctx, ft, grid, iso)
+    T = c0.shape[0]
+    assert T > 100
+    c1, n1 = gpu.marching_cubes_any_order(ctx, ft, grid, iso, capacity=T // 3)
+    assert c1.shape[0] == T
+    assert np.array_equal(_rows_sorted(c0.cpu().numpy(), n0.cpu().numpy()),
+                          _rows_sorted(c1.cpu().numpy(), n1.cpu().numpy()))
+    # exact capacity, then no triangles at all
+    c2, n2 = gpu.marching_cubes_any_order(ctx, ft, grid, iso, capacity=T)
+    assert np.array_equal(_rows_sorted(c0.cpu().numpy(), n0.cpu().numpy()),
+                          _rows_sorted(c2.cpu().numpy(), n2.cpu().numpy()))
+    c3, _ = gpu.marching_cubes_any_order(ctx, ft, grid, float(f.max()) + 1.0, capacity=16)
+    assert c3.shape[0] == 0
+
+
 def test_radii_and_transfer(ctx):
     P, _, _ = _sphere(20000, 16)
     P = P * 0.5
