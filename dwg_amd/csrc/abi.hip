@@ -79,6 +79,7 @@ void dwg_ctx_destroy(dwg_ctx* ctx) {
         }
     ctx->scratch.clear();
     if (ctx->pinned) (void)hipHostFree(ctx->pinned);
+    if (ctx->readback) (void)hipEventDestroy(ctx->readback);
     delete ctx;
 }
 

@@ -77,6 +77,8 @@ struct dwg_ctx {
     std::map<std::string, dwg::DevBuf> scratch;
     // pinned host staging for small read-backs
     void* pinned = nullptr;
+    // marks the copy of an outstanding asynchronous read-back (reduce_sum_f32_async)
+    hipEvent_t readback = nullptr;
 
     void* buf(const std::string& name, size_t bytes) { return scratch[name].ensure(bytes); }
     size_t buf_size(const std::string& name) const {

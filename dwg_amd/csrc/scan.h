@@ -24,5 +24,11 @@ void radix_sort_pairs(dwg_ctx* ctx, const uint64_t* kin, const int* vin, uint64_
 
 // FP64 sum of a float array (synchronises, returns the host value).
 double reduce_sum_f32(dwg_ctx* ctx, const float* d, int64_t n);
+// The same sum without waiting: queued on the stream with its copy into
+// pinned host memory; reduce_sum_f32_wait returns it once that copy is done
+// (work queued in between keeps the GPU busy while the host waits).  One
+// sum may be outstanding per context.
+void reduce_sum_f32_async(dwg_ctx* ctx, const float* d, int64_t n);
+double reduce_sum_f32_wait(dwg_ctx* ctx);
 
 }  // namespace dwg

@@ -2,6 +2,8 @@
 // reference's thrust::reduce / CUB calls, TurboDWG.cu:541-542).
 #include "scan.h"
 
+#include <cstring>
+
 namespace dwg {
 
 namespace {
@@ -259,14 +261,40 @@ void fill_u32(dwg_ctx* ctx, void* d, uint32_t value, int64_t count) {
     check_launch();
 }
 
-double reduce_sum_f32(dwg_ctx* ctx, const float* d, int64_t n) {
+// the partial sums and their total (at partial[blocks]) queued on the stream
+static const double* reduce_sum_f32_queue(dwg_ctx* ctx, const float* d, int64_t n) {
     const int blocks = 1024;
     double* partial = static_cast<double*>(ctx->buf("reduce_partial", sizeof(double) * (blocks + 1)));
     hipLaunchKernelGGL(reduce_f32_f64, dim3(blocks), dim3(256), 0, ctx->stream, d, n, partial);
     check_launch();
     hipLaunchKernelGGL(reduce_final_f64, dim3(1), dim3(256), 0, ctx->stream, partial, blocks, partial + blocks);
     check_launch();
-    return read_scalar<double>(ctx, partial + blocks);
+    return partial + blocks;
+}
+
+double reduce_sum_f32(dwg_ctx* ctx, const float* d, int64_t n) {
+    return read_scalar<double>(ctx, reduce_sum_f32_queue(ctx, d, n));
+}
+
+// the outstanding sum's place in the context's pinned page (read_scalar
+// uses its start)
+constexpr size_t kAsyncSumSlot = 2048;
+
+void reduce_sum_f32_async(dwg_ctx* ctx, const float* d, int64_t n) {
+    const double* total = reduce_sum_f32_queue(ctx, d, n);
+    if (!ctx->pinned) DWG_HIP_CHECK(hipHostMalloc(&ctx->pinned, 4096, hipHostMallocDefault));
+    if (!ctx->readback) DWG_HIP_CHECK(hipEventCreateWithFlags(&ctx->readback, hipEventDisableTiming));
+    DWG_HIP_CHECK(hipMemcpyAsync(static_cast<char*>(ctx->pinned) + kAsyncSumSlot, total, sizeof(double),
+                                 hipMemcpyDeviceToHost, ctx->stream));
+    DWG_HIP_CHECK(hipEventRecord(ctx->readback, ctx->stream));
+}
+
+double reduce_sum_f32_wait(dwg_ctx* ctx) {
+    DWG_REQUIRE(ctx->readback, "no sum outstanding");
+    DWG_HIP_CHECK(hipEventSynchronize(ctx->readback));
+    double v;
+    std::memcpy(&v, static_cast<const char*>(ctx->pinned) + kAsyncSumSlot, sizeof(v));
+    return v;
 }
 
 }  // namespace dwg

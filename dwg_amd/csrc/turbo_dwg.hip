@@ -24,6 +24,7 @@
 #include "gwn.h"
 #include "knn.h"
 #include "octree.h"
+#include "scan.h"
 
 namespace dwg_host {
 
@@ -359,17 +360,20 @@ double TurboDWG::TurboScreenedDWGLaunch(int total_iters, double* w_mean, int64_t
         const float sigma = sigma_schedule(it, opt_.sigma, opt_.sigma_warm_iters);
         const float* rad = opt_.use_radii ? radii_.ptr : nullptr;
         check(dwg_octree_update(ctx_, tree_, nrm_.ptr, area_.ptr, rad), "moments");
+        // the sources' mean (the iso value) is read back while the grid is
+        // evaluated, so marching cubes queues right behind the grid
         if (opt_.beta > 0) {
             check(dwg_gwn_tree_self(ctx_, tree_, opt_.beta, sigma, flags, w_src_.ptr), "gwn sources");
+            dwg::reduce_sum_f32_async(ctx_, w_src_.ptr, n_);
             check(dwg_gwn_tree_grid(ctx_, tree_, &g, opt_.beta, sigma, flags, w_grid_.ptr), "gwn grid");
         } else {
             check(dwg_gwn_direct(ctx_, pts_.ptr, nrm_.ptr, area_.ptr, rad, n_, pts_.ptr, n_, sigma, flags, w_src_.ptr,
                                  nullptr), "gwn sources");
+            dwg::reduce_sum_f32_async(ctx_, w_src_.ptr, n_);
             check(dwg_gwn_direct(ctx_, pts_.ptr, nrm_.ptr, area_.ptr, rad, n_, grid_pts.ptr, grid_.size(), sigma, flags,
                                  w_grid_.ptr, nullptr), "gwn grid");
         }
-        double sum = 0;
-        check(dwg_reduce_sum(ctx_, w_src_.ptr, n_, &sum), "reduce");
+        const double sum = dwg::reduce_sum_f32_wait(ctx_);
         float iso = float(sum / double(n_));
         if (w_mean) w_mean[it] = double(iso);
         if (it % 5 == 0 && it != 0) iso *= 0.5f;  // TurboDWG.cu:547-550
@@ -476,9 +480,9 @@ void TurboDWG::mesh_pass(const Grid& grid, float iso_scale, int64_t* tri_count, 
     }
     const float beta = opt_.beta > 0 ? opt_.beta : 1e30f;  // β → ∞ takes every point term
     check(dwg_gwn_tree_self(ctx_, tree_, beta, opt_.sigma, flags, w_src_.ptr), "gwn points");
+    dwg::reduce_sum_f32_async(ctx_, w_src_.ptr, n_);  // read back while the grid is evaluated
     check(dwg_gwn_tree_grid(ctx_, tree_, &g, beta, opt_.sigma, flags, wg), "gwn final grid");
-    double sum = 0;
-    check(dwg_reduce_sum(ctx_, w_src_.ptr, n_, &sum), "reduce");
+    const double sum = dwg::reduce_sum_f32_wait(ctx_);
     const float iso = float(sum / double(n_)) * iso_scale;
     int64_t ntri = 0;
     if (!tris) {
