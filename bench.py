@@ -187,13 +187,17 @@ def hbm_rates(ktimes):
     launch that the FETCH_SIZE/WRITE_SIZE counters gave on this workload
     (committed summaries) over this run's mean launch times (HIP events)."""
     out = {}
-    for name, path, key in [("gwn_tree_grid_screened", TRAFFIC_SUMMARY, "traffic_bytes_per_launch"),
-                            ("mc_count", MC_TRAFFIC_SUMMARY, "mc_count_bytes_per_launch"),
-                            ("mc_emit", MC_TRAFFIC_SUMMARY, "mc_emit_bytes_per_launch")]:
+    # the iterations run the one-pass marching cubes (mc_fused); the ordered
+    # count and emit passes run in the final pass (recorded as "<kernel>:final")
+    for name, tname, path, key in [
+            ("gwn_tree_grid_screened", "gwn_tree_grid_screened", TRAFFIC_SUMMARY, "traffic_bytes_per_launch"),
+            ("mc_fused", "mc_fused", MC_TRAFFIC_SUMMARY, "mc_fused_bytes_per_launch"),
+            ("mc_count", "mc_count:final", MC_TRAFFIC_SUMMARY, "mc_count_bytes_per_launch"),
+            ("mc_emit", "mc_emit:final", MC_TRAFFIC_SUMMARY, "mc_emit_bytes_per_launch")]:
         b = measured_traffic(path, key)
-        if b is None or name not in ktimes:
+        if b is None or tname not in ktimes:
             continue
-        ms = ktimes[name][0] / ktimes[name][1]
+        ms = ktimes[tname][0] / ktimes[tname][1]
         gbs = b / (ms * 1e-3) / 1e9
         out[name] = {"bytes_per_launch": b, "avg_launch_ms": round(ms, 3), "gb_per_s": round(gbs, 1),
                      "frac_of_8000": round(gbs / PEAK_HBM, 4)}
