@@ -300,8 +300,8 @@ def marching_cubes_any_order(ctx: Context, field, grid: DwgGrid, iso, capacity=1
     Starts with `capacity` triangles and retries once at the reported count."""
     field = ctx.f32(field)
     cnt = ctypes.c_int64()
-    for cap in (int(capacity), None):
-        cap = cnt.value if cap is None else cap
+    cap = int(capacity)
+    for _ in range(2):
         c = torch.empty((max(cap, 1), 3), device=ctx.device, dtype=torch.float32)
         n = torch.empty((max(cap, 1), 3), device=ctx.device, dtype=torch.float32)
         rc = ctx.L.dwg_marching_cubes_any_order(ctx.h, _ptr(field), ctypes.byref(grid), float(iso), cap, _ptr(c),
@@ -309,6 +309,7 @@ def marching_cubes_any_order(ctx: Context, field, grid: DwgGrid, iso, capacity=1
         if rc != _lib.DWG_ERR_CAPACITY:
             ctx.check(rc)
             return c[:cnt.value], n[:cnt.value]
+        cap = cnt.value  # short: the call reported the count
     ctx.check(rc)
 
 
