@@ -112,6 +112,20 @@ void diffuse_normals(dwg_ctx* ctx, const dwg_knn_index* I, const float* src, int
         hipLaunchKernelGGL(normalize_kernel, dim3(grid_for(n, 256)), dim3(256), 0, ctx->stream, acc, n, nrm, zero, nz);
         check_launch();
     }
+    // the zero sources' positions and indices, compacted (room for n): queued
+    // before their count is read back, as the zero fill nearly always runs
+    int64_t* slot = static_cast<int64_t*>(ctx->buf("diff_nz_slot", sizeof(int64_t) * (n + 1)));
+    float* zq = static_cast<float*>(ctx->buf("diff_z_pts", sizeof(float) * 3 * n));
+    int32_t* zlist = static_cast<int32_t*>(ctx->buf("diff_z_list", sizeof(int32_t) * n));
+    {
+        ProfileScope ps(ctx, "diffuse_nonzero_index");
+        hipLaunchKernelGGL(nonzero_flags_kernel, dim3(grid_for(n + 1, 256)), dim3(256), 0, ctx->stream, zero, n, slot);
+        check_launch();
+        exclusive_scan<int64_t>(ctx, slot, n);
+        hipLaunchKernelGGL(zero_gather_kernel, dim3(grid_for(n, 256)), dim3(256), 0, ctx->stream, slot, src, n, zq,
+                           zlist);
+        check_launch();
+    }
     const int64_t nzero = int64_t(read_scalar<unsigned long long>(ctx, nz));
     if (h_nzero) *h_nzero = nzero;
     if (nzero > zero_threshold && nzero < n) {
@@ -119,19 +133,6 @@ void diffuse_normals(dwg_ctx* ctx, const dwg_knn_index* I, const float* src, int
         // skipped (knn_query then walks a pyramid of the non-zero sources
         // only, as the zero sources come in clusters); results are source
         // indices, ties broken by source index as over the non-zero set
-        int64_t* slot = static_cast<int64_t*>(ctx->buf("diff_nz_slot", sizeof(int64_t) * (n + 1)));
-        float* zq = static_cast<float*>(ctx->buf("diff_z_pts", sizeof(float) * 3 * nzero));
-        int32_t* zlist = static_cast<int32_t*>(ctx->buf("diff_z_list", sizeof(int32_t) * nzero));
-        {
-            ProfileScope ps(ctx, "diffuse_nonzero_index");
-            hipLaunchKernelGGL(nonzero_flags_kernel, dim3(grid_for(n + 1, 256)), dim3(256), 0, ctx->stream, zero, n,
-                               slot);
-            check_launch();
-            exclusive_scan<int64_t>(ctx, slot, n);
-            hipLaunchKernelGGL(zero_gather_kernel, dim3(grid_for(n, 256)), dim3(256), 0, ctx->stream, slot, src, n, zq,
-                               zlist);
-            check_launch();
-        }
         int32_t* zknn = static_cast<int32_t*>(ctx->buf("diff_zknn", sizeof(int32_t) * nzero * zero_k));
         knn_query(ctx, I, zq, nzero, zero_k, zero, nullptr, zknn, nullptr, "knn_query_zero_fill");
         ProfileScope ps(ctx, "diffuse_zero_fill");

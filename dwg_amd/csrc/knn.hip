@@ -1599,7 +1599,7 @@ struct TileBuckets {
 };
 
 static TileBuckets tile_buckets(dwg_ctx* ctx, const dwg_knn_index* I, const float* q, int64_t nq, const int* qlist,
-                                const uint8_t* qmask, int sub) {
+                                const uint8_t* qmask, int sub, bool want_list = true) {
     const KnnGridArgs& G = I->lvl[0].G;
     TileBuckets B;
     TileArgs& T = B.T;
@@ -1625,6 +1625,13 @@ static TileBuckets tile_buckets(dwg_ctx* ctx, const dwg_knn_index* I, const floa
     hipLaunchKernelGGL(tile_scatter_kernel, dim3(grid_for(nq, 256)), dim3(256), 0, ctx->stream, tile_of, nq, qlist,
                        cursor, order);
     check_launch();
+    B.start = start;
+    B.order = order;
+    if (!want_list) {  // the caller takes the queries in tile order only: no list, no host read
+        B.nlist = 0;
+        B.list = nullptr;
+        return B;
+    }
     hipLaunchKernelGGL(tile_flags_kernel, dim3(grid_for(ntiles + 1, 256)), dim3(256), 0, ctx->stream, start, ntiles,
                        sub, flags);
     check_launch();
@@ -1632,8 +1639,6 @@ static TileBuckets tile_buckets(dwg_ctx* ctx, const dwg_knn_index* I, const floa
     hipLaunchKernelGGL(tile_list_kernel, dim3(grid_for(ntiles, 256)), dim3(256), 0, ctx->stream, flags, ntiles, list);
     check_launch();
     B.nlist = read_scalar<int64_t>(ctx, flags + ntiles);
-    B.start = start;
-    B.order = order;
     B.list = list;
     return B;
 }
@@ -1748,7 +1753,7 @@ void knn_query(dwg_ctx* ctx, const dwg_knn_index* I, const float* q, int64_t nq,
         // many and short
         ctx->count(base + ":sparse", nq);
         // the queries in tile order: neighbouring waves read the same cells
-        const TileBuckets B = tile_buckets(ctx, I, q, nq, nullptr, nullptr, 1);
+        const TileBuckets B = tile_buckets(ctx, I, q, nq, nullptr, nullptr, 1, false);
         launch_wave(ctx, I, q, nq, B.order, k, skip, out_idx, out_d2);
         if (sc) {
             hipLaunchKernelGGL(scatter_pairs_kernel, dim3(grid_for(nq * k, 256)), dim3(256), 0, ctx->stream, out_idx, k,
