@@ -1387,9 +1387,17 @@ __global__ __launch_bounds__(kPwWaves * 64) void knn_pyramid_wave_kernel(KnnGrid
         s_off[threadIdx.x] = Y.off[threadIdx.x];
     }
     __syncthreads();
+    // waves take the 64-query chunks alternately from the front and the back
+    // of the list (wave w: chunk w/2 if w is even, else the (w/2)-th from
+    // the end).  The longest walks are not known before they run and come
+    // in runs along the list; started from both ends, fewer of them start
+    // late and end after everything else (far walk 1.65 → 1.54 ms per
+    // learned iteration)
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int64_t t = (int64_t(blockIdx.x) * kPwWaves + wave) * 64 + lane;
-    const bool valid = t < nq;
+    const int64_t nchunks = (nq + 63) / 64, w = int64_t(blockIdx.x) * kPwWaves + wave;
+    const int64_t chunk = (w & 1) ? nchunks - 1 - (w >> 1) : (w >> 1);
+    const int64_t t = chunk * 64 + lane;
+    const bool valid = w < nchunks && t < nq;
     const int64_t i = valid ? (qlist ? int64_t(qlist[t]) : t) : 0;
     const float qx = valid ? q[3 * i] : 0.f, qy = valid ? q[3 * i + 1] : 0.f, qz = valid ? q[3 * i + 2] : 0.f;
     uint64_t best[K];
