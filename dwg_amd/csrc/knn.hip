@@ -1099,8 +1099,13 @@ __global__ __launch_bounds__(256) void knn_wave_kernel(KnnGridArgs G, const int6
                                                        const uint8_t* __restrict__ skip, int32_t* __restrict__ out_idx,
                                                        float* __restrict__ out_d2) {
     const int lane = threadIdx.x & 63;
-    const int64_t t = int64_t(blockIdx.x) * (blockDim.x >> 6) + (threadIdx.x >> 6);
-    if (t >= nq) return;  // uniform per wave
+    // wave w takes query w/2 of the list if w is even, else the (w/2)-th
+    // from the end: the long searches, which come in runs along the list,
+    // start early from either end (zero fill 0.61 → 0.57 ms per iteration;
+    // the tiled passes lose L2 reuse that way and keep their order)
+    const int64_t w = int64_t(blockIdx.x) * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    if (w >= nq) return;  // uniform per wave
+    const int64_t t = (w & 1) ? nq - 1 - (w >> 1) : (w >> 1);
     const int64_t i = qlist ? int64_t(qlist[t]) : t;
     const float qx = q[3 * i], qy = q[3 * i + 1], qz = q[3 * i + 2];
     uint64_t best[K];
